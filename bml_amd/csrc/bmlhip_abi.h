@@ -1,0 +1,105 @@
+/*
+ * Thin C ABI between the C host layer (ellpack_hip.c, ellblock_hip.c,
+ * sp2_hip.c) and the HIP sources under hip/.  Plain pointers and sizes only;
+ * device pointers are marked d_.  Every function returns 0 on success or a
+ * BML_HIP_ERR_* code, with a message available from bmlhip_errmsg().
+ */
+#ifndef BMLHIP_ABI_H
+#define BMLHIP_ABI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status block resident in device memory, written by kernels, read back by
+ * bmlhip_status_fetch() (one per process). */
+typedef struct bmlhip_status {
+    int m_overflow_rows;     /* rows that needed more slots than M */
+    int first_m_overflow;    /* lowest such row (or INT_MAX) */
+    int table_overflow_rows; /* rows handed to the global-table pass */
+    int max_nnz;             /* largest row written by the last op */
+    int max_touched;         /* largest pre-threshold row of the last product */
+    int cursor;              /* work-queue cursor of the global-table pass */
+    long long total_nnz;     /* entries written by the last op */
+    double scalars[4];       /* trace X, trace X^2 / norm / gershgorin */
+} bmlhip_status;
+
+/* ---- runtime (hip/runtime.hip) ----------------------------------------- */
+int bmlhip_init(int device);
+void bmlhip_shutdown(void);
+const char *bmlhip_errmsg(void);
+void bmlhip_set_errmsg(const char *msg);
+int bmlhip_malloc(void **d_ptr, size_t bytes);
+void bmlhip_free(void *d_ptr);
+int bmlhip_h2d(void *d_dst, const void *src, size_t bytes);
+int bmlhip_d2h(void *dst, const void *d_src, size_t bytes);
+int bmlhip_d2d(void *d_dst, const void *d_src, size_t bytes);
+int bmlhip_zero(void *d_ptr, size_t bytes);
+int bmlhip_sync(void);
+void *bmlhip_stream(void);
+/* Library-owned scratch, grown on demand; slot selects one of a few
+ * independent buffers.  The pointer stays valid until the next call with
+ * the same slot and a larger size. */
+void *bmlhip_scratch(int slot, size_t bytes);
+bmlhip_status *bmlhip_status_dev(void);
+/* reset the per-operation fields (not the overflow fields) */
+int bmlhip_status_reset(void);
+/* reset everything, overflow fields included */
+int bmlhip_status_clear_errors(void);
+int bmlhip_status_fetch(bmlhip_status *host);
+
+/* ---- ELLPACK kernels (hip/ellpack.hip) ---------------------------------- */
+
+/* C = A*B, thresholded (diagonal kept).  kind 0 = x2: per-row traces are
+ * reduced into status->scalars[0..1] and the kept count is checked against
+ * MC; kind 1 = AB: no traces, the touched count is checked against MC (as
+ * the reference's multiply_AB does).  group = lanes per row (8..64),
+ * table = hash slots per row in LDS (power of two, >= 64). */
+typedef struct bmlhip_spgemm_args {
+    int prec, kind, N;
+    int MA, MB, MC;
+    const void *d_Av;
+    const int *d_Ai, *d_An;
+    const void *d_Bv;
+    const int *d_Bi, *d_Bn;
+    void *d_Cv;
+    int *d_Ci, *d_Cn;
+    double threshold;
+    int group, table;
+} bmlhip_spgemm_args;
+int bmlhip_spgemm(const bmlhip_spgemm_args *a);
+
+/* A <- alpha*A + beta*B (mode 0), or that plus the norm over the union of
+ * (A - B): mode 1 Re((A-B)^2) as bml_add_norm, mode 2 |A - B|^2; the norm
+ * goes to status->scalars[2].  Entries kept iff |v| > threshold. */
+int bmlhip_add(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
+               int MB, const void *d_Bv, const int *d_Bi, const int *d_Bn,
+               double alpha, double beta, double threshold, int mode);
+/* A <- A + beta*I then threshold (add_identity); scale: values *= s first
+ * (s_bits points to one element of the precision; NULL = no scaling). */
+int bmlhip_scale_add_identity(int prec, int N, int M, void *d_Av, int *d_Ai,
+                              int *d_An, const void *s_bits, double beta,
+                              double threshold, int do_identity);
+int bmlhip_threshold(int prec, int N, int M, void *d_Av, int *d_Ai, int *d_An,
+                     double threshold);
+/* trace -> status->scalars[0] */
+int bmlhip_trace(int prec, int N, int M, const void *d_Av, const int *d_Ai,
+                 const int *d_An);
+/* gershgorin -> status->scalars[0] (min), [1] (max) */
+int bmlhip_gershgorin(int prec, int N, int M, const void *d_Av, const int *d_Ai,
+                      const int *d_An);
+/* copy rows from an (N, MA) ELLPACK-R matrix to an (N, MB) one */
+int bmlhip_copy_rows(int prec, int N, int MA, const void *d_Av, const int *d_Ai,
+                     const int *d_An, int MB, void *d_Bv, int *d_Bi, int *d_Bn);
+/* fill the diagonal: value 1, nnz 1 (identity) */
+int bmlhip_identity(int prec, int N, int M, void *d_Av, int *d_Ai, int *d_An);
+/* nnz statistics of a matrix -> status max_nnz / total_nnz */
+int bmlhip_row_stats(int N, const int *d_An);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BMLHIP_ABI_H */

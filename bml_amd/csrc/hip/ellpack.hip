@@ -1,0 +1,912 @@
+// gfx950 kernels for the ELLPACK-R thresholded multiply path.
+//
+// Row accumulator.  Every output row is built by a "group" of GROUP lanes
+// (8..64, so 64/GROUP rows per 64-lane wavefront) that owns an open-addressed
+// hash table in LDS: keys[] (column, -1 = empty), vals[] (the running sum)
+// and ord[] (slot of the r-th distinct column, in first-touch order).  The
+// group walks row i of A in jp order; for each (a, j) it streams row j of B
+// in GROUP-wide chunks (one coalesced load per chunk), probes/inserts the
+// columns (linear probing, CAS to claim a slot) and adds a*b into vals[].
+// All columns of one row of B are distinct, so within a chunk no two lanes
+// touch the same slot, and successive chunks/steps are ordered by the
+// wavefront's program order: every output value is summed in exactly the
+// reference's order (jp ascending), with no atomics on values.  New columns
+// get their rank from a ballot prefix count in lane (= kp) order, so ord[]
+// reproduces the reference's first-touch order and the epilogue emits each
+// output row in the reference's column order.
+//
+// Epilogue: the group walks ord[], keeps the diagonal and |v| > threshold,
+// compacts with a ballot prefix count into the output row, records per-row
+// trace contributions and resets the touched slots.
+//
+// Rows whose distinct-column count would pass 3/4 of the LDS table stop and
+// are listed for a second pass that runs the same code on a large table in
+// global memory (one wavefront per row), so no input is ever refused.
+#include "../bmlhip_abi.h"
+#include "devtypes.h"
+
+#include <limits.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+using namespace bmlhip;
+
+namespace {
+
+constexpr int EMPTY = -1;
+constexpr int FALLBACK_BLOCKS = 32;
+
+template <typename T>
+__device__ __forceinline__ T shfl_any(T v, int src) {
+    static_assert(sizeof(T) % 4 == 0, "element must be a multiple of 4 bytes");
+    union U {
+        T t;
+        int w[sizeof(T) / 4];
+    } u, r;
+    u.t = v;
+#pragma unroll
+    for (int k = 0; k < (int) (sizeof(T) / 4); k++) r.w[k] = __shfl(u.w[k], src, 64);
+    return r.t;
+}
+
+__device__ __forceinline__ unsigned hash_slot(int col, int tbits) {
+    return ((unsigned) col * 0x9E3779B1u) >> (32 - tbits);
+}
+
+struct SpParams {
+    int N, MA, MB, MC, kind;
+    const void *Av;
+    const int *Ai, *An;
+    const void *Bv;
+    const int *Bi, *Bn;
+    void *Cv;
+    int *Ci, *Cn;
+    double thr;
+    double *rowtr;  // [2*N]: per-row tr X and tr X^2 contributions (x2)
+    bmlhip_status *st;
+    int *ovf;  // rows handed to the global-table pass
+};
+
+// One row per group.  keys/vals/ord point at the group's table (LDS or
+// global); returns through the output arrays and the status block.
+template <typename T, int GROUP, typename ORD>
+__device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *ord, int tbits,
+                           int cap) {
+    using E = Elem<T>;
+    const int lane = lane_id();
+    const int g = lane / GROUP, gl = lane % GROUP;
+    const uint64_t gmask = GROUP == 64 ? ~0ull : (((1ull << GROUP) - 1ull) << (g * GROUP));
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const T *Av = (const T *) p.Av;
+    const T *Bv = (const T *) p.Bv;
+    const bool live = i >= 0;
+    const int nA = live ? p.An[i] : 0;
+    const size_t rowA = (size_t) (live ? i : 0) * p.MA;
+    const unsigned tmask = (1u << tbits) - 1u;
+
+    int count = 0;      // distinct columns so far (group-uniform)
+    bool tovf = false;  // table would overflow (group-uniform)
+    T trx = E::zero();
+
+    for (int c0 = 0; __any(c0 < nA); c0 += GROUP) {
+        const int jl = c0 + gl;
+        T aval = E::zero();
+        int acol = 0;
+        if (jl < nA) {
+            aval = Av[rowA + jl];
+            acol = p.Ai[rowA + jl];
+        }
+        const int steps = nA - c0 < GROUP ? nA - c0 : GROUP;
+        for (int t = 0; __any(t < steps && !tovf); t++) {
+            bool gact = t < steps && !tovf;
+            const T a = shfl_any(aval, g * GROUP + t);
+            const int j = __shfl(acol, g * GROUP + t, 64);
+            const int nb = gact ? p.Bn[j] : 0;
+            if (gact && j == i) trx = E::add(trx, a);
+            const size_t rowB = (size_t) j * p.MB;
+            for (int k0 = 0; __any(gact && k0 < nb); k0 += GROUP) {
+                if (gact && count + GROUP > cap) {
+                    tovf = true;
+                    gact = false;
+                }
+                const int kp = k0 + gl;
+                const bool act = gact && kp < nb;
+                int col = 0, slot = 0;
+                bool isnew = false;
+                T b = E::zero();
+                if (act) {
+                    col = p.Bi[rowB + kp];
+                    b = Bv[rowB + kp];
+                    unsigned h = hash_slot(col, tbits);
+                    while (true) {
+                        const int kk = keys[h];
+                        if (kk == col) break;
+                        if (kk == EMPTY) {
+                            const int old = atomicCAS(&keys[h], EMPTY, col);
+                            if (old == EMPTY) {
+                                isnew = true;
+                                break;
+                            }
+                            if (old == col) break;
+                        }
+                        h = (h + 1u) & tmask;
+                    }
+                    slot = (int) h;
+                }
+                const uint64_t nm = ballot64(isnew) & gmask;
+                if (isnew) ord[count + popc64(nm & ltmask)] = (ORD) slot;
+                count += popc64(nm);
+                if (act) vals[slot] = E::add(vals[slot], E::mul(a, b));
+            }
+        }
+    }
+
+    // epilogue: thresholded compaction in first-touch order
+    const bool rowact = live && !tovf;
+    const int MC = p.MC;
+    const size_t rowC = (size_t) (live ? i : 0) * MC;
+    T *Cv = (T *) p.Cv;
+    int kept = 0;
+    T d2 = E::zero();
+    for (int r0 = 0; __any(rowact && r0 < count); r0 += GROUP) {
+        const int r = r0 + gl;
+        const bool act = rowact && r < count;
+        int s = 0, col = 0;
+        T v = E::zero();
+        if (act) {
+            s = (int) ord[r];
+            col = keys[s];
+            v = vals[s];
+        }
+        const bool isdiag = act && col == i;
+        const bool keep = isdiag || (act && E::above(v, p.thr));
+        const uint64_t km = ballot64(keep) & gmask;
+        const int pos = kept + popc64(km & ltmask);
+        if (keep && pos < MC) {
+            p.Ci[rowC + pos] = col;
+            Cv[rowC + pos] = v;
+        }
+        const uint64_t dm = ballot64(isdiag) & gmask;
+        if (dm) d2 = shfl_any(v, __ffsll((unsigned long long) dm) - 1);
+        if (act) {
+            keys[s] = EMPTY;
+            vals[s] = E::zero();
+        }
+        kept += popc64(km);
+    }
+    if (live && tovf) {
+        // leave the table clean, then hand the row to the global-table pass
+        for (int r = gl; r < count; r += GROUP) {
+            const int s = (int) ord[r];
+            keys[s] = EMPTY;
+            vals[s] = E::zero();
+        }
+        if (gl == 0) {
+            const int q = atomicAdd(&p.st->table_overflow_rows, 1);
+            p.ovf[q] = i;
+        }
+    }
+    if (rowact && gl == 0) {
+        const int stored = kept < MC ? kept : MC;
+        p.Cn[i] = stored;
+        // kind 1 (AB) checks the touched count, as the reference does
+        const bool over = p.kind == 1 ? count > MC : kept > MC;
+        if (over) {
+            atomicAdd(&p.st->m_overflow_rows, 1);
+            atomicMin(&p.st->first_m_overflow, i);
+        }
+        atomicMax(&p.st->max_nnz, stored);
+        atomicMax(&p.st->max_touched, count);
+        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        if (p.kind == 0) {
+            p.rowtr[i] = E::re(trx);
+            p.rowtr[p.N + i] = E::re(d2);
+        }
+    }
+}
+
+// First pass: table in LDS, 64/GROUP rows per 64-thread block.
+template <typename T, int GROUP>
+__global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, int tbits) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int R = 64 / GROUP;
+    const int tsize = 1 << tbits;
+    T *vals = (T *) smem;
+    int *keys = (int *) (smem + (size_t) R * tsize * sizeof(T));
+    uint16_t *ord = (uint16_t *) (smem + (size_t) R * tsize * (sizeof(T) + sizeof(int)));
+    for (int s = threadIdx.x; s < R * tsize; s += 64) {
+        keys[s] = EMPTY;
+        vals[s] = Elem<T>::zero();
+    }
+    __syncthreads();
+    const int g = threadIdx.x / GROUP;
+    const int row = blockIdx.x * R + g;
+    const int i = row < p.N ? row : -1;
+    const int cap = tsize - tsize / 4;
+    spgemm_row<T, GROUP, uint16_t>(p, i, keys + g * tsize, vals + (size_t) g * tsize,
+                                   ord + g * tsize, tbits, cap);
+}
+
+// Second pass: rows listed in p.ovf, one wavefront per row, table in global
+// memory (tsize = 2^tbits slots per block, at least twice the row's bound).
+template <typename T>
+__global__ __launch_bounds__(64) void k_spgemm_global(SpParams p, int tbits, unsigned char *tabs,
+                                                      int *cursor) {
+    const int n = p.st->table_overflow_rows;
+    if (n == 0) return;
+    const size_t tsize = (size_t) 1 << tbits;
+    unsigned char *base = tabs + blockIdx.x * tsize * (sizeof(T) + 2 * sizeof(int));
+    T *vals = (T *) base;
+    int *keys = (int *) (base + tsize * sizeof(T));
+    int *ord = (int *) (base + tsize * (sizeof(T) + sizeof(int)));
+    for (size_t s = threadIdx.x; s < tsize; s += 64) {
+        keys[s] = EMPTY;
+        vals[s] = Elem<T>::zero();
+    }
+    __syncthreads();
+    while (true) {
+        int q = 0;
+        if (threadIdx.x == 0) q = atomicAdd(cursor, 1);
+        q = __shfl(q, 0, 64);
+        if (q >= n) break;
+        spgemm_row<T, 64, int>(p, p.ovf[q], keys, vals, ord, tbits, (int) (tsize - tsize / 4));
+        __syncthreads();
+    }
+}
+
+// Deterministic sum of n doubles (fixed partition, fixed tree).
+__global__ __launch_bounds__(1024) void k_sum(const double *x, int n, double *out) {
+    __shared__ double part[1024];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += 1024) s += x[k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int) threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
+}
+
+// min of x[0..n) -> out[0], max of y[0..n) -> out[1]
+__global__ __launch_bounds__(1024) void k_minmax(const double *x, const double *y, int n,
+                                                 double *out, double init_max) {
+    __shared__ double lo[1024], hi[1024];
+    double a = 1.7976931348623157e308, b = init_max;
+    for (int k = threadIdx.x; k < n; k += 1024) {
+        a = x[k] < a ? x[k] : a;
+        b = y[k] > b ? y[k] : b;
+    }
+    lo[threadIdx.x] = a;
+    hi[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int) threadIdx.x < w) {
+            lo[threadIdx.x] = lo[threadIdx.x + w] < lo[threadIdx.x] ? lo[threadIdx.x + w]
+                                                                    : lo[threadIdx.x];
+            hi[threadIdx.x] = hi[threadIdx.x + w] > hi[threadIdx.x] ? hi[threadIdx.x + w]
+                                                                    : hi[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = lo[0];
+        out[1] = hi[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// add / add_norm: A <- alpha*A + beta*B per row, union in first-touch order
+// (A's columns, then B's new ones), every entry kept iff |v| > threshold.
+// mode 0: add (A skipped if alpha == 0, B if beta == 0); mode 1: add_norm
+// with Re((A-B)^2); mode 2: add_norm with |A-B|^2.  One wavefront per row;
+// y (A - B) lives in a second value table.
+
+struct AddParams {
+    int N, MA, MB, mode;
+    void *Av;
+    int *Ai, *An;
+    const void *Bv;
+    const int *Bi, *Bn;
+    double alpha, beta, thr;
+    double *rownorm;
+    bmlhip_status *st;
+};
+
+template <typename T>
+__device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int *ord, int tbits) {
+    using E = Elem<T>;
+    const int lane = lane_id();
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const unsigned tmask = (1u << tbits) - 1u;
+    T *Av = (T *) p.Av;
+    const T *Bv = (const T *) p.Bv;
+    const size_t rA = (size_t) i * p.MA, rB = (size_t) i * p.MB;
+    int count = 0;
+    for (int part = 0; part < 2; part++) {
+        const double s = part ? p.beta : p.alpha;
+        if (p.mode == 0 && !(s > 0.0 || s < 0.0)) continue;
+        const int n = part ? p.Bn[i] : p.An[i];
+        for (int k0 = 0; k0 < n; k0 += 64) {
+            const int kp = k0 + lane;
+            const bool act = kp < n;
+            int col = 0, slot = 0;
+            bool isnew = false;
+            T v = E::zero();
+            if (act) {
+                col = part ? p.Bi[rB + kp] : p.Ai[rA + kp];
+                v = part ? Bv[rB + kp] : Av[rA + kp];
+                unsigned h = hash_slot(col, tbits);
+                while (true) {
+                    const int kk = keys[h];
+                    if (kk == col) break;
+                    if (kk == EMPTY) {
+                        const int old = atomicCAS(&keys[h], EMPTY, col);
+                        if (old == EMPTY) {
+                            isnew = true;
+                            break;
+                        }
+                        if (old == col) break;
+                    }
+                    h = (h + 1u) & tmask;
+                }
+                slot = (int) h;
+            }
+            const uint64_t nm = ballot64(isnew);
+            if (isnew) ord[count + popc64(nm & ltmask)] = slot;
+            count += popc64(nm);
+            if (act) {
+                xs[slot] = E::axpy(xs[slot], s, v);
+                if (p.mode) ys[slot] = part ? E::sub(ys[slot], v) : E::add(ys[slot], v);
+            }
+        }
+    }
+    int kept = 0;
+    double nrm = 0.0;
+    for (int r0 = 0; r0 < count; r0 += 64) {
+        const int r = r0 + lane;
+        const bool act = r < count;
+        int s = 0, col = 0;
+        T x = E::zero();
+        if (act) {
+            s = ord[r];
+            col = keys[s];
+            x = xs[s];
+            if (p.mode == 1) nrm += E::re_sq(ys[s]);
+            if (p.mode == 2) nrm += E::abs2(ys[s]);
+        }
+        const bool keep = act && E::above(x, p.thr);
+        const uint64_t km = ballot64(keep);
+        const int pos = kept + popc64(km & ltmask);
+        if (keep && pos < p.MA) {
+            p.Ai[rA + pos] = col;
+            Av[rA + pos] = x;
+        }
+        if (act) {
+            keys[s] = EMPTY;
+            xs[s] = E::zero();
+            ys[s] = E::zero();
+        }
+        kept += popc64(km);
+    }
+    // fixed-order wavefront reduction of the row's norm
+    for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, 64);
+    if (lane == 0) {
+        const int stored = kept < p.MA ? kept : p.MA;
+        p.An[i] = stored;
+        if (kept > p.MA) {
+            atomicAdd(&p.st->m_overflow_rows, 1);
+            atomicMin(&p.st->first_m_overflow, i);
+        }
+        atomicMax(&p.st->max_nnz, stored);
+        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        if (p.mode) p.rownorm[i] = nrm;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_add_lds(AddParams p, int tbits) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tsize = 1 << tbits;
+    T *xs = (T *) smem;
+    T *ys = xs + tsize;
+    int *keys = (int *) (ys + tsize);
+    int *ord = keys + tsize;
+    for (int s = threadIdx.x; s < tsize; s += 64) {
+        keys[s] = EMPTY;
+        xs[s] = Elem<T>::zero();
+        ys[s] = Elem<T>::zero();
+    }
+    __syncthreads();
+    add_row<T>(p, blockIdx.x, keys, xs, ys, ord, tbits);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_add_global(AddParams p, int tbits, unsigned char *tabs) {
+    const size_t tsize = (size_t) 1 << tbits;
+    unsigned char *base = tabs + blockIdx.x * tsize * (2 * sizeof(T) + 2 * sizeof(int));
+    T *xs = (T *) base;
+    T *ys = xs + tsize;
+    int *keys = (int *) (ys + tsize);
+    int *ord = keys + tsize;
+    for (size_t s = threadIdx.x; s < tsize; s += 64) {
+        keys[s] = EMPTY;
+        xs[s] = Elem<T>::zero();
+        ys[s] = Elem<T>::zero();
+    }
+    __syncthreads();
+    for (int i = blockIdx.x; i < p.N; i += gridDim.x) {
+        add_row<T>(p, i, keys, xs, ys, ord, tbits);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// threshold: in-place stable compaction, one wavefront per row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_threshold(int N, int M, T *Av, int *Ai, int *An,
+                                                   double thr) {
+    const int i = blockIdx.x * 4 + threadIdx.x / 64;
+    if (i >= N) return;
+    const int lane = threadIdx.x & 63;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const size_t r = (size_t) i * M;
+    const int n = An[i];
+    int kept = 0;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int kp = k0 + lane;
+        const bool act = kp < n;
+        T v = Elem<T>::zero();
+        int c = 0;
+        if (act) {
+            v = Av[r + kp];
+            c = Ai[r + kp];
+        }
+        const bool keep = act && Elem<T>::above(v, thr);
+        const uint64_t km = ballot64(keep);
+        const int pos = kept + popc64(km & ltmask);
+        if (keep) {
+            Av[r + pos] = v;
+            Ai[r + pos] = c;
+        }
+        kept += popc64(km);
+    }
+    if (lane == 0) An[i] = kept;
+}
+
+// trace: first diagonal entry of each row -> rowv[i]
+template <typename T>
+__global__ __launch_bounds__(256) void k_trace_rows(int N, int M, const T *Av, const int *Ai,
+                                                    const int *An, double *rowv) {
+    const int i = blockIdx.x * 4 + threadIdx.x / 64;
+    if (i >= N) return;
+    const int lane = threadIdx.x & 63;
+    const size_t r = (size_t) i * M;
+    const int n = An[i];
+    double v = 0.0;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int kp = k0 + lane;
+        const uint64_t dm = ballot64(kp < n && Ai[r + kp] == i);
+        if (dm) {
+            const int first = k0 + __ffsll((unsigned long long) dm) - 1;
+            v = Elem<T>::re(Av[r + first]);
+            break;
+        }
+    }
+    if (lane == 0) rowv[i] = v;
+}
+
+// gershgorin: per row, d = last diagonal entry, radius = sum |offdiag| held
+// in T (sequential, the reference's order); rowlo = Re(d - r), rowhi = Re(d + r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gersh_rows(int N, int M, const T *Av, const int *Ai,
+                                                    const int *An, double *rowlo,
+                                                    double *rowhi) {
+    using E = Elem<T>;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const size_t r = (size_t) i * M;
+    T d = E::zero(), rad = E::zero();
+    for (int k = 0; k < An[i]; k++) {
+        const T v = Av[r + k];
+        if (Ai[r + k] == i)
+            d = v;
+        else
+            rad = E::acc_mag(rad, E::mag(v));
+    }
+    rowlo[i] = E::re_minus(d, rad);
+    rowhi[i] = E::re_plus(d, rad);
+}
+
+// scale (optional) then add_identity with threshold: one thread per row,
+// sequential like the reference (bml_add_ellpack_typed.c:420-549)
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale_add_identity(int N, int M, T *Av, int *Ai,
+                                                            int *An, T s, int do_scale,
+                                                            double beta, double thr,
+                                                            int do_identity,
+                                                            bmlhip_status *st) {
+    using E = Elem<T>;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const size_t r = (size_t) i * M;
+    const int n = An[i];
+    if (do_scale)
+        for (int k = 0; k < n; k++) Av[r + k] = E::mul(s, Av[r + k]);
+    if (!do_identity) return;
+    int diag = -1;
+    for (int k = 0; k < n; k++)
+        if (Ai[r + k] == i) diag = k;
+    bool extra = false;
+    T ev = E::zero();
+    if (beta > 0.0 || beta < 0.0) {
+        if (diag < 0) {
+            extra = true;
+            ev = E::from_real(beta);
+        } else {
+            Av[r + diag] = E::add_real(Av[r + diag], beta);
+        }
+    }
+    int kept = 0;
+    for (int k = 0; k < n; k++) {
+        const T v = Av[r + k];
+        if (E::above(v, thr)) {
+            Av[r + kept] = v;
+            Ai[r + kept] = Ai[r + k];
+            kept++;
+        }
+    }
+    if (extra && E::above(ev, thr)) {
+        if (kept < M) {
+            Av[r + kept] = ev;
+            Ai[r + kept] = i;
+            kept++;
+        } else {
+            atomicAdd(&st->m_overflow_rows, 1);
+            atomicMin(&st->first_m_overflow, i);
+        }
+    }
+    An[i] = kept;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_rows(int N, int MA, const T *Av, const int *Ai,
+                                                   const int *An, int MB, T *Bv, int *Bi,
+                                                   int *Bn, bmlhip_status *st) {
+    const int i = blockIdx.x * 4 + threadIdx.x / 64;
+    if (i >= N) return;
+    const int lane = threadIdx.x & 63;
+    const int n = An[i];
+    const int m = n < MB ? n : MB;
+    for (int k = lane; k < m; k += 64) {
+        Bv[(size_t) i * MB + k] = Av[(size_t) i * MA + k];
+        Bi[(size_t) i * MB + k] = Ai[(size_t) i * MA + k];
+    }
+    if (lane == 0) {
+        Bn[i] = m;
+        if (n > MB) {
+            atomicAdd(&st->m_overflow_rows, 1);
+            atomicMin(&st->first_m_overflow, i);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_identity(int N, int M, T *Av, int *Ai, int *An) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    Av[(size_t) i * M] = Elem<T>::from_real(1.0);
+    Ai[(size_t) i * M] = i;
+    An[i] = 1;
+}
+
+__global__ __launch_bounds__(256) void k_row_stats(int N, const int *An, bmlhip_status *st) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int n = i < N ? An[i] : 0;
+    // wavefront reductions, then one atomic per wavefront
+    int mx = n;
+    unsigned long long sm = (unsigned long long) n;
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_down(mx, off, 64);
+        mx = o > mx ? o : mx;
+        sm += __shfl_down(sm, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&st->max_nnz, mx);
+        atomicAdd((unsigned long long *) &st->total_nnz, sm);
+    }
+}
+
+inline int ceil_log2(long long v) {
+    int b = 0;
+    while ((1ll << b) < v) b++;
+    return b;
+}
+
+int fail_msg(int code, const char *fmt, ...) {
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    bmlhip_set_errmsg(buf);
+    return code;
+}
+
+int report(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return 0;
+    return fail_msg(3, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename T, int GROUP>
+int launch_spgemm_lds(const SpParams &p, int tbits, hipStream_t s) {
+    constexpr int R = 64 / GROUP;
+    const size_t lds = (size_t) R * ((size_t) 1 << tbits) * (sizeof(T) + sizeof(int) + 2);
+    if (lds > 160 * 1024) {
+        return fail_msg(2, "LDS table of %zu bytes exceeds 160 KiB", lds);
+    }
+    if (lds > 64 * 1024)
+        (void) hipFuncSetAttribute((const void *) k_spgemm_lds<T, GROUP>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+    const int blocks = (p.N + R - 1) / R;
+    if (blocks > 0)
+        hipLaunchKernelGGL((k_spgemm_lds<T, GROUP>), dim3(blocks), dim3(64), lds, s, p, tbits);
+    return report("k_spgemm_lds");
+}
+
+template <typename T>
+int spgemm_typed(const bmlhip_spgemm_args *a) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    SpParams p;
+    p.N = a->N;
+    p.MA = a->MA;
+    p.MB = a->MB;
+    p.MC = a->MC;
+    p.kind = a->kind;
+    p.Av = a->d_Av;
+    p.Ai = a->d_Ai;
+    p.An = a->d_An;
+    p.Bv = a->d_Bv;
+    p.Bi = a->d_Bi;
+    p.Bn = a->d_Bn;
+    p.Cv = a->d_Cv;
+    p.Ci = a->d_Ci;
+    p.Cn = a->d_Cn;
+    p.thr = a->threshold;
+    p.st = bmlhip_status_dev();
+    p.rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
+    p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
+    // global-table pass: one table per block, >= 2x the row bound
+    long long bound = (long long) a->MA * a->MB;
+    if (bound > a->N) bound = a->N;
+    int gbits = ceil_log2(2 * bound + 64);
+    if (gbits < 10) gbits = 10;
+    const size_t gbytes =
+        (size_t) FALLBACK_BLOCKS * ((size_t) 1 << gbits) * (sizeof(T) + 2 * sizeof(int));
+    unsigned char *gtab = (unsigned char *) bmlhip_scratch(2, gbytes + 256);
+    int *cursor = &p.st->cursor;
+    if (!p.rowtr || !p.ovf || !gtab) {
+        return fail_msg(4, "scratch allocation failed");
+    }
+    int tbits = ceil_log2(a->table);
+    if (tbits < 6) tbits = 6;
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    switch (a->group) {
+    case 8: rc = launch_spgemm_lds<T, 8>(p, tbits, s); break;
+    case 16: rc = launch_spgemm_lds<T, 16>(p, tbits, s); break;
+    case 32: rc = launch_spgemm_lds<T, 32>(p, tbits, s); break;
+    case 64: rc = launch_spgemm_lds<T, 64>(p, tbits, s); break;
+    default:
+        return fail_msg(2, "group %d not one of 8/16/32/64", a->group);
+    }
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_spgemm_global<T>, dim3(FALLBACK_BLOCKS), dim3(64), 0, s, p, gbits, gtab,
+                       cursor);
+    if ((rc = report("k_spgemm_global"))) return rc;
+    if (a->kind == 0) {
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, p.rowtr, a->N, &p.st->scalars[0]);
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, p.rowtr + a->N, a->N,
+                           &p.st->scalars[1]);
+        if ((rc = report("k_sum"))) return rc;
+    }
+    return 0;
+}
+
+template <typename T>
+int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv, const int *Bi,
+              const int *Bn, double alpha, double beta, double thr, int mode) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    AddParams p;
+    p.N = N;
+    p.MA = MA;
+    p.MB = MB;
+    p.mode = mode;
+    p.Av = Av;
+    p.Ai = Ai;
+    p.An = An;
+    p.Bv = Bv;
+    p.Bi = Bi;
+    p.Bn = Bn;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.thr = thr;
+    p.st = bmlhip_status_dev();
+    p.rownorm = (double *) bmlhip_scratch(0, sizeof(double) * (size_t) (N + 1));
+    if (!p.rownorm) return 4;
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    // a row's union has at most MA + MB columns; table at most 3/4 full
+    const int tbits = ceil_log2((long long) (MA + MB) * 4 / 3 + 64);
+    const size_t lds = ((size_t) 1 << tbits) * (2 * sizeof(T) + 2 * sizeof(int));
+    if (N > 0) {
+        if (lds <= 64 * 1024) {
+            hipLaunchKernelGGL(k_add_lds<T>, dim3(N), dim3(64), lds, s, p, tbits);
+            if ((rc = report("k_add_lds"))) return rc;
+        } else {
+            const int blocks = 256;
+            unsigned char *tabs = (unsigned char *) bmlhip_scratch(2, blocks * lds + 256);
+            if (!tabs) return 4;
+            hipLaunchKernelGGL(k_add_global<T>, dim3(blocks), dim3(64), 0, s, p, tbits, tabs);
+            if ((rc = report("k_add_global"))) return rc;
+        }
+    }
+    if (mode) {
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, p.rownorm, N, &p.st->scalars[2]);
+        if ((rc = report("k_sum"))) return rc;
+    }
+    return 0;
+}
+
+template <typename T>
+int scale_add_identity_typed(int N, int M, void *Av, int *Ai, int *An, const void *s_bits,
+                             double beta, double thr, int do_identity) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    T sv;
+    memset(&sv, 0, sizeof sv);
+    if (s_bits) memcpy(&sv, s_bits, sizeof(T));
+    if (N > 0)
+        hipLaunchKernelGGL(k_scale_add_identity<T>, dim3((N + 255) / 256), dim3(256), 0, s, N, M,
+                           (T *) Av, Ai, An, sv, s_bits ? 1 : 0, beta, thr, do_identity,
+                           bmlhip_status_dev());
+    return report("k_scale_add_identity");
+}
+
+template <typename T>
+int threshold_typed(int N, int M, void *Av, int *Ai, int *An, double thr) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    if (N > 0)
+        hipLaunchKernelGGL(k_threshold<T>, dim3((N + 3) / 4), dim3(256), 0, s, N, M, (T *) Av, Ai,
+                           An, thr);
+    return report("k_threshold");
+}
+
+template <typename T>
+int trace_typed(int N, int M, const void *Av, const int *Ai, const int *An) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    double *rowv = (double *) bmlhip_scratch(0, sizeof(double) * (size_t) (N + 1));
+    if (!rowv) return 4;
+    if (N > 0)
+        hipLaunchKernelGGL(k_trace_rows<T>, dim3((N + 3) / 4), dim3(256), 0, s, N, M,
+                           (const T *) Av, Ai, An, rowv);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rowv, N, &bmlhip_status_dev()->scalars[0]);
+    return report("k_trace");
+}
+
+template <typename T>
+int gershgorin_typed(int N, int M, const void *Av, const int *Ai, const int *An) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    double *rows = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (N + 1));
+    if (!rows) return 4;
+    if (N > 0)
+        hipLaunchKernelGGL(k_gersh_rows<T>, dim3((N + 255) / 256), dim3(256), 0, s, N, M,
+                           (const T *) Av, Ai, An, rows, rows + N);
+    // emax starts at DBL_MIN as in the reference
+    hipLaunchKernelGGL(k_minmax, dim3(1), dim3(1024), 0, s, rows, rows + N, N,
+                       &bmlhip_status_dev()->scalars[0], 2.2250738585072014e-308);
+    return report("k_gershgorin");
+}
+
+template <typename T>
+int copy_rows_typed(int N, int MA, const void *Av, const int *Ai, const int *An, int MB, void *Bv,
+                    int *Bi, int *Bn) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    if (N > 0)
+        hipLaunchKernelGGL(k_copy_rows<T>, dim3((N + 3) / 4), dim3(256), 0, s, N, MA,
+                           (const T *) Av, Ai, An, MB, (T *) Bv, Bi, Bn, bmlhip_status_dev());
+    return report("k_copy_rows");
+}
+
+template <typename T>
+int identity_typed(int N, int M, void *Av, int *Ai, int *An) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    if (N > 0)
+        hipLaunchKernelGGL(k_identity<T>, dim3((N + 255) / 256), dim3(256), 0, s, N, M, (T *) Av,
+                           Ai, An);
+    return report("k_identity");
+}
+
+}  // namespace
+
+#define DISPATCH(prec, call_tmpl)                                                  \
+    switch (prec) {                                                                \
+    case 1: return call_tmpl(float);                                               \
+    case 2: return call_tmpl(double);                                              \
+    case 3: return call_tmpl(cf32);                                                \
+    case 4: return call_tmpl(cf64);                                                \
+    default: return fail_msg(2, "unknown precision %d", prec);                      \
+    }
+
+extern "C" {
+
+int bmlhip_spgemm(const bmlhip_spgemm_args *a) {
+#define CALL(T) spgemm_typed<T>(a)
+    DISPATCH(a->prec, CALL)
+#undef CALL
+}
+
+int bmlhip_add(int prec, int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
+               const int *Bi, const int *Bn, double alpha, double beta, double thr, int mode) {
+#define CALL(T) add_typed<T>(N, MA, Av, Ai, An, MB, Bv, Bi, Bn, alpha, beta, thr, mode)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_scale_add_identity(int prec, int N, int M, void *Av, int *Ai, int *An,
+                              const void *s_bits, double beta, double thr, int do_identity) {
+#define CALL(T) scale_add_identity_typed<T>(N, M, Av, Ai, An, s_bits, beta, thr, do_identity)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_threshold(int prec, int N, int M, void *Av, int *Ai, int *An, double thr) {
+#define CALL(T) threshold_typed<T>(N, M, Av, Ai, An, thr)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_trace(int prec, int N, int M, const void *Av, const int *Ai, const int *An) {
+#define CALL(T) trace_typed<T>(N, M, Av, Ai, An)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_gershgorin(int prec, int N, int M, const void *Av, const int *Ai, const int *An) {
+#define CALL(T) gershgorin_typed<T>(N, M, Av, Ai, An)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_copy_rows(int prec, int N, int MA, const void *Av, const int *Ai, const int *An,
+                     int MB, void *Bv, int *Bi, int *Bn) {
+#define CALL(T) copy_rows_typed<T>(N, MA, Av, Ai, An, MB, Bv, Bi, Bn)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_identity(int prec, int N, int M, void *Av, int *Ai, int *An) {
+#define CALL(T) identity_typed<T>(N, M, Av, Ai, An)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_row_stats(int N, const int *An) {
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    if (N > 0)
+        hipLaunchKernelGGL(k_row_stats, dim3((N + 255) / 256), dim3(256), 0,
+                           (hipStream_t) bmlhip_stream(), N, An, bmlhip_status_dev());
+    return report("k_row_stats");
+}
+
+}  // extern "C"

@@ -1,0 +1,188 @@
+// Device runtime for the bml HIP backend: one device, one stream (default
+// priority), a device status block, and grow-on-demand scratch buffers.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../bmlhip_abi.h"
+
+namespace {
+
+struct Runtime {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    bmlhip_status *status = nullptr;
+    void *scratch[8] = {};
+    size_t scratch_bytes[8] = {};
+    char msg[512] = {0};
+};
+
+Runtime g;
+
+int fail(hipError_t e, const char *what) {
+    snprintf(g.msg, sizeof g.msg, "%s: %s", what, hipGetErrorString(e));
+    return 3;  // BML_HIP_ERR_DEVICE
+}
+
+}  // namespace
+
+// Clears the per-operation fields of the status block on the library stream
+// (a kernel rather than a memset node, so a captured graph replays it
+// faithfully).  The overflow fields accumulate until the host has checked
+// them and asks for them to be cleared too.
+__global__ void k_status_reset(bmlhip_status *st, int clear_errors) {
+    if (threadIdx.x == 0) {
+        if (clear_errors) {
+            st->m_overflow_rows = 0;
+            st->first_m_overflow = INT_MAX;
+        }
+        st->table_overflow_rows = 0;
+        st->max_nnz = 0;
+        st->max_touched = 0;
+        st->cursor = 0;
+        st->total_nnz = 0;
+        for (int k = 0; k < 4; k++) st->scalars[k] = 0.0;
+    }
+}
+
+#define CHECK(call)                                   \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return fail(e_, #call); \
+    } while (0)
+
+extern "C" {
+
+const char *bmlhip_errmsg(void) { return g.msg; }
+
+void bmlhip_set_errmsg(const char *m) { snprintf(g.msg, sizeof g.msg, "%s", m); }
+
+int bmlhip_init(int device) {
+    if (g.device >= 0) return 0;
+    int n = 0;
+    CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        snprintf(g.msg, sizeof g.msg, "device %d not present (%d devices)", device, n);
+        return 3;
+    }
+    CHECK(hipSetDevice(device));
+    CHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    CHECK(hipMalloc((void **) &g.status, sizeof(bmlhip_status)));
+    hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(64), 0, g.stream, g.status, 1);
+    CHECK(hipGetLastError());
+    CHECK(hipStreamSynchronize(g.stream));
+    g.device = device;
+    return 0;
+}
+
+void bmlhip_shutdown(void) {
+    if (g.device < 0) return;
+    (void) hipStreamSynchronize(g.stream);
+    for (int s = 0; s < 8; s++) {
+        if (g.scratch[s]) (void) hipFree(g.scratch[s]);
+        g.scratch[s] = nullptr;
+        g.scratch_bytes[s] = 0;
+    }
+    (void) hipFree(g.status);
+    (void) hipStreamDestroy(g.stream);
+    g.status = nullptr;
+    g.stream = nullptr;
+    g.device = -1;
+}
+
+void *bmlhip_stream(void) { return (void *) g.stream; }
+
+int bmlhip_malloc(void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        snprintf(g.msg, sizeof g.msg, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+        *p = nullptr;
+        return 4;  // BML_HIP_ERR_ALLOC
+    }
+    return 0;
+}
+
+void bmlhip_free(void *p) {
+    if (!p) return;
+    // frees must not race kernels still reading the buffer
+    (void) hipStreamSynchronize(g.stream);
+    (void) hipFree(p);
+}
+
+int bmlhip_h2d(void *d, const void *s, size_t n) {
+    if (!n) return 0;
+    CHECK(hipMemcpyAsync(d, s, n, hipMemcpyHostToDevice, g.stream));
+    CHECK(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int bmlhip_d2h(void *d, const void *s, size_t n) {
+    if (!n) return 0;
+    CHECK(hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, g.stream));
+    CHECK(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int bmlhip_d2d(void *d, const void *s, size_t n) {
+    if (!n) return 0;
+    CHECK(hipMemcpyAsync(d, s, n, hipMemcpyDeviceToDevice, g.stream));
+    return 0;
+}
+
+int bmlhip_zero(void *d, size_t n) {
+    if (!n) return 0;
+    CHECK(hipMemsetAsync(d, 0, n, g.stream));
+    return 0;
+}
+
+int bmlhip_sync(void) {
+    CHECK(hipStreamSynchronize(g.stream));
+    CHECK(hipGetLastError());
+    return 0;
+}
+
+void *bmlhip_scratch(int slot, size_t bytes) {
+    if (slot < 0 || slot >= 8) return nullptr;
+    if (bytes == 0) bytes = 256;
+    if (g.scratch_bytes[slot] >= bytes) return g.scratch[slot];
+    if (g.scratch[slot]) {
+        (void) hipStreamSynchronize(g.stream);
+        (void) hipFree(g.scratch[slot]);
+        g.scratch[slot] = nullptr;
+        g.scratch_bytes[slot] = 0;
+    }
+    size_t want = bytes + bytes / 4;  // headroom against regrowth
+    if (hipMalloc(&g.scratch[slot], want) != hipSuccess) {
+        snprintf(g.msg, sizeof g.msg, "scratch allocation of %zu bytes failed", want);
+        g.scratch[slot] = nullptr;
+        return nullptr;
+    }
+    g.scratch_bytes[slot] = want;
+    return g.scratch[slot];
+}
+
+bmlhip_status *bmlhip_status_dev(void) { return g.status; }
+
+int bmlhip_status_reset(void) {
+    hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(64), 0, g.stream, g.status, 0);
+    CHECK(hipGetLastError());
+    return 0;
+}
+
+int bmlhip_status_clear_errors(void) {
+    hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(64), 0, g.stream, g.status, 1);
+    CHECK(hipGetLastError());
+    return 0;
+}
+
+int bmlhip_status_fetch(bmlhip_status *h) {
+    CHECK(hipMemcpyAsync(h, g.status, sizeof *h, hipMemcpyDeviceToHost, g.stream));
+    CHECK(hipStreamSynchronize(g.stream));
+    CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"

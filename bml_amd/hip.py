@@ -1,0 +1,357 @@
+"""ctypes binding of libbml_hip.so and a thin Python mirror of bml's C API.
+
+The functions here keep bml's names and argument meaning (reference
+``src/C-interface/bml_multiply.h``, ``bml_add.h``, ``bml_threshold.h``,
+``bml_trace.h``, ``bml_normalize.h``): matrices are handles to device-resident
+``ellpack_hip`` / ``ellblock_hip`` matrices, outputs are preallocated by the
+caller and overwritten in place, ``multiply_x2`` returns ``(tr X, tr X^2)``.
+bml's fatal errors surface as :class:`BmlError` (the library runs in its
+record-errors mode under Python).
+
+There is no fallback: if the shared library is missing or the device is not
+usable, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .ell import Blk, DTYPES, Ell, precision_of
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbml_hip.so")
+
+ELLPACK_HIP = 6
+ELLBLOCK_HIP = 7
+ERRORS_EXIT, ERRORS_RECORD = 0, 1
+SYNC_EACH_CALL, SYNC_DEFERRED = 0, 1
+ERR_M_OVERFLOW = 1
+
+
+class BmlError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class _Ell(C.Structure):
+    _fields_ = [("matrix_type", C.c_int), ("matrix_precision", C.c_int),
+                ("distribution_mode", C.c_int), ("N", C.c_int), ("M", C.c_int),
+                ("value", C.c_void_p), ("index", C.c_void_p), ("nnz", C.c_void_p),
+                ("aux", C.c_void_p)]
+
+
+class _Blk(C.Structure):
+    _fields_ = [("matrix_type", C.c_int), ("matrix_precision", C.c_int),
+                ("distribution_mode", C.c_int), ("N", C.c_int), ("M", C.c_int),
+                ("NB", C.c_int), ("MB", C.c_int), ("bt", C.c_int),
+                ("value", C.c_void_p), ("indexb", C.c_void_p), ("nnzb", C.c_void_p),
+                ("bsize", C.c_void_p), ("bsize_host", C.c_void_p), ("aux", C.c_void_p)]
+
+
+_lib = None
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+
+_SIGS = {
+    "bml_hip_init": (_I, [_I]),
+    "bml_hip_shutdown": (None, []),
+    "bml_hip_set_error_mode": (None, [_I]),
+    "bml_hip_set_sync_mode": (None, [_I]),
+    "bml_hip_last_error": (_I, [C.c_char_p, _I]),
+    "bml_hip_clear_error": (None, []),
+    "bml_hip_synchronize": (None, []),
+    "bml_hip_stream": (_P, []),
+    "bml_hip_version": (C.c_char_p, []),
+    "bml_zero_matrix_ellpack_hip": (_P, [_I, _I, _I, _I]),
+    "bml_noinit_matrix_ellpack_hip": (_P, [_I, _I, _I, _I]),
+    "bml_identity_matrix_ellpack_hip": (_P, [_I, _I, _I, _I]),
+    "bml_deallocate_ellpack_hip": (None, [_P]),
+    "bml_clear_ellpack_hip": (None, [_P]),
+    "bml_copy_ellpack_hip_new": (_P, [_P]),
+    "bml_copy_ellpack_hip": (None, [_P, _P]),
+    "bml_import_from_dense_ellpack_hip": (_P, [_I, _I, _I, _I, _P, _D, _I]),
+    "bml_export_to_dense_ellpack_hip": (_P, [_P, _I]),
+    "bml_ellpack_hip_set_arrays": (None, [_P, _P, _P, _P, _I]),
+    "bml_ellpack_hip_get_arrays": (None, [_P, _P, _P, _P]),
+    "bml_multiply_x2_ellpack_hip": (_P, [_P, _P, _D]),
+    "bml_multiply_ellpack_hip": (None, [_P, _P, _P, _D, _D, _D]),
+    "bml_multiply_AB_ellpack_hip": (None, [_P, _P, _P, _D]),
+    "bml_multiply_adjust_AB_ellpack_hip": (None, [_P, _P, _P, _D]),
+    "bml_add_ellpack_hip": (None, [_P, _P, _D, _D, _D]),
+    "bml_add_norm_ellpack_hip": (_D, [_P, _P, _D, _D, _D]),
+    "bml_add_identity_ellpack_hip": (None, [_P, _D, _D]),
+    "bml_scale_add_identity_ellpack_hip": (None, [_P, _D, _D, _D]),
+    "bml_scale_inplace_ellpack_hip": (None, [_P, _P]),
+    "bml_threshold_ellpack_hip": (None, [_P, _D]),
+    "bml_threshold_new_ellpack_hip": (_P, [_P, _D]),
+    "bml_trace_ellpack_hip": (_D, [_P]),
+    "bml_gershgorin_ellpack_hip": (_P, [_P]),
+    "bml_normalize_ellpack_hip": (None, [_P, _D, _D]),
+    "bml_ellpack_hip_row_stats": (None, [_P, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
+}
+_OPTIONAL = {
+    "bml_block_matrix_ellblock_hip": (_P, [_I, _I, _I, _I, _P, _I]),
+    "bml_deallocate_ellblock_hip": (None, [_P]),
+    "bml_copy_ellblock_hip_new": (_P, [_P]),
+    "bml_copy_ellblock_hip": (None, [_P, _P]),
+    "bml_ellblock_hip_set_blocks": (None, [_P, _P, _P, _P, _I, _I]),
+    "bml_ellblock_hip_get_blocks": (None, [_P, _P, _P, _P, _I]),
+    "bml_export_to_dense_ellblock_hip": (_P, [_P, _I]),
+    "bml_multiply_x2_ellblock_hip": (_P, [_P, _P, _D]),
+    "bml_multiply_ellblock_hip": (None, [_P, _P, _P, _D, _D, _D]),
+    "bml_multiply_AB_ellblock_hip": (None, [_P, _P, _P, _D]),
+    "bml_add_ellblock_hip": (None, [_P, _P, _D, _D, _D]),
+    "bml_threshold_ellblock_hip": (None, [_P, _D]),
+    "bml_trace_ellblock_hip": (_D, [_P]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Entry points declared in include/*.h (parsed from the headers)."""
+    import re
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    names = []
+    for f in sorted(os.listdir(inc)):
+        if f.endswith(".h"):
+            txt = open(os.path.join(inc, f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names += re.findall(r"\b(bml_\w+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def lib():
+    """Load libbml_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in {**_SIGS, **_OPTIONAL}.items():
+            if not hasattr(L, name):
+                if name in _SIGS:
+                    raise ImportError(f"{LIB_PATH} lacks {name}")
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L.bml_hip_set_error_mode(ERRORS_RECORD)
+        _libc.free.argtypes = [_P]
+        _lib = L
+    return _lib
+
+
+_libc = C.CDLL(None)
+
+
+def _check():
+    buf = C.create_string_buffer(1024)
+    code = lib().bml_hip_last_error(buf, 1024)
+    if code:
+        lib().bml_hip_clear_error()
+        raise BmlError(code, buf.value.decode())
+
+
+def _call(name, *args):
+    r = getattr(lib(), name)(*args)
+    _check()
+    return r
+
+
+def init(device: int = 0) -> None:
+    _call("bml_hip_init", device)
+
+
+def synchronize() -> None:
+    _call("bml_hip_synchronize")
+
+
+def stream() -> int:
+    return _call("bml_hip_stream")
+
+
+def set_sync_mode(mode: int) -> None:
+    lib().bml_hip_set_sync_mode(mode)
+
+
+class EllpackHip:
+    """Handle to a device-resident ellpack_hip matrix."""
+
+    def __init__(self, ptr):
+        if not ptr:
+            _check()
+            raise BmlError(2, "allocation returned NULL")
+        self.ptr = C.c_void_p(ptr)
+
+    @property
+    def _s(self) -> _Ell:
+        return C.cast(self.ptr, C.POINTER(_Ell)).contents
+
+    @property
+    def N(self) -> int:
+        return self._s.N
+
+    @property
+    def M(self) -> int:
+        return self._s.M
+
+    @property
+    def precision(self) -> int:
+        return self._s.matrix_precision
+
+    @property
+    def dtype(self):
+        return np.dtype(DTYPES[self.precision])
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().bml_deallocate_ellpack_hip(self.ptr)
+            self.ptr = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            if _lib is not None and self.ptr:
+                self.free()
+        except Exception:
+            pass
+
+    # host <-> device
+    def set(self, A: Ell) -> "EllpackHip":
+        if A.N != self.N or A.dtype != self.dtype:
+            raise BmlError(2, "shape/precision mismatch")
+        v = np.ascontiguousarray(A.value)
+        ix = np.ascontiguousarray(A.index, np.int32)
+        nn = np.ascontiguousarray(A.nnz, np.int32)
+        _call("bml_ellpack_hip_set_arrays", self.ptr, v.ctypes.data, ix.ctypes.data,
+              nn.ctypes.data, A.M)
+        return self
+
+    def get(self) -> Ell:
+        out = Ell.zeros(self.N, self.M, self.dtype)
+        _call("bml_ellpack_hip_get_arrays", self.ptr, out.value.ctypes.data,
+              out.index.ctypes.data, out.nnz.ctypes.data)
+        dead = ~out.live_mask()
+        out.value[dead] = 0
+        out.index[dead] = 0
+        return out
+
+    def row_stats(self):
+        mx = C.c_int(0)
+        tot = C.c_longlong(0)
+        _call("bml_ellpack_hip_row_stats", self.ptr, C.byref(mx), C.byref(tot))
+        return mx.value, tot.value
+
+
+def zero_matrix(precision: int, N: int, M: int) -> EllpackHip:
+    return EllpackHip(_call("bml_zero_matrix_ellpack_hip", precision, N, M, 0))
+
+
+def noinit_matrix(precision: int, N: int, M: int) -> EllpackHip:
+    return EllpackHip(_call("bml_noinit_matrix_ellpack_hip", precision, N, M, 0))
+
+
+def identity_matrix(precision: int, N: int, M: int) -> EllpackHip:
+    return EllpackHip(_call("bml_identity_matrix_ellpack_hip", precision, N, M, 0))
+
+
+def from_ell(A: Ell, M: int | None = None) -> EllpackHip:
+    h = zero_matrix(A.precision, A.N, M or A.M)
+    return h.set(A)
+
+
+def import_from_dense(dense: np.ndarray, M: int, threshold: float = 0.0,
+                      order: int = 0) -> EllpackHip:
+    d = np.ascontiguousarray(dense)
+    return EllpackHip(_call("bml_import_from_dense_ellpack_hip", precision_of(d.dtype), order,
+                            d.shape[0], M, d.ctypes.data, threshold, 0))
+
+
+def export_to_dense(A: EllpackHip, order: int = 0) -> np.ndarray:
+    p = _call("bml_export_to_dense_ellpack_hip", A.ptr, order)
+    n = A.N * A.N
+    buf = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_ubyte)), shape=(n * A.dtype.itemsize,))
+    out = np.frombuffer(buf.tobytes(), A.dtype).reshape(A.N, A.N)
+    _libc.free(p)
+    return out
+
+
+def copy_new(A: EllpackHip) -> EllpackHip:
+    return EllpackHip(_call("bml_copy_ellpack_hip_new", A.ptr))
+
+
+def copy(A: EllpackHip, B: EllpackHip) -> None:
+    _call("bml_copy_ellpack_hip", A.ptr, B.ptr)
+
+
+def clear(A: EllpackHip) -> None:
+    _call("bml_clear_ellpack_hip", A.ptr)
+
+
+def _take2(p):
+    arr = C.cast(p, C.POINTER(C.c_double))
+    out = (arr[0], arr[1])
+    _libc.free(p)
+    return out
+
+
+def multiply_x2(X: EllpackHip, X2: EllpackHip, threshold: float):
+    """X2 <- X*X; returns (tr X, tr X^2)."""
+    return _take2(_call("bml_multiply_x2_ellpack_hip", X.ptr, X2.ptr, threshold))
+
+
+def multiply(A: EllpackHip, B: EllpackHip, Cm: EllpackHip, alpha: float = 1.0,
+             beta: float = 0.0, threshold: float = 0.0) -> None:
+    _call("bml_multiply_ellpack_hip", A.ptr, B.ptr, Cm.ptr, alpha, beta, threshold)
+
+
+def multiply_AB(A: EllpackHip, B: EllpackHip, Cm: EllpackHip, threshold: float) -> None:
+    _call("bml_multiply_AB_ellpack_hip", A.ptr, B.ptr, Cm.ptr, threshold)
+
+
+def multiply_adjust_AB(A: EllpackHip, B: EllpackHip, Cm: EllpackHip, threshold: float) -> None:
+    _call("bml_multiply_adjust_AB_ellpack_hip", A.ptr, B.ptr, Cm.ptr, threshold)
+
+
+def add(A: EllpackHip, B: EllpackHip, alpha: float, beta: float, threshold: float) -> None:
+    _call("bml_add_ellpack_hip", A.ptr, B.ptr, alpha, beta, threshold)
+
+
+def add_norm(A: EllpackHip, B: EllpackHip, alpha: float, beta: float,
+             threshold: float) -> float:
+    return _call("bml_add_norm_ellpack_hip", A.ptr, B.ptr, alpha, beta, threshold)
+
+
+def add_identity(A: EllpackHip, beta: float, threshold: float) -> None:
+    _call("bml_add_identity_ellpack_hip", A.ptr, beta, threshold)
+
+
+def scale_add_identity(A: EllpackHip, alpha: float, beta: float, threshold: float) -> None:
+    _call("bml_scale_add_identity_ellpack_hip", A.ptr, alpha, beta, threshold)
+
+
+def scale_inplace(scale, A: EllpackHip) -> None:
+    s = np.array([scale], A.dtype)
+    _call("bml_scale_inplace_ellpack_hip", s.ctypes.data, A.ptr)
+
+
+def threshold(A: EllpackHip, thr: float) -> None:
+    _call("bml_threshold_ellpack_hip", A.ptr, thr)
+
+
+def threshold_new(A: EllpackHip, thr: float) -> EllpackHip:
+    return EllpackHip(_call("bml_threshold_new_ellpack_hip", A.ptr, thr))
+
+
+def trace(A: EllpackHip) -> float:
+    return _call("bml_trace_ellpack_hip", A.ptr)
+
+
+def gershgorin(A: EllpackHip):
+    return _take2(_call("bml_gershgorin_ellpack_hip", A.ptr))
+
+
+def normalize(A: EllpackHip, mineval: float, maxeval: float) -> None:
+    _call("bml_normalize_ellpack_hip", A.ptr, mineval, maxeval)

@@ -1,0 +1,205 @@
+/*
+ * bml_hip.h -- C ABI of the MI355X (gfx950) ELLPACK-R / ELLBLOCK matrix types
+ * `ellpack_hip` and `ellblock_hip`, drop-ins behind bml's C API.
+ *
+ * Every entry point below replaces the bml backend function cited beside it
+ * (paths relative to bml's src/C-interface/).  bml's public dispatchers
+ * (bml_multiply_x2, bml_multiply, bml_add, bml_threshold, bml_trace, ...)
+ * switch on the type tag read from the first int of the matrix
+ * (bml_introspection.c:22-35); the two structs below start with the same
+ * three tag fields as bml_matrix_ellpack_t (ellpack/bml_types_ellpack.h:13-18)
+ * so a `case ellpack_hip:` / `case ellblock_hip:` line in each dispatcher
+ * is all the integration needs.
+ *
+ * Conventions shared with bml:
+ *   - precision: bml_matrix_precision_t values (1 single_real, 2 double_real,
+ *     3 single_complex, 4 double_complex), bml_types.h:24-36;
+ *   - ELLPACK-R storage: value[N*M], index[N*M] row-major
+ *     (ROWMAJOR(i,jp,N,M) = i*M+jp, macros.h:13), nnz[N]; here the three
+ *     arrays live in device memory (HBM);
+ *   - thresholds compare |v| > threshold strictly (typed.h:36,73,112,150);
+ *   - errors are fatal: by default the library prints the message and exits
+ *     with EXIT_FAILURE, as LOG_ERROR does (bml_logger.c:111-115).  A caller
+ *     may select BML_HIP_ERRORS_RECORD to get a return to the caller with the
+ *     message kept for bml_hip_last_error() instead.
+ *   - results are returned the way bml returns them: bml_multiply_x2 and
+ *     bml_gershgorin hand back a malloc'd double[2] the caller frees;
+ *     exports hand back malloc'd host arrays.
+ *
+ * All device work runs on one HIP stream per process (default priority).
+ * With the default BML_HIP_SYNC_EACH_CALL every entry point returns with its
+ * work complete and its overflow checks done; BML_HIP_SYNC_DEFERRED lets
+ * void entry points return early, checks deferred to the next call that
+ * synchronises (an export, a trace, bml_hip_synchronize()).
+ */
+#ifndef BML_HIP_H
+#define BML_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* bml_matrix_type_t values for the new types, appended after distributed2d
+ * (= 5, bml_types.h:7-21) so the Fortran ids 0-4 stay unchanged. */
+#define BML_HIP_TYPE_ELLPACK 6
+#define BML_HIP_TYPE_ELLBLOCK 7
+
+/* bml_matrix_precision_t (bml_types.h:24-36) */
+#define BML_HIP_SINGLE_REAL 1
+#define BML_HIP_DOUBLE_REAL 2
+#define BML_HIP_SINGLE_COMPLEX 3
+#define BML_HIP_DOUBLE_COMPLEX 4
+
+/* bml_dense_order_t (bml_types.h:38-44) */
+#define BML_HIP_DENSE_ROW_MAJOR 0
+#define BML_HIP_DENSE_COLUMN_MAJOR 1
+
+/* error modes */
+#define BML_HIP_ERRORS_EXIT 0   /* print and exit(EXIT_FAILURE), as LOG_ERROR */
+#define BML_HIP_ERRORS_RECORD 1 /* keep the message, return to the caller */
+
+/* synchronisation modes */
+#define BML_HIP_SYNC_EACH_CALL 0
+#define BML_HIP_SYNC_DEFERRED 1
+
+/* error codes reported by bml_hip_last_error() */
+#define BML_HIP_OK 0
+#define BML_HIP_ERR_M_OVERFLOW 1   /* "Number of non-zeroes per row > M" */
+#define BML_HIP_ERR_ARGUMENT 2     /* bad argument, mismatched shapes/types */
+#define BML_HIP_ERR_DEVICE 3       /* HIP runtime failure */
+#define BML_HIP_ERR_ALLOC 4        /* device or host allocation failed */
+#define BML_HIP_ERR_UNSUPPORTED 5  /* precision/shape the backend lacks */
+
+struct bml_hip_aux;
+
+/* ELLPACK-R matrix resident in HBM.  Field order of the first five members
+ * is that of bml_matrix_ellpack_t (ellpack/bml_types_ellpack.h:11-24). */
+typedef struct bml_matrix_ellpack_hip_t {
+    int matrix_type;       /* BML_HIP_TYPE_ELLPACK */
+    int matrix_precision;  /* bml_matrix_precision_t */
+    int distribution_mode; /* bml_distribution_mode_t; sequential only */
+    int N;                 /* rows (and columns) */
+    int M;                 /* slots per row */
+    void *value;           /* device: N*M elements, row-major */
+    int *index;            /* device: N*M column indices */
+    int *nnz;              /* device: N live-slot counts */
+    struct bml_hip_aux *aux; /* library-private bookkeeping */
+} bml_matrix_ellpack_hip_t;
+
+/* ---- runtime ------------------------------------------------------------ */
+
+/* Select the device and create the library stream; called implicitly by the
+ * first allocation (device 0).  No bml counterpart (bml_init.c initialises
+ * MPI only). */
+int bml_hip_init(int device);
+void bml_hip_shutdown(void);
+void bml_hip_set_error_mode(int mode);
+void bml_hip_set_sync_mode(int mode);
+/* Last error code (BML_HIP_OK if none); copies the message into buf. */
+int bml_hip_last_error(char *buf, int len);
+void bml_hip_clear_error(void);
+/* Wait for all queued work and run the deferred overflow checks. */
+void bml_hip_synchronize(void);
+/* The hipStream_t every kernel of the library runs on. */
+void *bml_hip_stream(void);
+/* Library build string. */
+const char *bml_hip_version(void);
+
+/* ---- ELLPACK-R: allocation and conversion ------------------------------- */
+
+/* bml_zero_matrix_ellpack (ellpack/bml_allocate_ellpack.c, typed
+ * bml_allocate_ellpack_typed.c:190-262) */
+bml_matrix_ellpack_hip_t *bml_zero_matrix_ellpack_hip(
+    int precision, int N, int M, int distrib_mode);
+/* bml_noinit_matrix_ellpack (bml_allocate_ellpack_typed.c:146-188) */
+bml_matrix_ellpack_hip_t *bml_noinit_matrix_ellpack_hip(
+    int precision, int N, int M, int distrib_mode);
+/* bml_identity_matrix_ellpack (bml_allocate_ellpack_typed.c) */
+bml_matrix_ellpack_hip_t *bml_identity_matrix_ellpack_hip(
+    int precision, int N, int M, int distrib_mode);
+/* bml_deallocate_ellpack (ellpack/bml_allocate_ellpack.c) */
+void bml_deallocate_ellpack_hip(bml_matrix_ellpack_hip_t *A);
+/* bml_clear_ellpack: every row emptied */
+void bml_clear_ellpack_hip(bml_matrix_ellpack_hip_t *A);
+/* bml_copy_ellpack_new / bml_copy_ellpack (ellpack/bml_copy_ellpack_typed.c);
+ * B may have a different M as long as every row fits. */
+bml_matrix_ellpack_hip_t *bml_copy_ellpack_hip_new(const bml_matrix_ellpack_hip_t *A);
+void bml_copy_ellpack_hip(const bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B);
+/* bml_import_from_dense_ellpack (ellpack/bml_import_ellpack_typed.c:28-76):
+ * entries with |a| > threshold are kept, in column order. */
+bml_matrix_ellpack_hip_t *bml_import_from_dense_ellpack_hip(
+    int precision, int order, int N, int M, const void *A, double threshold,
+    int distrib_mode);
+/* bml_export_to_dense_ellpack (ellpack/bml_export_ellpack_typed.c:18-73):
+ * returns a malloc'd N*N host array. */
+void *bml_export_to_dense_ellpack_hip(const bml_matrix_ellpack_hip_t *A, int order);
+/* Upload/download the ELLPACK-R arrays of a host bml ellpack matrix
+ * (value/index/nnz of bml_matrix_ellpack_t, bml_types_ellpack.h:19-26).
+ * src_M is the row stride of the host arrays; rows must fit A->M. */
+void bml_ellpack_hip_set_arrays(bml_matrix_ellpack_hip_t *A, const void *value,
+                                const int *index, const int *nnz, int src_M);
+void bml_ellpack_hip_get_arrays(const bml_matrix_ellpack_hip_t *A, void *value,
+                                int *index, int *nnz);
+
+/* ---- ELLPACK-R: the thresholded multiply path --------------------------- */
+
+/* bml_multiply_x2_ellpack (ellpack/bml_multiply_ellpack_typed.c:130-349):
+ * X2 <- X*X; diagonal always stored, off-diagonals iff |v| > threshold;
+ * returns malloc'd double[2] = {tr X, tr X^2}. */
+void *bml_multiply_x2_ellpack_hip(bml_matrix_ellpack_hip_t *X,
+                                  bml_matrix_ellpack_hip_t *X2, double threshold);
+/* bml_multiply_ellpack (bml_multiply_ellpack_typed.c:55-118):
+ * C <- alpha*A*B + beta*C. */
+void bml_multiply_ellpack_hip(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
+                              bml_matrix_ellpack_hip_t *C, double alpha, double beta,
+                              double threshold);
+/* bml_multiply_AB_ellpack (bml_multiply_ellpack_typed.c:362-553): C <- A*B. */
+void bml_multiply_AB_ellpack_hip(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
+                                 bml_matrix_ellpack_hip_t *C, double threshold);
+/* bml_multiply_adjust_AB_ellpack (bml_multiply_ellpack_typed.c:566-707):
+ * C <- A*B, threshold doubled until every row fits C->M. */
+void bml_multiply_adjust_AB_ellpack_hip(bml_matrix_ellpack_hip_t *A,
+                                        bml_matrix_ellpack_hip_t *B,
+                                        bml_matrix_ellpack_hip_t *C, double threshold);
+/* bml_add_ellpack (ellpack/bml_add_ellpack_typed.c:50-216):
+ * A <- alpha*A + beta*B, every entry kept iff |v| > threshold. */
+void bml_add_ellpack_hip(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
+                         double alpha, double beta, double threshold);
+/* bml_add_norm_ellpack (bml_add_ellpack_typed.c:230-408): as add; returns
+ * sum over the union of (A - B)^2 before the update. */
+double bml_add_norm_ellpack_hip(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
+                                double alpha, double beta, double threshold);
+/* bml_add_identity_ellpack (bml_add_ellpack_typed.c:420-549) */
+void bml_add_identity_ellpack_hip(bml_matrix_ellpack_hip_t *A, double beta,
+                                  double threshold);
+/* bml_scale_add_identity_ellpack (bml_add_ellpack_typed.c:562-573) */
+void bml_scale_add_identity_ellpack_hip(bml_matrix_ellpack_hip_t *A, double alpha,
+                                        double beta, double threshold);
+/* bml_scale_inplace_ellpack (ellpack/bml_scale_ellpack_typed.c:93-130):
+ * scale_factor points to one element of A's precision. */
+void bml_scale_inplace_ellpack_hip(const void *scale_factor, bml_matrix_ellpack_hip_t *A);
+/* bml_threshold_ellpack (ellpack/bml_threshold_ellpack_typed.c:88-133) */
+void bml_threshold_ellpack_hip(bml_matrix_ellpack_hip_t *A, double threshold);
+/* bml_threshold_new_ellpack (bml_threshold_ellpack_typed.c:28-78) */
+bml_matrix_ellpack_hip_t *bml_threshold_new_ellpack_hip(const bml_matrix_ellpack_hip_t *A,
+                                                        double threshold);
+/* bml_trace_ellpack (ellpack/bml_trace_ellpack_typed.c:29-92) */
+double bml_trace_ellpack_hip(const bml_matrix_ellpack_hip_t *A);
+/* bml_gershgorin_ellpack (ellpack/bml_normalize_ellpack_typed.c:83-175):
+ * returns malloc'd double[2] = {min eigenvalue bound, max bound}. */
+void *bml_gershgorin_ellpack_hip(const bml_matrix_ellpack_hip_t *A);
+/* bml_normalize_ellpack (bml_normalize_ellpack_typed.c:33-45):
+ * A <- (maxeval*I - A)/(maxeval - mineval). */
+void bml_normalize_ellpack_hip(bml_matrix_ellpack_hip_t *A, double mineval,
+                               double maxeval);
+/* Row statistics of the last synchronised state: max and total nnz.
+ * (no bml counterpart; bml_get_sparsity is the nearest,
+ * ellpack/bml_getters_ellpack_typed.c) */
+void bml_ellpack_hip_row_stats(const bml_matrix_ellpack_hip_t *A, int *max_nnz,
+                               long long *total_nnz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BML_HIP_H */

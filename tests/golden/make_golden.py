@@ -1,0 +1,232 @@
+"""Writes the golden vectors under tests/golden/ from the real reference.
+
+Runs oracle/_ref/ref_driver (bml 2.4.0's own C sources, built by
+oracle/ref.mk in this container) on seeded inputs from bml_amd.generators and
+stores inputs and outputs as .npz fixtures.  Not run by the tests; rerun by
+hand after changing the cases:  python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from bml_amd import generators as G  # noqa: E402
+from bml_amd.ell import Blk, Ell  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+OUT = os.path.dirname(os.path.abspath(__file__))
+DTYPES = [np.float32, np.float64, np.complex64, np.complex128]
+TAG = {np.float32: "f32", np.float64: "f64", np.complex64: "c64", np.complex128: "c128"}
+
+
+def run(*args):
+    r = subprocess.run([DRIVER, *map(str, args)], capture_output=True, text=True)
+    return r.returncode, r.stdout, r.stderr
+
+
+def hexf(x):
+    return float(x).hex()
+
+
+def scalars(out, key):
+    for line in out.splitlines():
+        if line.startswith(key + " "):
+            return [float.fromhex(t) for t in line.split()[1:]]
+    return []
+
+
+def pack_ell(prefix, A, d):
+    if prefix not in ("A", "B", "C", "H"):
+        A = A.with_m(max(1, int(A.nnz.max(initial=0))))
+    d[prefix + "_value"] = A.value
+    d[prefix + "_index"] = A.index
+    d[prefix + "_nnz"] = A.nnz
+
+
+def pack_blk(prefix, A, d):
+    d[prefix + "_value"] = A.value
+    d[prefix + "_indexb"] = A.indexb
+    d[prefix + "_nnzb"] = A.nnzb
+    d[prefix + "_bsize"] = A.bsize
+
+
+def ellpack_cases(tmp):
+    cases = {}
+    for dt in DTYPES:
+        t = TAG[dt]
+        for (N, M, fill, seed, thr) in ((13, 13, 4, 11, 0.0), (64, 24, 6, 12, 0.05),
+                                         (120, 24, 6, 13, 0.02)):
+            A = G.random_ell(N, M, fill, seed, dt)
+            B = G.random_ell(N, M, fill, seed + 100, dt)
+            Cm = G.random_ell(N, M, fill, seed + 200, dt)
+            fa, fb, fc = (os.path.join(tmp, n) for n in ("a.ell", "b.ell", "c.ell"))
+            A.write(fa), B.write(fb), Cm.write(fc)
+            name = f"ell_{t}_N{N}"
+            d = {"N": N, "M": M, "threshold": thr}
+            pack_ell("A", A, d), pack_ell("B", B, d), pack_ell("C", Cm, d)
+            MO = min(N, M * M)
+            rc, out, _ = run("x2", fa, os.path.join(tmp, "o.ell"), MO, hexf(thr))
+            assert rc == 0
+            pack_ell("x2", Ell.read(os.path.join(tmp, "o.ell")), d)
+            d["x2_trace"] = np.array(scalars(out, "trace"))
+            rc, out, _ = run("ab", fa, fb, os.path.join(tmp, "o.ell"), MO, hexf(thr))
+            assert rc == 0
+            pack_ell("ab", Ell.read(os.path.join(tmp, "o.ell")), d)
+            for tag, (alpha, beta, same) in {"m1": (1.2, 0.8, False), "m2": (1.0, 0.0, True),
+                                             "m3": (0.7, 0.5, True), "m4": (-1.0, 0.0, False)}.items():
+                Cw = Cm.with_m(MO)
+                Cw.write(fc)
+                rc, out, _ = run("multiply", fa, "same" if same else fb, fc,
+                                 os.path.join(tmp, "o.ell"), hexf(alpha), hexf(beta), hexf(thr))
+                assert rc == 0
+                pack_ell(tag, Ell.read(os.path.join(tmp, "o.ell")), d)
+                d[tag + "_args"] = np.array([alpha, beta, float(same)])
+            Aw = A.with_m(2 * M)
+            Aw.write(os.path.join(tmp, "aw.ell"))
+            for tag, (alpha, beta) in {"add1": (1.2, 0.8), "add2": (0.0, 1.0), "add3": (2.0, -1.0)}.items():
+                rc, out, _ = run("add", os.path.join(tmp, "aw.ell"), fb, os.path.join(tmp, "o.ell"),
+                                 hexf(alpha), hexf(beta), hexf(thr))
+                assert rc == 0
+                pack_ell(tag, Ell.read(os.path.join(tmp, "o.ell")), d)
+                d[tag + "_args"] = np.array([alpha, beta])
+            rc, out, _ = run("add_norm", os.path.join(tmp, "aw.ell"), fb, os.path.join(tmp, "o.ell"),
+                             hexf(2.0), hexf(-1.0), hexf(thr))
+            assert rc == 0
+            pack_ell("addn", Ell.read(os.path.join(tmp, "o.ell")), d)
+            d["addn_norm"] = np.array(scalars(out, "norm"))
+            Ap = A.with_m(M + 1)
+            Ap.write(os.path.join(tmp, "ap.ell"))
+            rc, out, _ = run("add_identity", os.path.join(tmp, "ap.ell"), os.path.join(tmp, "o.ell"),
+                             hexf(0.75), hexf(thr))
+            assert rc == 0
+            pack_ell("addi", Ell.read(os.path.join(tmp, "o.ell")), d)
+            rc, out, _ = run("threshold", fa, os.path.join(tmp, "o.ell"), hexf(0.5))
+            assert rc == 0
+            pack_ell("thr", Ell.read(os.path.join(tmp, "o.ell")), d)
+            rc, out, _ = run("trace", fa)
+            d["trace"] = np.array(scalars(out, "trace"))
+            rc, out, _ = run("gershgorin", fa)
+            d["gersh"] = np.array(scalars(out, "gersh"))
+            cases[name] = d
+        # M overflow is fatal in the reference: record its exit status
+        A = G.random_ell(64, 24, 10, 99, dt)
+        A.write(os.path.join(tmp, "a.ell"))
+        rc, _, err = run("x2", os.path.join(tmp, "a.ell"), os.path.join(tmp, "o.ell"), 8, hexf(0.0))
+        d = {"N": 64, "M": 24, "M2": 8, "exit_code": rc}
+        pack_ell("A", A, d)
+        cases[f"ovf_{t}"] = d
+    return cases
+
+
+def ellblock_cases(tmp):
+    cases = {}
+    for dt in DTYPES:
+        t = TAG[dt]
+        # the reference's own block-size pattern (multiply_matrix_x2_typed.c:85-111)
+        for N, MB, fill, seed, thr, kind in ((13, 4, 2, 21, 0.0, "var"), (40, 6, 3, 22, 0.5, "var"),
+                                             (48, 5, 2, 23, 0.3, "b4")):
+            if kind == "var":
+                bs, c = [], 0
+                for i in range(N):
+                    b = (5 + i) % 6 + 1
+                    if c + b > N:
+                        b = N - c
+                    bs.append(b)
+                    c += b
+                    if c == N:
+                        break
+            else:
+                bs = [4] * (N // 4)
+            A = G.random_blk(bs, MB, fill, seed, dt, bmax=6)
+            B = G.random_blk(bs, MB, fill, seed + 100, dt, bmax=6)
+            Cm = G.random_blk(bs, MB, fill, seed + 200, dt, bmax=6)
+            fa, fb, fc = (os.path.join(tmp, n) for n in ("a.blk", "b.blk", "c.blk"))
+            name = f"blk_{t}_N{N}"
+            d = {"threshold": thr}
+            pack_blk("A", A, d), pack_blk("B", B, d), pack_blk("C", Cm, d)
+            NB = len(bs)
+            MBo = NB
+            Aw, Bw, Cw = (Blk(X.value, X.indexb, X.nnzb, X.bsize) for X in (A, B, Cm))
+            Aw = _with_mb(A, MBo)
+            Bw = _with_mb(B, MBo)
+            Cw = _with_mb(Cm, MBo)
+            Aw.write(fa), Bw.write(fb), Cw.write(fc)
+            rc, out, _ = run("eb_x2", fa, os.path.join(tmp, "o.blk"), MBo, hexf(thr))
+            assert rc == 0
+            pack_blk("x2", Blk.read(os.path.join(tmp, "o.blk")), d)
+            d["x2_trace"] = np.array(scalars(out, "trace"))
+            rc, out, _ = run("eb_ab", fa, fb, os.path.join(tmp, "o.blk"), MBo, hexf(thr))
+            assert rc == 0
+            pack_blk("ab", Blk.read(os.path.join(tmp, "o.blk")), d)
+            # bml_add_ellblock reuses a slot's block buffer for whatever block
+            # lands in that slot (bml_add_ellblock_typed.c:170-180), which
+            # overruns it when block sizes differ; with the variable sizes
+            # the reference aborts in free() or corrupts its heap, so the
+            # add-based operations are recorded for uniform sizes only.
+            if kind != "var":
+                for tag, (alpha, beta, same) in {"m1": (1.2, 0.8, False),
+                                                 "m3": (0.7, 0.5, True)}.items():
+                    rc, out, _ = run("eb_multiply", fa, "same" if same else fb, fc,
+                                     os.path.join(tmp, "o.blk"), hexf(alpha), hexf(beta),
+                                     hexf(thr))
+                    assert rc == 0
+                    pack_blk(tag, Blk.read(os.path.join(tmp, "o.blk")), d)
+                    d[tag + "_args"] = np.array([alpha, beta, float(same)])
+                rc, out, _ = run("eb_add", fa, fb, os.path.join(tmp, "o.blk"), hexf(1.2),
+                                 hexf(0.8), hexf(thr))
+                assert rc == 0
+                pack_blk("add", Blk.read(os.path.join(tmp, "o.blk")), d)
+            rc, out, _ = run("eb_threshold", fa, os.path.join(tmp, "o.blk"), hexf(0.9))
+            assert rc == 0
+            pack_blk("thr", Blk.read(os.path.join(tmp, "o.blk")), d)
+            rc, out, _ = run("eb_trace", fa)
+            d["trace"] = np.array(scalars(out, "trace"))
+            cases[name] = d
+    return cases
+
+
+def _with_mb(A, MB):
+    out = Blk.zeros(A.bsize, MB, A.dtype, A.bmax)
+    k = min(MB, A.MB)
+    out.value[:, :k] = A.value[:, :k]
+    out.indexb[:, :k] = A.indexb[:, :k]
+    out.nnzb[:] = A.nnzb
+    return out
+
+
+def sp2_cases(tmp):
+    cases = {}
+    for dt in (np.float64, np.float32):
+        H = G.lattice_hamiltonian(6, 6, 6, M=216, hopping=-0.1, disorder=0.01, dtype=dt)
+        H.write(os.path.join(tmp, "h.ell"))
+        rc, out, _ = run("sp2", os.path.join(tmp, "h.ell"), os.path.join(tmp, "o.ell"), 216,
+                         108.0, hexf(1e-5), hexf(1e-8), 30)
+        assert rc == 0
+        d = {"nocc": 108.0, "threshold": 1e-5, "tol": 1e-8, "maxit": 30, "M": 216}
+        pack_ell("H", H, d)
+        pack_ell("X", Ell.read(os.path.join(tmp, "o.ell")), d)
+        hist = [[float.fromhex(v) for v in l.split()[2:5]] + [int(l.split()[5])]
+                for l in out.splitlines() if l.startswith("iter ")]
+        d["hist"] = np.array(hist)
+        cases[f"sp2_{TAG[dt]}"] = d
+    return cases
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        sys.exit(f"{DRIVER} missing: run make -C oracle ref")
+    with tempfile.TemporaryDirectory() as tmp:
+        for fname, fn in (("ellpack.npz", ellpack_cases), ("ellblock.npz", ellblock_cases),
+                          ("sp2.npz", sp2_cases)):
+            cases = fn(tmp)
+            flat = {f"{k}/{kk}": np.asarray(vv) for k, v in cases.items() for kk, vv in v.items()}
+            np.savez_compressed(os.path.join(OUT, fname), **flat)
+            print(fname, len(cases), "cases", os.path.getsize(os.path.join(OUT, fname)), "bytes")
+
+
+if __name__ == "__main__":
+    main()

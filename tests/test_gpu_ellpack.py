@@ -1,0 +1,191 @@
+"""Parity of the gfx950 ELLPACK-R path (libbml_hip.so through its C ABI)
+with the CPU oracle and with the reference's golden vectors.
+
+The GPU kernels sum every output entry in the reference's order without FMA
+contraction and emit rows in the reference's first-touch column order, so
+real-typed results are required to be bit-identical; complex types too
+(componentwise multiply-add in the reference's formula).  Trace and norm
+sums are reduced in a different (fixed) order: relative 1e-12 in double,
+2e-5 where the reference sums in float.
+"""
+import numpy as np
+import pytest
+
+import orc
+from bml_amd import generators as G
+from bml_amd.ell import Ell, precision_of
+from helpers import (DTYPES, assert_ell_identical, ell_of, load_cases, rel_close)
+
+pytestmark = pytest.mark.gpu
+
+ELL = load_cases("ellpack.npz")
+ELL_CASES = sorted(k for k in ELL if k.startswith("ell_"))
+
+
+def ttol(dtype):
+    return 1e-12 if np.dtype(dtype) in (np.float64, np.complex128) else 2e-5
+
+
+def dev(hip, A: Ell, M=None):
+    return hip.from_ell(A, M)
+
+
+@pytest.mark.parametrize("case", ELL_CASES)
+def test_golden_vectors(hip, case):
+    """Every ELLPACK operation on the reference's own outputs."""
+    d = ELL[case]
+    A, B, Cm = ell_of(d, "A"), ell_of(d, "B"), ell_of(d, "C")
+    thr = float(d["threshold"])
+    N, M = A.N, A.M
+    MO = min(N, M * M)
+    p = A.precision
+    dA, dB = dev(hip, A), dev(hip, B)
+    X2 = hip.zero_matrix(p, N, MO)
+    tr = hip.multiply_x2(dA, X2, thr)
+    assert_ell_identical(X2.get(), ell_of(d, "x2"), f"{case} x2")
+    assert rel_close(tr[0], d["x2_trace"][0], ttol(A.dtype))
+    assert rel_close(tr[1], d["x2_trace"][1], ttol(A.dtype))
+    C = hip.zero_matrix(p, N, MO)
+    hip.multiply_AB(dA, dB, C, thr)
+    assert_ell_identical(C.get(), ell_of(d, "ab"), f"{case} ab")
+    for tag in ("m1", "m2", "m3", "m4"):
+        alpha, beta, same = d[tag + "_args"]
+        dC = dev(hip, Cm, MO)
+        hip.multiply(dA, dA if same else dB, dC, alpha, beta, thr)
+        assert_ell_identical(dC.get(), ell_of(d, tag), f"{case} {tag}")
+    for tag in ("add1", "add2", "add3"):
+        alpha, beta = d[tag + "_args"]
+        dAw = dev(hip, A, 2 * M)
+        hip.add(dAw, dB, alpha, beta, thr)
+        assert_ell_identical(dAw.get(), ell_of(d, tag), f"{case} {tag}")
+    dAw = dev(hip, A, 2 * M)
+    n = hip.add_norm(dAw, dB, 2.0, -1.0, thr)
+    assert_ell_identical(dAw.get(), ell_of(d, "addn"), f"{case} add_norm")
+    assert rel_close(n, float(d["addn_norm"][0]), 1e-5 if ttol(A.dtype) > 1e-12 else 1e-12)
+    dAp = dev(hip, A, M + 1)
+    hip.add_identity(dAp, 0.75, thr)
+    assert_ell_identical(dAp.get(), ell_of(d, "addi"), f"{case} add_identity")
+    dT = dev(hip, A)
+    hip.threshold(dT, 0.5)
+    assert_ell_identical(dT.get(), ell_of(d, "thr"), f"{case} threshold")
+    assert rel_close(hip.trace(dA), float(d["trace"][0]), ttol(A.dtype))
+    g = hip.gershgorin(dA)
+    assert g == (float(d["gersh"][0]), float(d["gersh"][1]))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(1, 1, 1), (7, 3, 2), (300, 40, 14), (1000, 64, 30)])
+def test_x2_matches_oracle(hip, dtype, shape):
+    N, M, fill = shape
+    A = G.random_ell(N, M, fill, 100 + N, dtype)
+    thr = 0.02
+    M2 = min(N, M * M)
+    ref, rtr = orc.x2(A, M2, thr)
+    X2 = hip.zero_matrix(A.precision, N, M2)
+    tr = hip.multiply_x2(dev(hip, A), X2, thr)
+    assert_ell_identical(X2.get(), ref, f"x2 {shape}")
+    assert rel_close(tr[0], rtr[0], 1e-13) and rel_close(tr[1], rtr[1], 1e-12)
+
+
+@pytest.mark.parametrize("group,table", [(8, 64), (16, 128), (32, 256), (64, 64), (64, 4096)])
+def test_x2_launch_shapes_agree(hip, monkeypatch, group, table):
+    """Every lanes-per-row / table-size choice gives the same bits; the tiny
+    tables push rows through the global-table pass."""
+    A = G.random_banded(N=1024, M=128, seed=7)
+    ref, _ = orc.x2(A, 512, 1e-5)
+    monkeypatch.setenv("BML_HIP_GROUP", str(group))
+    monkeypatch.setenv("BML_HIP_TABLE", str(table))
+    X2 = hip.zero_matrix(A.precision, A.N, 512)
+    hip.multiply_x2(dev(hip, A), X2, 1e-5)
+    assert_ell_identical(X2.get(), ref, f"group {group} table {table}")
+
+
+def test_x2_config1_shape(hip):
+    """N=4096, ~31 nnz/row within |i-j|<=256, threshold 1e-5 (M2=256: X^2
+    rows reach ~220 entries, more than the input's M=128)."""
+    A = G.random_banded()
+    ref, rtr = orc.x2(A, 256, 1e-5)
+    X2 = hip.zero_matrix(A.precision, A.N, 256)
+    tr = hip.multiply_x2(dev(hip, A), X2, 1e-5)
+    assert_ell_identical(X2.get(), ref, "config-1 x2")
+    assert rel_close(tr[0], rtr[0], 1e-13) and rel_close(tr[1], rtr[1], 1e-13)
+    C = hip.zero_matrix(A.precision, A.N, 256)
+    dA = dev(hip, A)
+    hip.multiply(dA, dA, C, 1.0, 0.0, 1e-5)
+    assert_ell_identical(C.get(), ref, "config-1 multiply")
+
+
+def test_m_overflow_raises(hip):
+    A = G.random_ell(64, 24, 10, 99, np.float64)
+    X2 = hip.zero_matrix(A.precision, 64, 8)
+    with pytest.raises(hip.BmlError) as e:
+        hip.multiply_x2(dev(hip, A), X2, 0.0)
+    assert e.value.code == hip.ERR_M_OVERFLOW
+    # the library stays usable after the error
+    X2b = hip.zero_matrix(A.precision, 64, 64)
+    hip.multiply_x2(dev(hip, A), X2b, 0.0)
+
+
+def test_ab_overflow_counts_touched_columns(hip):
+    """multiply_AB checks the pre-threshold row against C's M, as the
+    reference does (bml_multiply_ellpack_typed.c:505-510)."""
+    A = G.random_ell(50, 10, 8, 3)
+    B = G.random_ell(50, 10, 8, 4)
+    touched = max(len(set(np.concatenate([B.index[j, :B.nnz[j]] for j in A.index[i, :A.nnz[i]]])))
+                  for i in range(50))
+    C = hip.zero_matrix(A.precision, 50, touched - 1)
+    with pytest.raises(hip.BmlError):
+        hip.multiply_AB(dev(hip, A), dev(hip, B), C, 1e9)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_adjust_ab(hip, dtype):
+    A = G.random_ell(200, 20, 12, 8, dtype)
+    B = G.random_ell(200, 20, 12, 9, dtype)
+    ref, used = orc.adjust_ab(A, B, 40, 1e-3)
+    C = hip.zero_matrix(A.precision, 200, 40)
+    hip.multiply_adjust_AB(dev(hip, A), dev(hip, B), C, 1e-3)
+    assert_ell_identical(C.get(), ref, "adjust_AB")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_scale_add_identity_normalize_gershgorin(hip, dtype):
+    H = G.lattice_hamiltonian(5, 5, 4, M=12, dtype=dtype) if np.dtype(dtype).kind == "f" \
+        else G.peierls_hamiltonian(4, M=12, dtype=dtype)
+    d = dev(hip, H)
+    g = hip.gershgorin(d)
+    assert g == orc.gershgorin(H)
+    hip.normalize(d, g[0], g[1])
+    assert_ell_identical(d.get(), orc.normalize(H, g[0], g[1]), "normalize")
+    d2 = dev(hip, H)
+    hip.scale_add_identity(d2, 0.5, 0.25, 0.01)
+    assert_ell_identical(d2.get(), orc.scale_add_identity(H, 0.5, 0.25, 0.01),
+                         "scale_add_identity")
+
+
+def test_copy_clear_identity_dense_roundtrip(hip):
+    A = G.random_ell(90, 20, 9, 17, np.complex128)
+    d = dev(hip, A)
+    c = hip.copy_new(d)
+    assert_ell_identical(c.get(), A, "copy_new")
+    wide = hip.zero_matrix(A.precision, 90, 33)
+    hip.copy(d, wide)
+    assert_ell_identical(wide.get(), A, "copy to wider M")
+    D = hip.export_to_dense(d)
+    np.testing.assert_array_equal(D, A.to_dense())
+    e = hip.import_from_dense(D, 20, 0.0)
+    np.testing.assert_array_equal(hip.export_to_dense(e), D)
+    hip.clear(e)
+    assert e.get().nnz.sum() == 0
+    I = hip.identity_matrix(A.precision, 90, 4)
+    assert hip.trace(I) == 90.0
+
+
+def test_empty_rows_and_zero_threshold(hip):
+    A = G.random_ell(120, 16, 5, 23, np.float64, diag=False)
+    A.nnz[::3] = 0
+    ref, rtr = orc.x2(A, 120, 0.0)
+    X2 = hip.zero_matrix(A.precision, 120, 120)
+    tr = hip.multiply_x2(dev(hip, A), X2, 0.0)
+    assert_ell_identical(X2.get(), ref, "empty rows")
+    assert tr[0] == rtr[0] and rel_close(tr[1], rtr[1], 1e-14)

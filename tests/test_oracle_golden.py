@@ -1,0 +1,182 @@
+"""The CPU oracle against the golden vectors written by the real reference
+(tests/golden/make_golden.py -> oracle/_ref/ref_driver), and against closed
+forms the mathematics fixes."""
+import numpy as np
+import pytest
+
+import orc
+from bml_amd import generators as G
+from bml_amd.ell import Ell
+from helpers import (DTYPES, assert_blk_identical, assert_ell_identical, blk_of, ell_of,
+                     load_cases, rel_close)
+
+ELL = load_cases("ellpack.npz")
+BLK = load_cases("ellblock.npz")
+SP2 = load_cases("sp2.npz")
+ELL_CASES = sorted(k for k in ELL if k.startswith("ell_"))
+BLK_CASES = sorted(BLK)
+
+
+def trace_tol(dtype):
+    # the reference sums traces in the element type (float for single)
+    return 2e-5 if np.dtype(dtype).itemsize in (4, 8) and np.dtype(dtype) != np.float64 else 1e-12
+
+
+@pytest.mark.parametrize("case", ELL_CASES)
+def test_ellpack_ops_bit_identical(case):
+    d = ELL[case]
+    A, B, Cm = ell_of(d, "A"), ell_of(d, "B"), ell_of(d, "C")
+    thr = float(d["threshold"])
+    N, M = A.N, A.M
+    MO = min(N, M * M)
+    X2, tr = orc.x2(A, MO, thr)
+    assert_ell_identical(X2, ell_of(d, "x2"), "x2")
+    rt = d["x2_trace"]
+    tol = 1e-12 if A.dtype in (np.float64, np.complex128) else 2e-5
+    assert rel_close(tr[0], rt[0], tol) and rel_close(tr[1], rt[1], tol)
+    assert_ell_identical(orc.ab(A, B, MO, thr), ell_of(d, "ab"), "ab")
+    for tag in ("m1", "m2", "m3", "m4"):
+        alpha, beta, same = d[tag + "_args"]
+        got = orc.multiply(A, None if same else B, Cm.with_m(MO), alpha, beta, thr)
+        assert_ell_identical(got, ell_of(d, tag), tag)
+    Aw = A.with_m(2 * M)
+    for tag in ("add1", "add2", "add3"):
+        alpha, beta = d[tag + "_args"]
+        assert_ell_identical(orc.add(Aw, B, alpha, beta, thr), ell_of(d, tag), tag)
+    got, n = orc.add_norm(Aw, B, 2.0, -1.0, thr)
+    assert_ell_identical(got, ell_of(d, "addn"), "add_norm")
+    assert rel_close(n, float(d["addn_norm"][0]), 1e-5 if tol > 1e-12 else 1e-12)
+    assert_ell_identical(orc.add_identity(A.with_m(M + 1), 0.75, thr), ell_of(d, "addi"),
+                         "add_identity")
+    assert_ell_identical(orc.threshold(A, 0.5), ell_of(d, "thr"), "threshold")
+    assert rel_close(orc.trace(A), float(d["trace"][0]), tol)
+    g = orc.gershgorin(A)
+    assert g[0] == float(d["gersh"][0]) and g[1] == float(d["gersh"][1])
+
+
+@pytest.mark.parametrize("tag", ["f32", "f64", "c64", "c128"])
+def test_m_overflow_is_fatal(tag):
+    d = ELL[f"ovf_{tag}"]
+    assert int(d["exit_code"]) == 1  # the reference exits with EXIT_FAILURE
+    with pytest.raises(orc.OracleError):
+        orc.x2(ell_of(d, "A"), int(d["M2"]), 0.0)
+
+
+@pytest.mark.parametrize("case", BLK_CASES)
+def test_ellblock_ops_bit_identical(case):
+    d = BLK[case]
+    A, B, Cm = blk_of(d, "A"), blk_of(d, "B"), blk_of(d, "C")
+    thr = float(d["threshold"])
+    NB = A.NB
+
+    def w(X):
+        from bml_amd.ell import Blk
+        out = Blk.zeros(X.bsize, NB, X.dtype, X.bmax)
+        out.value[:, : X.MB] = X.value
+        out.indexb[:, : X.MB] = X.indexb
+        out.nnzb[:] = X.nnzb
+        return out
+
+    X2, tr = orc.eb_x2(w(A), NB, thr)
+    assert_blk_identical(X2, blk_of(d, "x2"), "eb_x2")
+    tol = 1e-12 if A.dtype in (np.float64, np.complex128) else 2e-5
+    # traces: the reference's tr X^2 strides the diagonal block by a list
+    # index (bml_multiply_ellblock_typed.c:413), right only for uniform sizes
+    assert rel_close(tr[0], float(d["x2_trace"][0]), tol)
+    if len(set(A.bsize.tolist())) == 1:
+        assert rel_close(tr[1], float(d["x2_trace"][1]), tol)
+    assert_blk_identical(orc.eb_ab(w(A), w(B), NB, thr), blk_of(d, "ab"), "eb_ab")
+    for tag in ("m1", "m3"):
+        if tag + "_args" not in d:
+            continue
+        alpha, beta, same = d[tag + "_args"]
+        got = orc.eb_multiply(w(A), None if same else w(B), w(Cm), alpha, beta, thr)
+        assert_blk_identical(got, blk_of(d, tag), tag)
+    if "add_value" in d:
+        assert_blk_identical(orc.eb_add(w(A), w(B), 1.2, 0.8, thr), blk_of(d, "add"), "eb_add")
+    assert_blk_identical(orc.eb_threshold(w(A), 0.9), blk_of(d, "thr"), "eb_threshold")
+    assert rel_close(orc.eb_trace(A), float(d["trace"][0]), tol)
+
+
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+def test_sp2_matches_reference_driven_sp2(tag):
+    """SP2 built from the oracle's operations equals SP2 built from the
+    reference's bml API (same algorithm, oracle/ref_driver.c)."""
+    d = SP2[f"sp2_{tag}"]
+    H = ell_of(d, "H")
+    X, it, hist = orc.sp2(H, int(d["M"]), float(d["nocc"]), float(d["threshold"]),
+                          float(d["tol"]), int(d["maxit"]))
+    ref_hist = d["hist"]
+    if tag == "f64":
+        assert it == len(ref_hist)
+        np.testing.assert_array_equal(hist[:, 3], ref_hist[:, 3])  # same branch sequence
+        assert_ell_identical(X, ell_of(d, "X"), "sp2 X")
+    else:
+        # single precision: the reference sums traces in float across OpenMP
+        # threads, the oracle in double, so branch choices agree only while
+        # the two candidate traces are apart by more than float rounding
+        np.testing.assert_array_equal(hist[:8, 3], ref_hist[:8, 3])
+        assert abs(hist[-1, 0] - ref_hist[-1, 0]) < 1e-3 * H.N
+
+
+# ---- closed forms and invariants -----------------------------------------
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_x2_equals_dense_product(dtype):
+    A = G.random_ell(80, 16, 6, 5, dtype)
+    D = A.to_dense()
+    X2, tr = orc.x2(A, 80, 0.0)
+    ref = D @ D
+    tol = 1e-12 if np.dtype(dtype).itemsize >= 8 and dtype != np.complex64 else 2e-5
+    assert np.abs(X2.to_dense() - ref).max() <= tol * max(1.0, np.abs(ref).max())
+    assert abs(tr[0] - np.trace(D).real) <= tol * 80
+    assert abs(tr[1] - np.trace(ref).real) <= tol * 80 * np.abs(ref).max()
+
+
+def test_threshold_keeps_only_large_and_diagonal():
+    A = G.random_banded(N=512, M=128, seed=3)
+    X2, _ = orc.x2(A, 512, 1e-2)
+    full, _ = orc.x2(A, 512, 0.0)
+    D = full.to_dense()
+    keep = (np.abs(D) > 1e-2) | np.eye(512, dtype=bool)
+    np.testing.assert_array_equal(X2.to_dense(), np.where(keep, D, 0.0))
+
+
+def test_trace_of_scaled_identity():
+    # the reference's trace test: tr(I) = N, tr(0.8 I) = 0.8 N
+    # (tests/C-tests/trace_matrix_typed.c:46-80)
+    N = 37
+    I = Ell.from_dense(np.eye(N), 4)
+    assert orc.trace(I) == N
+    assert abs(orc.trace(orc.scale_add_identity(I, 0.8, 0.0, 0.0)) - 0.8 * N) < 1e-12
+
+
+def test_sp2_is_the_spectral_projector():
+    """SP2 -> the projector onto the lowest nocc eigenvectors (numpy eigh)."""
+    H = G.lattice_hamiltonian(6, 6, 6, M=216, hopping=-0.1, disorder=0.01)
+    N = H.N
+    X, it, hist = orc.sp2(H, 216, N / 2, 0.0, 1e-10, 60)
+    w, V = np.linalg.eigh(H.to_dense())
+    P = V[:, : N // 2] @ V[:, : N // 2].T
+    assert hist[-1, 2] <= 1e-10
+    assert np.abs(X.to_dense() - P).max() < 1e-9
+    assert abs(orc.trace(X) - N / 2) < 1e-9
+
+
+def test_normalize_maps_spectrum_into_unit_interval():
+    H = G.lattice_hamiltonian(4, 4, 4, M=64)
+    lo, hi = orc.gershgorin(H)
+    w = np.linalg.eigvalsh(H.to_dense())
+    assert lo <= w.min() + 1e-12 and hi >= w.max() - 1e-12
+    X0 = orc.normalize(H.with_m(64), lo, hi)
+    w0 = np.linalg.eigvalsh(X0.to_dense())
+    assert w0.min() >= -1e-12 and w0.max() <= 1 + 1e-12
+    np.testing.assert_allclose(X0.to_dense(), (hi * np.eye(64) - H.to_dense()) / (hi - lo),
+                               atol=1e-15)
+
+
+def test_add_norm_is_frobenius_of_difference():
+    A = G.random_ell(50, 12, 5, 1)
+    B = G.random_ell(50, 12, 5, 2)
+    _, n = orc.add_norm(A.with_m(24), B, 1.0, 0.0, 0.0)
+    assert abs(n - np.sum((A.to_dense() - B.to_dense()) ** 2)) < 1e-12
