@@ -1,0 +1,189 @@
+"""Benchmark: GPU-resident SP2 purification on ELLPACK-R (thresholded X^2
+SpGEMM) on one MI355X.
+
+A step is one SP2 solve of the 32^3 lattice Hamiltonian (N = 32768, ELLPACK-R
+M = 256, threshold 1e-5, nocc = N/2, Gershgorin bounds, stop at
+||X^2 - X||_F <= 1e-8 or 60 iterations), the density matrix staying in HBM
+from the upload to the end.  The timed solves run in mixed precision (single
+precision iterations, the last 4 in double, traces and norms in double); the
+same number of double-precision solves is timed afterwards for the
+comparison.  Inputs are synthetic (seeded generator, bml_amd.generators).
+
+The lattice uses hopping -0.1 (+-0.01 disorder) instead of -0.5: at -0.5 the
+density matrix keeps ~9000 entries per row above 1e-5, so M = 256 overflows
+(the reference exits with "Number of non-zeroes per row > M").
+
+Prints one JSON line (rank 0) with the roofline of the dominant kernel
+(HIP-event durations inside the timed region) and the reference's CPU SP2
+iterations timed on the host (oracle/_ref, bml's own C code) as cpu_baseline.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+L, M, THR, TOL, MAXIT, REFINE = 32, 256, 1e-5, 1e-8, 60, 4
+HOP, DISORDER = -0.1, 0.01
+
+
+def bytes_alg(prof, p, s):
+    """Streamed-Gustavson bytes of the precision-p row-product launches in a
+    profile: A rows (s = element + index bytes per entry) and nnz, each
+    referenced B row plus its nnz entry, the C rows and nnz."""
+    return (s * prof.in_nnz[p] + 4 * prof.rows[p] + s * prof.products[p]
+            + 4 * prof.in_nnz[p] + s * prof.out_nnz[p] + 4 * prof.rows[p])
+
+
+def roofline(prof, p, s, name):
+    n = prof.launches[p]
+    if not n:
+        return None
+    avg_ms = prof.kernel_ms[p] / n
+    per_launch = bytes_alg(prof, p, s) / n
+    ach = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(avg_ms, 4), "launches": n,
+            "bytes_alg_per_launch": int(per_launch),
+            "products_per_launch": int(prof.products[p] / n),
+            "gflops": round(2.0 * prof.products[p] / (prof.kernel_ms[p] * 1e-3) / 1e9, 2)}
+
+
+def ell_max_diff(A, B):
+    """max |A - B| over the union of the two sparsity patterns."""
+    ra, ca, va = A.to_coo()
+    rb, cb, vb = B.to_coo()
+    ka, kb = ra * A.N + ca, rb * B.N + cb
+    keys = np.union1d(ka, kb)
+    xa = np.zeros(len(keys), va.dtype)
+    xb = np.zeros(len(keys), vb.dtype)
+    xa[np.searchsorted(keys, ka)] = va
+    xb[np.searchsorted(keys, kb)] = vb
+    return float(np.abs(xa - xb).max(initial=0.0))
+
+
+def time_solves(hip, dH, N, k, mixed):
+    X = hip.zero_matrix(dH.precision, N, M)
+    hip.synchronize()
+    hip.profile_begin()
+    t0 = time.perf_counter()
+    results = []
+    for _ in range(k):
+        r, hist = hip.sp2(dH, X, N / 2, threshold=THR, tol=TOL, maxit=MAXIT, mixed=mixed,
+                          refine_iters=REFINE)
+        results.append((r, hist))
+    hip.synchronize()
+    t1 = time.perf_counter()
+    prof = hip.profile_end()
+    return (t1 - t0), prof, results, X
+
+
+def cpu_baseline(H, N):
+    """The reference's SP2 iterations (bml_multiply_x2 + bml_add_norm, bml's
+    own OpenMP C code built from its sources in oracle/_ref) on the host."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    iters = 14
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores", OMP_STACKSIZE="64M")
+    with tempfile.TemporaryDirectory() as tmp:
+        f = os.path.join(tmp, "h.ell")
+        H.write(f)
+        out = subprocess.run([drv, "sp2_iters", f, str(M), str(N / 2), THR.hex(), str(iters)],
+                             capture_output=True, text=True, env=env, timeout=600)
+    if out.returncode != 0:
+        return None
+    t = [float(l.split()[2]) for l in out.stdout.splitlines() if l.startswith("iter_ms")]
+    steady = float(np.median(t[-6:]))
+    solve_ms = sum(t) + (MAXIT - iters) * steady
+    return {"value": 1000.0 / solve_ms, "unit": "solves/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"first {iters} of {MAXIT} FP64 SP2 iterations (bml_multiply_x2 + "
+                      f"bml_add_norm) of the same workload; solve time = their sum + "
+                      f"{MAXIT - iters} x median of the last 6 ({steady:.1f} ms)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from bml_amd import generators as G
+    from bml_amd import hip
+
+    hip.init(0)
+    H = G.lattice_hamiltonian(L, L, L, M=16, hopping=HOP, disorder=DISORDER)
+    N = H.N
+    dH = hip.from_ell(H)
+
+    for _ in range(args.warmup):
+        time_solves(hip, dH, N, 1, True)
+        time_solves(hip, dH, N, 1, False)
+
+    t_mix, p_mix, r_mix, X_mix = time_solves(hip, dH, N, args.steps, True)
+    t_dbl, p_dbl, r_dbl, X_dbl = time_solves(hip, dH, N, args.steps, False)
+    rm, hm = r_mix[-1]
+    rd, hd = r_dbl[-1]
+    diff = ell_max_diff(X_mix.get(), X_dbl.get())
+
+    # dominant kernel of the timed (mixed) region: the FP32 row product
+    roof = roofline(p_mix, 1, 8, "k_spgemm_lds<float> (FP32 SP2 iterations)")
+    roof64 = roofline(p_dbl, 2, 12, "k_spgemm_lds<double> (FP64 SP2 solves)")
+
+    value = args.steps / t_mix
+    fp64_solves = args.steps / t_dbl
+    line = {
+        "metric": "X^2 SpGEMM GFLOP/s + SP2 solves/s, mixed vs FP64 at ||X^2-X||_F <= 1e-8",
+        "value": round(value, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * t_mix / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"SP2 purification, ELLPACK-R, {L}^3 cubic lattice N={N}, onsite +-1 "
+                        f"checkerboard, hopping {HOP} +- {DISORDER} disorder, M={M}, "
+                        f"threshold {THR}, nocc N/2, Gershgorin, stop at ||X^2-X||_F<={TOL} "
+                        f"or {MAXIT} iterations; mixed = FP32 iterations, last {REFINE} FP64",
+            "N": N, "M": M, "threshold": THR, "maxit": MAXIT,
+        },
+        "fp64_solves_per_s": round(fp64_solves, 4),
+        "mixed_over_fp64": round(value / fp64_solves, 3),
+        "x2_fp64_gflops": roof64["gflops"] if roof64 else None,
+        "iterations": {"mixed": rm.iterations, "mixed_fp32": rm.single_iterations,
+                       "fp64": rd.iterations},
+        "final_idempotency": {"mixed": rm.idempotency, "fp64": rd.idempotency},
+        "trace_error": {"mixed": abs(rm.trace_x - N / 2), "fp64": abs(rd.trace_x - N / 2)},
+        "max_abs_diff_mixed_vs_fp64": diff,
+        "roofline": roof,
+        "roofline_fp64_x2": roof64,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(H, N),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,6 +24,12 @@ typedef struct bmlhip_status {
     int cursor;              /* work-queue cursor of the global-table pass */
     long long total_nnz;     /* entries written by the last op */
     double scalars[4];       /* trace X, trace X^2 / norm / gershgorin */
+    /* accumulated over a profiling window (not reset per operation), per
+     * precision 1..4 (index 0 unused) */
+    long long prof_products[5]; /* partial products of row-product kernels */
+    long long prof_rows[5];     /* rows those kernels produced */
+    long long prof_in_nnz[5];   /* entries of A's rows they read */
+    long long prof_out_nnz[5];  /* entries they wrote */
 } bmlhip_status;
 
 /* ---- runtime (hip/runtime.hip) ----------------------------------------- */
@@ -68,6 +74,7 @@ typedef struct bmlhip_spgemm_args {
     int *d_Ci, *d_Cn;
     double threshold;
     int group, table;
+    const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
 } bmlhip_spgemm_args;
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);
 
@@ -77,6 +84,42 @@ int bmlhip_spgemm(const bmlhip_spgemm_args *a);
 int bmlhip_add(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
                int MB, const void *d_Bv, const int *d_Bi, const int *d_Bn,
                double alpha, double beta, double threshold, int mode);
+/* as bmlhip_add with alpha = d_ab[0], beta = d_ab[1] read on the device and
+ * nothing done when *d_skip != 0 */
+int bmlhip_add_dev(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
+                   int MB, const void *d_Bv, const int *d_Bi, const int *d_Bn,
+                   const double *d_ab, double threshold, int mode, const int *d_skip);
+
+/* ---- profiling window ----------------------------------------------------- */
+/* begin: zero the prof_* counters and time every row-product launch with HIP
+ * events on the library stream; end: wait, return launches and total kernel
+ * milliseconds (the prof_* counters come with bmlhip_status_fetch). */
+int bmlhip_prof_begin(void);
+int bmlhip_prof_end(int launches[5], double kernel_ms[5]);
+/* event pair around one launch of a precision-prec kernel (launchers) */
+void bmlhip_prof_mark_begin(int prec);
+void bmlhip_prof_mark_end(void);
+
+/* ---- SP2 (hip/sp2.hip) ---------------------------------------------------- */
+/* Device state of one SP2 solve. */
+typedef struct bmlhip_sp2_state {
+    int it;          /* iterations done */
+    int done;        /* converged: later iterations are no-ops */
+    int branch;      /* last branch: 0 X <- X^2, 1 X <- 2X - X^2 */
+    int max_nnz;     /* X's longest row after the last update */
+    int max_touched; /* longest pre-threshold row of the last product */
+    int max_nnz_x2;  /* longest row of the last X^2 */
+    double ab[2];    /* alpha, beta of the pending add */
+    double trx, trx2, norm; /* last tr X, tr X^2, ||X - X^2||_F */
+} bmlhip_sp2_state;
+/* After x2 (status scalars 0,1 hold tr X, tr X^2): choose the branch that
+ * lands tr closer to nocc, store alpha/beta, record history[it*4..]. */
+int bmlhip_sp2_decide(bmlhip_sp2_state *d_state, double nocc, double *d_hist, int maxit);
+/* After add (status scalars[2] holds ||X - X^2||^2): record the norm, set
+ * done when it is <= tol, advance it. */
+int bmlhip_sp2_check(bmlhip_sp2_state *d_state, double tol, double *d_hist, int maxit);
+/* values of an ELLPACK-R matrix from one precision to another (same N, M) */
+int bmlhip_convert(int prec_in, int prec_out, size_t count, const void *d_in, void *d_out);
 /* A <- A + beta*I then threshold (add_identity); scale: values *= s first
  * (s_bits points to one element of the precision; NULL = no scaling). */
 int bmlhip_scale_add_identity(int prec, int N, int M, void *d_Av, int *d_Ai,

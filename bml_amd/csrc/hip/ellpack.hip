@@ -35,6 +35,13 @@ using namespace bmlhip;
 namespace {
 
 constexpr int EMPTY = -1;
+
+// bml_matrix_precision_t of an element type
+template <typename T> struct PrecOf;
+template <> struct PrecOf<float> { static constexpr int value = 1; };
+template <> struct PrecOf<double> { static constexpr int value = 2; };
+template <> struct PrecOf<cf32> { static constexpr int value = 3; };
+template <> struct PrecOf<cf64> { static constexpr int value = 4; };
 constexpr int FALLBACK_BLOCKS = 32;
 
 template <typename T>
@@ -65,7 +72,8 @@ struct SpParams {
     double thr;
     double *rowtr;  // [2*N]: per-row tr X and tr X^2 contributions (x2)
     bmlhip_status *st;
-    int *ovf;  // rows handed to the global-table pass
+    int *ovf;         // rows handed to the global-table pass
+    const int *skip;  // when set and nonzero, the launch does nothing
 };
 
 // One row per group.  keys/vals/ord point at the group's table (LDS or
@@ -87,6 +95,7 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
 
     int count = 0;      // distinct columns so far (group-uniform)
     bool tovf = false;  // table would overflow (group-uniform)
+    long long prods = 0;  // partial products of the row (group-uniform)
     T trx = E::zero();
 
     for (int c0 = 0; __any(c0 < nA); c0 += GROUP) {
@@ -103,6 +112,7 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
             const T a = shfl_any(aval, g * GROUP + t);
             const int j = __shfl(acol, g * GROUP + t, 64);
             const int nb = gact ? p.Bn[j] : 0;
+            prods += nb;
             if (gact && j == i) trx = E::add(trx, a);
             const size_t rowB = (size_t) j * p.MB;
             for (int k0 = 0; __any(gact && k0 < nb); k0 += GROUP) {
@@ -199,6 +209,11 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
         atomicMax(&p.st->max_nnz, stored);
         atomicMax(&p.st->max_touched, count);
         atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        const int pi = PrecOf<T>::value;
+        atomicAdd((unsigned long long *) &p.st->prof_products[pi], (unsigned long long) prods);
+        atomicAdd((unsigned long long *) &p.st->prof_rows[pi], 1ull);
+        atomicAdd((unsigned long long *) &p.st->prof_in_nnz[pi], (unsigned long long) nA);
+        atomicAdd((unsigned long long *) &p.st->prof_out_nnz[pi], (unsigned long long) stored);
         if (p.kind == 0) {
             p.rowtr[i] = E::re(trx);
             p.rowtr[p.N + i] = E::re(d2);
@@ -210,6 +225,7 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
 template <typename T, int GROUP>
 __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, int tbits) {
     extern __shared__ __align__(16) unsigned char smem[];
+    if (p.skip && *p.skip) return;
     constexpr int R = 64 / GROUP;
     const int tsize = 1 << tbits;
     T *vals = (T *) smem;
@@ -233,6 +249,7 @@ __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, int tbits) {
 template <typename T>
 __global__ __launch_bounds__(64) void k_spgemm_global(SpParams p, int tbits, unsigned char *tabs,
                                                       int *cursor) {
+    if (p.skip && *p.skip) return;
     const int n = p.st->table_overflow_rows;
     if (n == 0) return;
     const size_t tsize = (size_t) 1 << tbits;
@@ -312,6 +329,8 @@ struct AddParams {
     double alpha, beta, thr;
     double *rownorm;
     bmlhip_status *st;
+    const double *ab;  // when set: alpha = ab[0], beta = ab[1] (read on the device)
+    const int *skip;   // when set and nonzero, the launch does nothing
 };
 
 template <typename T>
@@ -324,8 +343,10 @@ __device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int 
     const T *Bv = (const T *) p.Bv;
     const size_t rA = (size_t) i * p.MA, rB = (size_t) i * p.MB;
     int count = 0;
+    const double alpha = p.ab ? p.ab[0] : p.alpha;
+    const double beta = p.ab ? p.ab[1] : p.beta;
     for (int part = 0; part < 2; part++) {
-        const double s = part ? p.beta : p.alpha;
+        const double s = part ? beta : alpha;
         if (p.mode == 0 && !(s > 0.0 || s < 0.0)) continue;
         const int n = part ? p.Bn[i] : p.An[i];
         for (int k0 = 0; k0 < n; k0 += 64) {
@@ -408,6 +429,7 @@ __device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int 
 template <typename T>
 __global__ __launch_bounds__(64) void k_add_lds(AddParams p, int tbits) {
     extern __shared__ __align__(16) unsigned char smem[];
+    if (p.skip && *p.skip) return;
     const int tsize = 1 << tbits;
     T *xs = (T *) smem;
     T *ys = xs + tsize;
@@ -424,6 +446,7 @@ __global__ __launch_bounds__(64) void k_add_lds(AddParams p, int tbits) {
 
 template <typename T>
 __global__ __launch_bounds__(64) void k_add_global(AddParams p, int tbits, unsigned char *tabs) {
+    if (p.skip && *p.skip) return;
     const size_t tsize = (size_t) 1 << tbits;
     unsigned char *base = tabs + blockIdx.x * tsize * (2 * sizeof(T) + 2 * sizeof(int));
     T *xs = (T *) base;
@@ -651,8 +674,11 @@ int launch_spgemm_lds(const SpParams &p, int tbits, hipStream_t s) {
         (void) hipFuncSetAttribute((const void *) k_spgemm_lds<T, GROUP>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
     const int blocks = (p.N + R - 1) / R;
-    if (blocks > 0)
+    if (blocks > 0) {
+        bmlhip_prof_mark_begin(PrecOf<T>::value);
         hipLaunchKernelGGL((k_spgemm_lds<T, GROUP>), dim3(blocks), dim3(64), lds, s, p, tbits);
+        bmlhip_prof_mark_end();
+    }
     return report("k_spgemm_lds");
 }
 
@@ -675,6 +701,7 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     p.Ci = a->d_Ci;
     p.Cn = a->d_Cn;
     p.thr = a->threshold;
+    p.skip = a->d_skip;
     p.st = bmlhip_status_dev();
     p.rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
     p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
@@ -717,7 +744,8 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
 
 template <typename T>
 int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv, const int *Bi,
-              const int *Bn, double alpha, double beta, double thr, int mode) {
+              const int *Bn, double alpha, double beta, double thr, int mode,
+              const double *abdev = nullptr, const int *skip = nullptr) {
     hipStream_t s = (hipStream_t) bmlhip_stream();
     AddParams p;
     p.N = N;
@@ -733,6 +761,8 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
     p.alpha = alpha;
     p.beta = beta;
     p.thr = thr;
+    p.ab = abdev;
+    p.skip = skip;
     p.st = bmlhip_status_dev();
     p.rownorm = (double *) bmlhip_scratch(0, sizeof(double) * (size_t) (N + 1));
     if (!p.rownorm) return 4;
@@ -858,6 +888,14 @@ int bmlhip_spgemm(const bmlhip_spgemm_args *a) {
 int bmlhip_add(int prec, int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
                const int *Bi, const int *Bn, double alpha, double beta, double thr, int mode) {
 #define CALL(T) add_typed<T>(N, MA, Av, Ai, An, MB, Bv, Bi, Bn, alpha, beta, thr, mode)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_add_dev(int prec, int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
+                   const int *Bi, const int *Bn, const double *ab, double thr, int mode,
+                   const int *skip) {
+#define CALL(T) add_typed<T>(N, MA, Av, Ai, An, MB, Bv, Bi, Bn, 0.0, 0.0, thr, mode, ab, skip)
     DISPATCH(prec, CALL)
 #undef CALL
 }

@@ -10,6 +10,10 @@
 namespace {
 
 struct Runtime {
+    int profiling = 0;
+    int prof_n = 0;
+    hipEvent_t prof_ev[2 * 4096];
+    int prof_prec[4096];
     int device = -1;
     hipStream_t stream = nullptr;
     bmlhip_status *status = nullptr;
@@ -175,6 +179,58 @@ int bmlhip_status_reset(void) {
 int bmlhip_status_clear_errors(void) {
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(64), 0, g.stream, g.status, 1);
     CHECK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_prof_reset(bmlhip_status *st) {
+    if (threadIdx.x < 5) {
+        st->prof_products[threadIdx.x] = 0;
+        st->prof_rows[threadIdx.x] = 0;
+        st->prof_in_nnz[threadIdx.x] = 0;
+        st->prof_out_nnz[threadIdx.x] = 0;
+    }
+}
+
+int bmlhip_prof_begin(void) {
+    hipLaunchKernelGGL(k_prof_reset, dim3(1), dim3(64), 0, g.stream, g.status);
+    CHECK(hipGetLastError());
+    g.profiling = 1;
+    g.prof_n = 0;
+    return 0;
+}
+
+void bmlhip_prof_mark_begin(int prec) {
+    if (!g.profiling || g.prof_n >= 4096) return;
+    hipEvent_t *e = &g.prof_ev[2 * g.prof_n];
+    if (hipEventCreate(&e[0]) != hipSuccess || hipEventCreate(&e[1]) != hipSuccess) return;
+    g.prof_prec[g.prof_n] = prec;
+    (void) hipEventRecord(e[0], g.stream);
+}
+
+void bmlhip_prof_mark_end(void) {
+    if (!g.profiling || g.prof_n >= 4096) return;
+    (void) hipEventRecord(g.prof_ev[2 * g.prof_n + 1], g.stream);
+    g.prof_n++;
+}
+
+int bmlhip_prof_end(int launches[5], double kernel_ms[5]) {
+    CHECK(hipStreamSynchronize(g.stream));
+    for (int p = 0; p < 5; p++) {
+        launches[p] = 0;
+        kernel_ms[p] = 0.0;
+    }
+    for (int k = 0; k < g.prof_n; k++) {
+        float ms = 0.0f;
+        const int p = g.prof_prec[k] >= 1 && g.prof_prec[k] <= 4 ? g.prof_prec[k] : 0;
+        if (hipEventElapsedTime(&ms, g.prof_ev[2 * k], g.prof_ev[2 * k + 1]) == hipSuccess) {
+            kernel_ms[p] += ms;
+            launches[p]++;
+        }
+        (void) hipEventDestroy(g.prof_ev[2 * k]);
+        (void) hipEventDestroy(g.prof_ev[2 * k + 1]);
+    }
+    g.profiling = 0;
+    g.prof_n = 0;
     return 0;
 }
 

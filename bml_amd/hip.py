@@ -51,6 +51,25 @@ class _Blk(C.Structure):
                 ("bsize", C.c_void_p), ("bsize_host", C.c_void_p), ("aux", C.c_void_p)]
 
 
+class Sp2Options(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("tol", C.c_double), ("maxit", C.c_int),
+                ("mixed", C.c_int), ("refine_iters", C.c_int), ("poll", C.c_int)]
+
+
+class Sp2Result(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("single_iterations", C.c_int),
+                ("converged", C.c_int), ("trace_x", C.c_double), ("trace_x2", C.c_double),
+                ("idempotency", C.c_double), ("emin", C.c_double), ("emax", C.c_double)]
+
+
+class Profile(C.Structure):
+    """Per precision (index 1..4): row-product launches, summed kernel ms and
+    the work they did."""
+    _fields_ = [("launches", C.c_int * 5), ("kernel_ms", C.c_double * 5),
+                ("products", C.c_longlong * 5), ("rows", C.c_longlong * 5),
+                ("in_nnz", C.c_longlong * 5), ("out_nnz", C.c_longlong * 5)]
+
+
 _lib = None
 _P = C.c_void_p
 _I = C.c_int
@@ -92,6 +111,10 @@ _SIGS = {
     "bml_gershgorin_ellpack_hip": (_P, [_P]),
     "bml_normalize_ellpack_hip": (None, [_P, _D, _D]),
     "bml_ellpack_hip_row_stats": (None, [_P, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
+    "bml_hip_sp2_defaults": (None, [C.POINTER(Sp2Options)]),
+    "bml_hip_sp2_ellpack": (None, [_P, _P, _D, C.POINTER(Sp2Options), C.POINTER(Sp2Result), _P]),
+    "bml_hip_profile_begin": (None, []),
+    "bml_hip_profile_end": (None, [C.POINTER(Profile)]),
 }
 _OPTIONAL = {
     "bml_block_matrix_ellblock_hip": (_P, [_I, _I, _I, _I, _P, _I]),
@@ -355,3 +378,27 @@ def gershgorin(A: EllpackHip):
 
 def normalize(A: EllpackHip, mineval: float, maxeval: float) -> None:
     _call("bml_normalize_ellpack_hip", A.ptr, mineval, maxeval)
+
+
+def sp2(H: EllpackHip, X: EllpackHip, nocc: float, threshold: float = 1e-5,
+        tol: float = 1e-8, maxit: int = 60, mixed: bool = False, refine_iters: int = 4,
+        poll: int = 1):
+    """X <- SP2 density matrix of H; returns (Sp2Result, history[it, 4])."""
+    o = Sp2Options()
+    lib().bml_hip_sp2_defaults(C.byref(o))
+    o.threshold, o.tol, o.maxit = threshold, tol, maxit
+    o.mixed, o.refine_iters, o.poll = int(mixed), refine_iters, poll
+    r = Sp2Result()
+    hist = np.zeros((maxit, 4))
+    _call("bml_hip_sp2_ellpack", H.ptr, X.ptr, nocc, C.byref(o), C.byref(r), hist.ctypes.data)
+    return r, hist[: r.iterations]
+
+
+def profile_begin() -> None:
+    _call("bml_hip_profile_begin")
+
+
+def profile_end() -> Profile:
+    p = Profile()
+    _call("bml_hip_profile_end", C.byref(p))
+    return p

@@ -198,6 +198,58 @@ void bml_normalize_ellpack_hip(bml_matrix_ellpack_hip_t *A, double mineval,
 void bml_ellpack_hip_row_stats(const bml_matrix_ellpack_hip_t *A, int *max_nnz,
                                long long *total_nnz);
 
+
+/* ---- SP2 purification ----------------------------------------------------
+ * No bml counterpart: client codes (PROGRESS, LATTE) build SP2 from
+ * bml_gershgorin, bml_normalize, bml_multiply_x2 and bml_add_norm; this runs
+ * the same steps resident on the GPU, the branch decision and convergence
+ * test included (Niklasson, Phys. Rev. B 66, 155115 (2002)). */
+typedef struct bml_hip_sp2_options {
+    double threshold;  /* drop |x| <= threshold in every product and update */
+    double tol;        /* stop when ||X - X^2||_F <= tol */
+    int maxit;         /* iteration cap */
+    int mixed;         /* 1: iterate in single precision, the last
+                          refine_iters iterations in double */
+    int refine_iters;  /* double-precision iterations at the end (mixed) */
+    int poll;          /* read the convergence flag every poll iterations */
+} bml_hip_sp2_options;
+
+typedef struct bml_hip_sp2_result {
+    int iterations;        /* iterations run */
+    int single_iterations; /* of which in single precision */
+    int converged;         /* ||X - X^2||_F <= tol was reached */
+    double trace_x;        /* tr X of the last iteration's input */
+    double trace_x2;       /* tr X^2 of the last iteration */
+    double idempotency;    /* ||X - X^2||_F of the last iteration */
+    double emin, emax;     /* Gershgorin bounds of H */
+} bml_hip_sp2_result;
+
+void bml_hip_sp2_defaults(bml_hip_sp2_options *opt);
+/* X <- density matrix of H for nocc electrons.  H and X are ellpack_hip
+ * matrices of equal N and precision (double_real or double_complex when
+ * opt->mixed); X->M bounds the fill.  history, if not NULL, receives
+ * opt->maxit * 4 doubles: tr X, tr X^2, ||X - X^2||_F, branch (0: X <- X^2,
+ * 1: X <- 2X - X^2) per iteration. */
+void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_hip_t *X,
+                         double nocc, const bml_hip_sp2_options *opt,
+                         bml_hip_sp2_result *res, double *history);
+
+/* ---- profiling window -----------------------------------------------------
+ * No bml counterpart (bml has no timers).  Between begin and end every
+ * ELLPACK row-product kernel launch is timed with HIP events on the library
+ * stream and its work is counted. */
+typedef struct bml_hip_profile {
+    /* per precision 1..4 (bml_matrix_precision_t; index 0 unused) */
+    int launches[5];         /* row-product launches timed */
+    double kernel_ms[5];     /* their summed durations */
+    long long products[5];   /* partial products (sum over rows of A of nnz of the rows of B) */
+    long long rows[5];       /* output rows produced */
+    long long in_nnz[5];     /* entries of A's rows read */
+    long long out_nnz[5];    /* entries written */
+} bml_hip_profile;
+void bml_hip_profile_begin(void);
+void bml_hip_profile_end(bml_hip_profile *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -399,6 +399,43 @@ int main(int argc, char **argv)
                      atoi(argv[8]), &X, 1);
         printf("iters %d\n", it);
         write_ell(argv[3], X);
+    } else if (!strcmp(op, "sp2_iters")) {
+        /* time K SP2 iterations (after Gershgorin + normalize) through the
+         * reference: sp2_iters <H> <M> <nocc> <thr> <K> */
+        NEED(7);
+        bml_matrix_ellpack_t *H = read_ell(argv[2], 0);
+        int M = atoi(argv[3]), K = atoi(argv[6]);
+        double nocc = num(argv[4]), thr = num(argv[5]);
+        bml_matrix_ellpack_t *X = bml_zero_matrix(ellpack, H->matrix_precision, H->N, M,
+                                                  sequential);
+        size_t es = elem_size(H->matrix_precision);
+        for (int i = 0; i < H->N; i++) {
+            memcpy((char *) X->value + es * (size_t) i * M,
+                   (char *) H->value + es * (size_t) i * H->M, es * H->nnz[i]);
+            memcpy(X->index + (size_t) i * M, H->index + (size_t) i * H->M,
+                   4 * (size_t) H->nnz[i]);
+            X->nnz[i] = H->nnz[i];
+        }
+        double *g = bml_gershgorin(H);
+        scale_values(X, -1.0 / (g[1] - g[0]));
+        bml_add_identity(X, g[1] / (g[1] - g[0]), 0.0);
+        bml_matrix_ellpack_t *X2 = bml_zero_matrix(ellpack, H->matrix_precision, H->N, M,
+                                                   sequential);
+        double *t = malloc(sizeof(double) * (K + 1));
+        for (int it = 0; it < K; it++) {
+            double t0 = now_ms();
+            double *tr = bml_multiply_x2(X, X2, thr);
+            int branch = fabs(tr[1] - nocc) < fabs(2.0 * tr[0] - tr[1] - nocc) ? 0 : 1;
+            bml_free_memory(tr);
+            if (branch)
+                bml_add_norm(X, X2, 2.0, -1.0, thr);
+            else
+                bml_add_norm(X, X2, 0.0, 1.0, thr);
+            t[it] = now_ms() - t0;
+            printf("iter_ms %d %.6f\n", it, t[it]);
+        }
+        qsort(t, K, sizeof(double), cmp_d);
+        printf("median_iter_ms %.6f\n", t[K / 2]);
     } else if (!strcmp(op, "bench_x2")) {
         NEED(6);
         bml_matrix_ellpack_t *X = read_ell(argv[2], 0);
