@@ -55,6 +55,8 @@ def roofline(prof, p, s, name):
             "avg_launch_ms": round(avg_ms, 4), "launches": n,
             "bytes_alg_per_launch": int(per_launch),
             "products_per_launch": int(prof.products[p] / n),
+            "touched_per_row": round(prof.touched[p] / max(prof.rows[p], 1), 1),
+            "table_overflow_rows_per_launch": round(prof.ovf_rows[p] / n, 1),
             "gflops": round(2.0 * prof.products[p] / (prof.kernel_ms[p] * 1e-3) / 1e9, 2)}
 
 
@@ -114,12 +116,42 @@ def cpu_baseline(H, N):
                       f"{MAXIT - iters} x median of the last 6 ({steady:.1f} ms)"}
 
 
+def traffic_probe(hip, dH, N):
+    """Kernels for the HBM-traffic counters: k_spgemm_lds<double> on the
+    density matrix after 20 SP2 iterations, then k_copy_rows moving that
+    matrix into one of wider rows (known bytes: it reads nnz and the live
+    entries, writes the same entries and nnz), to scale FETCH_SIZE and
+    WRITE_SIZE."""
+    X = hip.zero_matrix(dH.precision, N, M)
+    hip.sp2(dH, X, N / 2, threshold=THR, tol=TOL, maxit=20)
+    X2 = hip.zero_matrix(dH.precision, N, M)
+    hip.profile_begin()
+    for _ in range(3):
+        hip.multiply_x2(X, X2, THR)
+    prof = hip.profile_end()
+    W = hip.zero_matrix(dH.precision, N, M + 64)
+    for _ in range(3):
+        hip.copy(X, W)
+    hip.synchronize()
+    mx, nnz = X.row_stats()
+    es = 8
+    print(json.dumps({
+        "x2_bytes_alg_per_launch": int(bytes_alg(prof, 2, 12) / max(prof.launches[2], 1)),
+        "x2_launches": prof.launches[2],
+        "copy_read_bytes": int(4 * N + (es + 4) * nnz),
+        "copy_write_bytes": int(4 * N + (es + 4) * nnz),
+        "nnz": int(nnz), "N": N}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: 3 FP64 X^2 launches on a converged "
+                         "density matrix of the workload and 3 row copies of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -133,6 +165,10 @@ def main():
     H = G.lattice_hamiltonian(L, L, L, M=16, hopping=HOP, disorder=DISORDER)
     N = H.N
     dH = hip.from_ell(H)
+
+    if args.traffic_probe:
+        traffic_probe(hip, dH, N)
+        return
 
     for _ in range(args.warmup):
         time_solves(hip, dH, N, 1, True)

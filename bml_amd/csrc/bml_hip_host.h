@@ -17,6 +17,7 @@ struct bml_hip_aux {
     long long total_nnz;
     int touched_hint;      /* distinct columns per row seen the last time this
                               matrix was squared (0 = unknown) */
+    int hint_max_nnz;      /* its longest row at that time */
 };
 
 /* Record an error.  In BML_HIP_ERRORS_EXIT mode prints it the way bml's

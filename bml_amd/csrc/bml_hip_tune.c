@@ -2,53 +2,65 @@
  * Launch-shape choice for the ELLPACK row-product kernels.
  *
  * group (lanes per output row) follows the length of the rows of B that a
- * step streams: one GROUP-wide chunk per step when rows are short, so a
- * 64-lane wavefront carries 64/GROUP output rows.  The LDS table holds the
- * row's distinct columns at <= 3/4 load; its size comes from the distinct
- * count observed on the previous product of the same matrix when known,
- * else from the bound min(N, max_a*max_b), and is capped so one wavefront's
- * tables stay within 40 KiB (four wavefronts per CU at least).  Rows that
- * outgrow the table are finished by the global-table pass, so the cap costs
- * speed, never correctness.  Environment overrides: BML_HIP_GROUP,
- * BML_HIP_TABLE.
+ * step streams: a step holds three GROUP-wide chunks of a row of B in
+ * registers, so GROUP is the smallest power of two with 3*GROUP >= max_b and
+ * a 64-lane wavefront carries 64/GROUP output rows.
+ *
+ * The LDS table holds a row's distinct columns.  est, the largest distinct
+ * count per row, comes from the previous product of the same matrix when
+ * known (scaled by the squared growth of its rows since: a product row can
+ * grow as fast as (row length)^2), else from the bound min(N, max_a*max_b).
+ * The table gets max(3/2 est, est + max_b) + 64 slots in 4-slot buckets: a
+ * row stops (and goes to the global-table pass) once count + (row length of
+ * B) could pass the table, and at ~2/3 load the probe rounds per step (the
+ * longest probe among the 3*64 lanes of a step) stay low; a smaller table
+ * buys occupancy but costs more rounds (on the 32^3 lattice X^2, 1177
+ * distinct columns per row, 1792 slots ran 1.4x faster than 1536 and as
+ * fast as 2048).  One wavefront's tables are capped at 32 KiB so at least
+ * five wavefronts share a CU.  Rows that outgrow the table are finished by the global-table
+ * pass, so the cap costs speed, never correctness.
+ *
+ * Environment overrides: BML_HIP_GROUP, BML_HIP_TABLE, BML_HIP_BUCKET,
+ * BML_HIP_LDS_BUDGET (bytes per wavefront), BML_HIP_XCD_REMAP (0/1).
  */
 #include "bml_hip_tune.h"
 #include "bml_hip_host.h"
 
 #include <stdlib.h>
 
-static long long next_pow2(long long v)
-{
-    long long p = 64;
-    while (p < v)
-        p <<= 1;
-    return p;
-}
-
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
-                      int *group, int *table)
+                      int hint_max_a, bmlh_spgemm_shape *out)
 {
-    int g = max_b <= 8 ? 8 : max_b <= 16 ? 16 : max_b <= 32 ? 32 : 64;
+    int g = 8;
+    while (g < 64 && 3 * g < max_b)
+        g <<= 1;
     long long bound = (long long) max_a * (long long) max_b;
     if (bound > N)
         bound = N;
-    long long est = touched_hint > 0 ? touched_hint : bound;
-    if (est > bound && bound > 0)
-        est = bound;
-    long long t = next_pow2(est + est / 3 + g + 16);
+    long long est = bound;
+    if (touched_hint > 0) {
+        double grow = hint_max_a > 0 && max_a > hint_max_a ? (double) max_a / hint_max_a : 1.0;
+        est = (long long) (touched_hint * grow * grow) + 1;
+        if (est > bound)
+            est = bound;
+    }
+    long long t = est * 3 / 2;
+    if (t < est + max_b)
+        t = est + max_b;
+    t = (t + 64 + 63) & ~63ll;
     const long long slot = (long long) bmlh_elem_size(precision) + 4 + 2;
-    const long long budget = 40 * 1024;
+    const char *eb = getenv("BML_HIP_LDS_BUDGET");
+    const long long budget = eb ? atoll(eb) : 32 * 1024;
     /* more rows per wavefront only while their tables fit */
-    while (g < 64 && (64 / g) * t * slot > budget)
+    while (g < 64 && (64 / g) * (t + 8 + g) * slot > budget)
         g <<= 1;
-    while (t > 64 && (64 / g) * t * slot > budget)
-        t >>= 1;
-    const char *eg = getenv("BML_HIP_GROUP");
-    const char *et = getenv("BML_HIP_TABLE");
-    if (eg)
-        g = atoi(eg);
-    if (et)
-        t = atoll(et);
-    *group = g;
-    *table = (int) t;
+    while (t > 64 && (64 / g) * (t + 8 + g) * slot > budget)
+        t -= 64;
+    const char *e;
+    out->group = (e = getenv("BML_HIP_GROUP")) ? atoi(e) : g;
+    out->table = (e = getenv("BML_HIP_TABLE")) ? atoi(e) : (int) t;
+    out->bucket = (e = getenv("BML_HIP_BUCKET")) ? atoi(e) : 4;
+    /* contiguous rows per XCD cut the HBM fetch of the lattice X^2 5x but
+     * ran 1.3x slower than the hardware's round-robin order */
+    out->remap = (e = getenv("BML_HIP_XCD_REMAP")) ? atoi(e) : 0;
 }

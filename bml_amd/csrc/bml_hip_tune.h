@@ -4,11 +4,18 @@
 #ifndef BML_HIP_TUNE_H
 #define BML_HIP_TUNE_H
 
-/* Pick lanes per row (group) and LDS hash slots per row (table) for a
- * product whose left rows hold up to max_a entries and right rows up to
- * max_b; touched_hint is the largest distinct-column count seen the last
- * time (0 = unknown). */
+typedef struct bmlh_spgemm_shape {
+    int group;  /* lanes per output row */
+    int table;  /* LDS hash slots per row */
+    int bucket; /* slots per probe */
+    int remap;  /* XCD-aware block order */
+} bmlh_spgemm_shape;
+
+/* Launch shape for a product whose left rows hold up to max_a entries and
+ * right rows up to max_b; touched_hint is the largest distinct-column count
+ * seen the last time (0 = unknown), when the left rows held up to
+ * hint_max_a entries. */
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
-                      int *group, int *table);
+                      int hint_max_a, bmlh_spgemm_shape *out);
 
 #endif

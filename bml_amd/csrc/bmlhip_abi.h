@@ -30,6 +30,8 @@ typedef struct bmlhip_status {
     long long prof_rows[5];     /* rows those kernels produced */
     long long prof_in_nnz[5];   /* entries of A's rows they read */
     long long prof_out_nnz[5];  /* entries they wrote */
+    long long prof_ovf_rows[5]; /* rows finished by the global-table pass */
+    long long prof_touched[5];  /* distinct columns before thresholding */
 } bmlhip_status;
 
 /* ---- runtime (hip/runtime.hip) ----------------------------------------- */
@@ -73,7 +75,10 @@ typedef struct bmlhip_spgemm_args {
     void *d_Cv;
     int *d_Ci, *d_Cn;
     double threshold;
-    int group, table;
+    int group;   /* lanes per output row: 8, 16, 32 or 64 */
+    int table;   /* LDS hash-table slots per row (rounded up to a multiple of 8) */
+    int bucket;  /* slots per probe: 2, 4 or 8 */
+    int remap;   /* 1: XCD-aware block-to-row order */
     const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
 } bmlhip_spgemm_args;
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);

@@ -444,8 +444,14 @@ static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
     a.d_Ci = C->index;
     a.d_Cn = C->nnz;
     a.threshold = threshold;
+    bmlh_spgemm_shape sh;
     bmlh_pick_spgemm(a.prec, A->N, max_nnz_of(A), max_nnz_of(B),
-                     hint_owner ? hint_owner->touched_hint : 0, &a.group, &a.table);
+                     hint_owner ? hint_owner->touched_hint : 0,
+                     hint_owner ? hint_owner->hint_max_nnz : 0, &sh);
+    a.group = sh.group;
+    a.table = sh.table;
+    a.bucket = sh.bucket;
+    a.remap = sh.remap;
     return bmlh_ok(bmlhip_spgemm(&a), what);
 }
 
@@ -466,6 +472,7 @@ void *bml_multiply_x2_ellpack_hip(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpac
         return NULL;
     bmlh_set_stats(X2->aux, &st);
     X->aux->touched_hint = st.max_touched;
+    X->aux->hint_max_nnz = max_nnz_of(X);
     double *trace = malloc(2 * sizeof(double));
     if (!trace) {
         bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);

@@ -42,7 +42,10 @@ template <> struct PrecOf<float> { static constexpr int value = 1; };
 template <> struct PrecOf<double> { static constexpr int value = 2; };
 template <> struct PrecOf<cf32> { static constexpr int value = 3; };
 template <> struct PrecOf<cf64> { static constexpr int value = 4; };
-constexpr int FALLBACK_BLOCKS = 32;
+// global-table pass: at most this many blocks, and at most this many bytes
+// of tables in total
+constexpr int FALLBACK_BLOCKS = 256;
+constexpr size_t FALLBACK_BYTES = (size_t) 1 << 30;
 
 template <typename T>
 __device__ __forceinline__ T shfl_any(T v, int src) {
@@ -61,6 +64,7 @@ __device__ __forceinline__ unsigned hash_slot(int col, int tbits) {
     return ((unsigned) col * 0x9E3779B1u) >> (32 - tbits);
 }
 
+
 struct SpParams {
     int N, MA, MB, MC, kind;
     const void *Av;
@@ -74,12 +78,103 @@ struct SpParams {
     bmlhip_status *st;
     int *ovf;         // rows handed to the global-table pass
     const int *skip;  // when set and nonzero, the launch does nothing
+    int remap;        // XCD-aware block-to-row order
 };
 
+// Broadcast lane src's value to the caller's lanes.  With one group per
+// wavefront src is wave-uniform and a scalar readlane does it.
+template <int GROUP, typename T>
+__device__ __forceinline__ T bcast(T v, int src) {
+    if constexpr (GROUP == 64) {
+        static_assert(sizeof(T) % 4 == 0, "element must be a multiple of 4 bytes");
+        union U {
+            T t;
+            int w[sizeof(T) / 4];
+        } u, r;
+        u.t = v;
+#pragma unroll
+        for (int k = 0; k < (int) (sizeof(T) / 4); k++)
+            r.w[k] = __builtin_amdgcn_readlane(u.w[k], src);
+        return r.t;
+    } else {
+        return shfl_any(v, src);
+    }
+}
+
+// A step holds up to NCH GROUP-wide chunks of a row of B in registers.
+constexpr int NCH = 3;
+
+template <typename T>
+struct BRow {
+    int col[NCH];
+    T val[NCH];
+};
+
+// The table is an array of buckets of BK slots (BK = 2, 4 or 8 keys, read
+// with one or two 8/16-byte LDS accesses).  A column's home bucket comes
+// from a multiplicative hash scaled to the bucket count (any count: tables
+// need not be powers of two).  Tables in LDS (fewer than 2^16 buckets) use
+// 24-bit multiplies, full rate on CDNA; tables in global memory may be larger
+// and use a 64-bit product.
+template <bool BIG>
+__device__ __forceinline__ unsigned home_bucket(int col, unsigned nbk) {
+    if constexpr (BIG) {
+        const unsigned x = (unsigned) col * 0x9E3779B1u;
+        return (unsigned) (((unsigned long long) x * nbk) >> 32);
+    } else {
+        const unsigned x = (unsigned) __umul24(col, 0x9E3779u) >> 16;
+        return (unsigned) __umul24(x, nbk) >> 16;
+    }
+}
+
+template <int BK>
+struct Bucket {
+    int k[BK];
+};
+
+template <int BK>
+__device__ __forceinline__ Bucket<BK> load_bucket(const int *keys, unsigned at) {
+    Bucket<BK> b;
+    if constexpr (BK == 2) {
+        const int2 v = *reinterpret_cast<const int2 *>(keys + at);
+        b.k[0] = v.x;
+        b.k[1] = v.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < BK / 4; q++) {
+            const int4 v = *reinterpret_cast<const int4 *>(keys + at + 4 * q);
+            b.k[4 * q] = v.x;
+            b.k[4 * q + 1] = v.y;
+            b.k[4 * q + 2] = v.z;
+            b.k[4 * q + 3] = v.w;
+        }
+    }
+    return b;
+}
+
 // One row per group.  keys/vals/ord point at the group's table (LDS or
-// global); returns through the output arrays and the status block.
-template <typename T, int GROUP, typename ORD>
-__device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *ord, int tbits,
+// global): tsize slots (nbk = tsize/BK buckets), then a dummy bucket at slot
+// tsize that finished lanes read, then (from slot tsize + 8) one private
+// slot per lane that idle lanes aim their CAS and value accesses at, so that
+// every LDS access of the probe loop is unconditional (no exec-mask branches,
+// which would force a wait after each access) and conflict-free for idle
+// lanes.
+//
+// Per step (one entry (a, j) of row i) the chunks of row j of B are in
+// registers (loaded one step ahead, branch-free, into two buffers used
+// alternately).  Their columns are distinct, so all NCH chunks are probed
+// together: one bucket read per chunk per round, NCH reads in flight.
+// Insertion takes the first empty slot of a bucket and nothing is deleted
+// while a row is built, so the filled slots of every bucket form a prefix:
+// the first slot holding the column or EMPTY ends the search (a match, or
+// proof that the column is absent, and the lane claims that slot with CAS --
+// if another lane took it first the lane reads the bucket again); a bucket
+// with neither sends the lane to the next bucket (linear probing over
+// buckets).  New columns are ranked in (chunk, lane) = kp order, then the
+// products are added into their slots (plain read-modify-write: a step's
+// slots are distinct and steps follow program order).
+template <typename T, int GROUP, int BK, bool BIG, typename ORD>
+__device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *ord, unsigned tsize,
                            int cap) {
     using E = Elem<T>;
     const int lane = lane_id();
@@ -88,67 +183,155 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
     const uint64_t ltmask = (1ull << lane) - 1ull;
     const T *Av = (const T *) p.Av;
     const T *Bv = (const T *) p.Bv;
+    const int *Ai = p.Ai, *Bi = p.Bi, *Bn = p.Bn;
     const bool live = i >= 0;
     const int nA = live ? p.An[i] : 0;
-    const size_t rowA = (size_t) (live ? i : 0) * p.MA;
-    const unsigned tmask = (1u << tbits) - 1u;
+    const unsigned rowA = live ? (unsigned) i * (unsigned) p.MA : 0u;
+    const unsigned MB = (unsigned) p.MB;
+    const unsigned nbk = tsize / BK, dummy = tsize, priv = tsize + 8u + (unsigned) gl;
 
-    int count = 0;      // distinct columns so far (group-uniform)
-    bool tovf = false;  // table would overflow (group-uniform)
+    int count = 0;        // distinct columns so far (group-uniform)
+    bool tovf = false;    // table would overflow (group-uniform)
     long long prods = 0;  // partial products of the row (group-uniform)
     T trx = E::zero();
 
-    for (int c0 = 0; __any(c0 < nA); c0 += GROUP) {
-        const int jl = c0 + gl;
-        T aval = E::zero();
-        int acol = 0;
-        if (jl < nA) {
-            aval = Av[rowA + jl];
-            acol = p.Ai[rowA + jl];
+    // branch-free loads of row j's first NCH chunks (idle lanes read element 0)
+    auto load_b = [&](BRow<T> &r, int j, int nb, bool act) {
+        const unsigned base = (unsigned) j * MB;
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const int kp = c * GROUP + gl;
+            const unsigned at = (act & (kp < nb)) ? base + (unsigned) kp : 0u;
+            r.col[c] = Bi[at];
+            r.val[c] = Bv[at];
         }
-        const int steps = nA - c0 < GROUP ? nA - c0 : GROUP;
-        for (int t = 0; __any(t < steps && !tovf); t++) {
-            bool gact = t < steps && !tovf;
-            const T a = shfl_any(aval, g * GROUP + t);
-            const int j = __shfl(acol, g * GROUP + t, 64);
-            const int nb = gact ? p.Bn[j] : 0;
-            prods += nb;
-            if (gact && j == i) trx = E::add(trx, a);
-            const size_t rowB = (size_t) j * p.MB;
-            for (int k0 = 0; __any(gact && k0 < nb); k0 += GROUP) {
-                if (gact && count + GROUP > cap) {
-                    tovf = true;
-                    gact = false;
+    };
+
+    // the NCH chunks of a row of B in r (entries kbase.. of the row) times a
+    auto accumulate = [&](const BRow<T> &r, int kbase, int nb, bool gact, T a) {
+        // h: bucket being probed, dummy once the lane is done (or idle)
+        unsigned h[NCH], slot[NCH];
+        bool isnew[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const bool act = gact & (kbase + c * GROUP + gl < nb);
+            h[c] = act ? home_bucket<BIG>(r.col[c], nbk) * BK : dummy;
+            slot[c] = priv;
+            isnew[c] = false;
+        }
+        while (true) {
+            Bucket<BK> kb[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++) kb[c] = load_bucket<BK>(keys, h[c]);
+            unsigned at[NCH];
+            bool ins[NCH], anyins = false;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                const int col = r.col[c];
+                const bool pend = h[c] != dummy;
+                // first slot of the bucket holding col or EMPTY (BK: none)
+                unsigned sel = BK;
+                bool m = false;
+#pragma unroll
+                for (int e = BK - 1; e >= 0; e--) {
+                    const bool me = kb[c].k[e] == col, ee = kb[c].k[e] == EMPTY;
+                    sel = (me | ee) ? (unsigned) e : sel;
+                    m = me ? true : (ee ? false : m);
                 }
-                const int kp = k0 + gl;
-                const bool act = gact && kp < nb;
-                int col = 0, slot = 0;
-                bool isnew = false;
-                T b = E::zero();
-                if (act) {
-                    col = p.Bi[rowB + kp];
-                    b = Bv[rowB + kp];
-                    unsigned h = hash_slot(col, tbits);
-                    while (true) {
-                        const int kk = keys[h];
-                        if (kk == col) break;
-                        if (kk == EMPTY) {
-                            const int old = atomicCAS(&keys[h], EMPTY, col);
-                            if (old == EMPTY) {
-                                isnew = true;
-                                break;
-                            }
-                            if (old == col) break;
-                        }
-                        h = (h + 1u) & tmask;
-                    }
-                    slot = (int) h;
-                }
-                const uint64_t nm = ballot64(isnew) & gmask;
-                if (isnew) ord[count + popc64(nm & ltmask)] = (ORD) slot;
-                count += popc64(nm);
-                if (act) vals[slot] = E::add(vals[slot], E::mul(a, b));
+                const bool stop = sel < (unsigned) BK;
+                at[c] = h[c] + sel;
+                const bool hit = pend & stop & m;
+                ins[c] = pend & stop & !m;
+                slot[c] = hit ? at[c] : slot[c];
+                unsigned hn = h[c] + BK;
+                hn = hn >= tsize ? hn - tsize : hn;
+                h[c] = (hit | !pend) ? dummy : (ins[c] ? h[c] : hn);
+                anyins = anyins | ins[c];
             }
+            if (__any(anyins)) {
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+                    const int old = atomicCAS(&keys[ins[c] ? at[c] : priv], EMPTY, r.col[c]);
+                    const bool won = ins[c] & (old == EMPTY);
+                    isnew[c] = isnew[c] | won;
+                    slot[c] = won ? at[c] : slot[c];
+                    h[c] = won ? dummy : h[c];
+                }
+            }
+            bool anyp = false;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) anyp = anyp | (h[c] != dummy);
+            if (!__any(anyp)) break;
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const uint64_t nm = ballot64(isnew[c]) & gmask;
+            if (isnew[c]) ord[count + popc64(nm & ltmask)] = (ORD) slot[c];
+            count += popc64(nm);
+        }
+        // idle lanes add into their private slot
+        T v[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++) v[c] = vals[slot[c]];
+#pragma unroll
+        for (int c = 0; c < NCH; c++) vals[slot[c]] = E::add(v[c], E::mul(a, r.val[c]));
+    };
+
+    BRow<T> buf0, buf1;
+    for (int c0 = 0; __any(c0 < nA); c0 += GROUP) {
+        // GROUP entries of row i, and nnz of the rows they name
+        const int jl = c0 + gl;
+        const unsigned at = jl < nA ? rowA + (unsigned) jl : 0u;
+        const T av = Av[at];
+        const int aj = jl < nA ? Ai[at] : 0;
+        const int an = Bn[aj];
+        const int steps = nA - c0 < GROUP ? (nA - c0 > 0 ? nA - c0 : 0) : GROUP;
+
+        T a = bcast<GROUP>(av, g * GROUP);
+        int j = bcast<GROUP>(aj, g * GROUP);
+        int nb = bcast<GROUP>(an, g * GROUP);
+        load_b(buf0, j, nb, 0 < steps && !tovf);
+
+        // step t: accumulate row j (in cur) times a, prefetch step t+1 into nxt
+        auto step = [&](int t, BRow<T> &cur, BRow<T> &nxt) {
+            const int t1 = t + 1 < GROUP ? t + 1 : t;
+            const T a1 = bcast<GROUP>(av, g * GROUP + t1);
+            const int j1 = bcast<GROUP>(aj, g * GROUP + t1);
+            const int nb1 = bcast<GROUP>(an, g * GROUP + t1);
+            load_b(nxt, j1, nb1, t + 1 < steps && !tovf);
+            // a step adds at most nb columns and the table must keep an empty
+            // slot: a row that could fill it stops and goes to the
+            // global-table pass
+            bool gact = t < steps && !tovf;
+            if (gact && count + nb > cap) {
+                tovf = true;
+                gact = false;
+            }
+            if (gact) {
+                prods += nb;
+                if (j == i) trx = E::add(trx, a);
+            }
+            accumulate(cur, 0, nb, gact, a);
+            // rows of B longer than NCH*GROUP entries: the rest NCH chunks at a time
+            for (int k0 = NCH * GROUP; __any(gact && k0 < nb); k0 += NCH * GROUP) {
+                BRow<T> extra;
+                const unsigned base = (unsigned) j * MB;
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+                    const int kp = k0 + c * GROUP + gl;
+                    const unsigned at2 = gact && kp < nb ? base + (unsigned) kp : 0u;
+                    extra.col[c] = Bi[at2];
+                    extra.val[c] = Bv[at2];
+                }
+                accumulate(extra, k0, nb, gact, a);
+            }
+            a = a1;
+            j = j1;
+            nb = nb1;
+        };
+        for (int t = 0; __any(t < steps && !tovf); t += 2) {
+            step(t, buf0, buf1);
+            step(t + 1, buf1, buf0);
         }
     }
 
@@ -195,6 +378,7 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
         if (gl == 0) {
             const int q = atomicAdd(&p.st->table_overflow_rows, 1);
             p.ovf[q] = i;
+            atomicAdd((unsigned long long *) &p.st->prof_ovf_rows[PrecOf<T>::value], 1ull);
         }
     }
     if (rowact && gl == 0) {
@@ -214,6 +398,7 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
         atomicAdd((unsigned long long *) &p.st->prof_rows[pi], 1ull);
         atomicAdd((unsigned long long *) &p.st->prof_in_nnz[pi], (unsigned long long) nA);
         atomicAdd((unsigned long long *) &p.st->prof_out_nnz[pi], (unsigned long long) stored);
+        atomicAdd((unsigned long long *) &p.st->prof_touched[pi], (unsigned long long) count);
         if (p.kind == 0) {
             p.rowtr[i] = E::re(trx);
             p.rowtr[p.N + i] = E::re(d2);
@@ -221,43 +406,61 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
     }
 }
 
+// slots per table beyond tsize: the dummy bucket (8 slots) and one private
+// slot per lane of the group; a multiple of 8 keeps every table 16-byte
+// aligned
+template <int GROUP>
+constexpr unsigned table_pad() { return 8u + (unsigned) GROUP; }
+
 // First pass: table in LDS, 64/GROUP rows per 64-thread block.
-template <typename T, int GROUP>
-__global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, int tbits) {
+template <typename T, int GROUP, int BK>
+__global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (p.skip && *p.skip) return;
     constexpr int R = 64 / GROUP;
-    const int tsize = 1 << tbits;
+    const unsigned stride = tsize + table_pad<GROUP>();
     T *vals = (T *) smem;
-    int *keys = (int *) (smem + (size_t) R * tsize * sizeof(T));
-    uint16_t *ord = (uint16_t *) (smem + (size_t) R * tsize * (sizeof(T) + sizeof(int)));
-    for (int s = threadIdx.x; s < R * tsize; s += 64) {
+    int *keys = (int *) (smem + (size_t) R * stride * sizeof(T));
+    uint16_t *ord = (uint16_t *) (smem + (size_t) R * stride * (sizeof(T) + sizeof(int)));
+    for (unsigned s = threadIdx.x; s < R * stride; s += 64) {
         keys[s] = EMPTY;
         vals[s] = Elem<T>::zero();
     }
     __syncthreads();
+    // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8);
+    // with p.remap each XCD gets a contiguous range of rows, so the rows
+    // resident on one XCD at a time are neighbours and share the rows of B
+    // in its L2.
+    int vb = blockIdx.x;
+    if (p.remap) {
+        const int nblk = gridDim.x;
+        const int per_xcd = (nblk + 7) / 8;
+        const int xcd = blockIdx.x % 8, k = blockIdx.x / 8;
+        const int full = nblk % 8 == 0 ? 8 : nblk % 8;  // XCDs that get per_xcd blocks
+        vb = xcd < full ? xcd * per_xcd + k : full * per_xcd + (xcd - full) * (per_xcd - 1) + k;
+    }
     const int g = threadIdx.x / GROUP;
-    const int row = blockIdx.x * R + g;
+    const int row = vb * R + g;
     const int i = row < p.N ? row : -1;
-    const int cap = tsize - tsize / 4;
-    spgemm_row<T, GROUP, uint16_t>(p, i, keys + g * tsize, vals + (size_t) g * tsize,
-                                   ord + g * tsize, tbits, cap);
+    const int cap = (int) tsize - 8;
+    spgemm_row<T, GROUP, BK, false, uint16_t>(p, i, keys + g * stride, vals + (size_t) g * stride,
+                                              ord + g * stride, tsize, cap);
 }
 
 // Second pass: rows listed in p.ovf, one wavefront per row, table in global
-// memory (tsize = 2^tbits slots per block, at least twice the row's bound).
+// memory (tsize slots per block, at least twice the row's bound).
 template <typename T>
-__global__ __launch_bounds__(64) void k_spgemm_global(SpParams p, int tbits, unsigned char *tabs,
-                                                      int *cursor) {
+__global__ __launch_bounds__(64) void k_spgemm_global(SpParams p, unsigned tsize,
+                                                      unsigned char *tabs, int *cursor) {
     if (p.skip && *p.skip) return;
     const int n = p.st->table_overflow_rows;
     if (n == 0) return;
-    const size_t tsize = (size_t) 1 << tbits;
-    unsigned char *base = tabs + blockIdx.x * tsize * (sizeof(T) + 2 * sizeof(int));
+    const size_t stride = (size_t) tsize + table_pad<64>();
+    unsigned char *base = tabs + blockIdx.x * stride * (sizeof(T) + 2 * sizeof(int));
     T *vals = (T *) base;
-    int *keys = (int *) (base + tsize * sizeof(T));
-    int *ord = (int *) (base + tsize * (sizeof(T) + sizeof(int)));
-    for (size_t s = threadIdx.x; s < tsize; s += 64) {
+    int *keys = (int *) (base + stride * sizeof(T));
+    int *ord = (int *) (base + stride * (sizeof(T) + sizeof(int)));
+    for (size_t s = threadIdx.x; s < stride; s += 64) {
         keys[s] = EMPTY;
         vals[s] = Elem<T>::zero();
     }
@@ -267,7 +470,8 @@ __global__ __launch_bounds__(64) void k_spgemm_global(SpParams p, int tbits, uns
         if (threadIdx.x == 0) q = atomicAdd(cursor, 1);
         q = __shfl(q, 0, 64);
         if (q >= n) break;
-        spgemm_row<T, 64, int>(p, p.ovf[q], keys, vals, ord, tbits, (int) (tsize - tsize / 4));
+        // the table holds twice the row's bound, so a step never fills it
+        spgemm_row<T, 64, 2, true, int>(p, p.ovf[q], keys, vals, ord, tsize, (int) tsize - 2);
         __syncthreads();
     }
 }
@@ -663,28 +867,44 @@ int report(const char *what) {
     return fail_msg(3, "%s: %s", what, hipGetErrorString(e));
 }
 
-template <typename T, int GROUP>
-int launch_spgemm_lds(const SpParams &p, int tbits, hipStream_t s) {
+template <typename T, int GROUP, int BK>
+int launch_spgemm_lds(const SpParams &p, unsigned tsize, hipStream_t s) {
     constexpr int R = 64 / GROUP;
-    const size_t lds = (size_t) R * ((size_t) 1 << tbits) * (sizeof(T) + sizeof(int) + 2);
+    const size_t lds = (size_t) R * (tsize + table_pad<GROUP>()) * (sizeof(T) + sizeof(int) + 2);
     if (lds > 160 * 1024) {
         return fail_msg(2, "LDS table of %zu bytes exceeds 160 KiB", lds);
     }
     if (lds > 64 * 1024)
-        (void) hipFuncSetAttribute((const void *) k_spgemm_lds<T, GROUP>,
+        (void) hipFuncSetAttribute((const void *) k_spgemm_lds<T, GROUP, BK>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
     const int blocks = (p.N + R - 1) / R;
     if (blocks > 0) {
         bmlhip_prof_mark_begin(PrecOf<T>::value);
-        hipLaunchKernelGGL((k_spgemm_lds<T, GROUP>), dim3(blocks), dim3(64), lds, s, p, tbits);
+        hipLaunchKernelGGL((k_spgemm_lds<T, GROUP, BK>), dim3(blocks), dim3(64), lds, s, p, tsize);
         bmlhip_prof_mark_end();
     }
     return report("k_spgemm_lds");
 }
 
+template <typename T, int GROUP>
+int launch_spgemm_bk(const SpParams &p, unsigned tsize, int bucket, hipStream_t s) {
+    switch (bucket) {
+    case 2: return launch_spgemm_lds<T, GROUP, 2>(p, tsize, s);
+    case 4: return launch_spgemm_lds<T, GROUP, 4>(p, tsize, s);
+    case 8: return launch_spgemm_lds<T, GROUP, 8>(p, tsize, s);
+    default: return fail_msg(2, "bucket %d not one of 2/4/8", bucket);
+    }
+}
+
 template <typename T>
 int spgemm_typed(const bmlhip_spgemm_args *a) {
     hipStream_t s = (hipStream_t) bmlhip_stream();
+    // the row kernels hash columns with a 24-bit multiply and address rows
+    // with 32-bit element offsets
+    if (a->N >= (1 << 24) || (size_t) a->N * (size_t) (a->MA > a->MB ? a->MA : a->MB) >= (1ull << 32) ||
+        (size_t) a->N * (size_t) a->MC >= (1ull << 32))
+        return fail_msg(5, "row product: N=%d with M up to %d exceeds the 2^24 columns / 2^32 "
+                           "entries the kernels address", a->N, a->MA > a->MB ? a->MA : a->MB);
     SpParams p;
     p.N = a->N;
     p.MA = a->MA;
@@ -705,32 +925,39 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     p.st = bmlhip_status_dev();
     p.rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
     p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
+    p.remap = a->remap;
     // global-table pass: one table per block, >= 2x the row bound
     long long bound = (long long) a->MA * a->MB;
     if (bound > a->N) bound = a->N;
-    int gbits = ceil_log2(2 * bound + 64);
-    if (gbits < 10) gbits = 10;
-    const size_t gbytes =
-        (size_t) FALLBACK_BLOCKS * ((size_t) 1 << gbits) * (sizeof(T) + 2 * sizeof(int));
-    unsigned char *gtab = (unsigned char *) bmlhip_scratch(2, gbytes + 256);
+    long long gsize = 2 * bound + 64;
+    if (gsize < 1024) gsize = 1024;
+    gsize = (gsize + 7) & ~7ll;
+    const size_t per_block = ((size_t) gsize + table_pad<64>()) * (sizeof(T) + 2 * sizeof(int));
+    int gblocks = (int) (FALLBACK_BYTES / per_block);
+    if (gblocks > FALLBACK_BLOCKS) gblocks = FALLBACK_BLOCKS;
+    if (gblocks < 1) gblocks = 1;
+    unsigned char *gtab = (unsigned char *) bmlhip_scratch(2, (size_t) gblocks * per_block + 256);
     int *cursor = &p.st->cursor;
     if (!p.rowtr || !p.ovf || !gtab) {
         return fail_msg(4, "scratch allocation failed");
     }
-    int tbits = ceil_log2(a->table);
-    if (tbits < 6) tbits = 6;
+    // LDS table: a multiple of 8 slots, at least 64, below 2^16 buckets
+    long long tsize = ((long long) a->table + 7) & ~7ll;
+    if (tsize < 64) tsize = 64;
+    if (tsize > 65528) tsize = 65528;
     int rc = bmlhip_status_reset();
     if (rc) return rc;
+    const unsigned ts = (unsigned) tsize;
     switch (a->group) {
-    case 8: rc = launch_spgemm_lds<T, 8>(p, tbits, s); break;
-    case 16: rc = launch_spgemm_lds<T, 16>(p, tbits, s); break;
-    case 32: rc = launch_spgemm_lds<T, 32>(p, tbits, s); break;
-    case 64: rc = launch_spgemm_lds<T, 64>(p, tbits, s); break;
+    case 8: rc = launch_spgemm_bk<T, 8>(p, ts, a->bucket, s); break;
+    case 16: rc = launch_spgemm_bk<T, 16>(p, ts, a->bucket, s); break;
+    case 32: rc = launch_spgemm_bk<T, 32>(p, ts, a->bucket, s); break;
+    case 64: rc = launch_spgemm_bk<T, 64>(p, ts, a->bucket, s); break;
     default:
         return fail_msg(2, "group %d not one of 8/16/32/64", a->group);
     }
     if (rc) return rc;
-    hipLaunchKernelGGL(k_spgemm_global<T>, dim3(FALLBACK_BLOCKS), dim3(64), 0, s, p, gbits, gtab,
+    hipLaunchKernelGGL(k_spgemm_global<T>, dim3(gblocks), dim3(64), 0, s, p, (unsigned) gsize, gtab,
                        cursor);
     if ((rc = report("k_spgemm_global"))) return rc;
     if (a->kind == 0) {

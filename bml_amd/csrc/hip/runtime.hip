@@ -188,6 +188,8 @@ __global__ void k_prof_reset(bmlhip_status *st) {
         st->prof_rows[threadIdx.x] = 0;
         st->prof_in_nnz[threadIdx.x] = 0;
         st->prof_out_nnz[threadIdx.x] = 0;
+        st->prof_ovf_rows[threadIdx.x] = 0;
+        st->prof_touched[threadIdx.x] = 0;
     }
 }
 

@@ -52,7 +52,12 @@ static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *
     a.threshold = o->threshold;
     a.d_skip = &d_state->done;
     int mx = X->aux->stats_valid ? X->aux->max_nnz : X->M;
-    bmlh_pick_spgemm(a.prec, X->N, mx, mx, X->aux->touched_hint, &a.group, &a.table);
+    bmlh_spgemm_shape sh;
+    bmlh_pick_spgemm(a.prec, X->N, mx, mx, X->aux->touched_hint, X->aux->hint_max_nnz, &sh);
+    a.group = sh.group;
+    a.table = sh.table;
+    a.bucket = sh.bucket;
+    a.remap = sh.remap;
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
            bmlh_ok(bmlhip_add_dev(X->matrix_precision, X->N, X->M, X->value, X->index, X->nnz,
@@ -142,9 +147,11 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
         if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {
             if (!bmlh_ok(bmlhip_d2h(&st, d_state, sizeof st), what))
                 goto out;
+            /* the tuner reads max_nnz; the totals are refreshed at the end */
+            cur->aux->touched_hint = st.max_touched;
+            cur->aux->hint_max_nnz = cur->aux->stats_valid ? cur->aux->max_nnz : cur->M;
             cur->aux->stats_valid = 1;
             cur->aux->max_nnz = st.max_nnz;
-            cur->aux->touched_hint = st.max_touched;
             if (single)
                 r.single_iterations = st.it;
             if (st.done)
@@ -175,6 +182,7 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
         if (!bmlh_finish_sync(what, &s))
             goto out;
     }
+    X->aux->stats_valid = 0; /* row totals recomputed on demand */
     r.iterations = st.it;
     r.converged = st.done;
     r.trace_x = st.trx;
@@ -211,6 +219,8 @@ void bml_hip_profile_end(bml_hip_profile *out)
             p.rows[k] = st.prof_rows[k];
             p.in_nnz[k] = st.prof_in_nnz[k];
             p.out_nnz[k] = st.prof_out_nnz[k];
+            p.ovf_rows[k] = st.prof_ovf_rows[k];
+            p.touched[k] = st.prof_touched[k];
         }
     }
     if (out)

@@ -246,6 +246,8 @@ typedef struct bml_hip_profile {
     long long rows[5];       /* output rows produced */
     long long in_nnz[5];     /* entries of A's rows read */
     long long out_nnz[5];    /* entries written */
+    long long ovf_rows[5];   /* rows whose LDS table overflowed (second pass) */
+    long long touched[5];    /* distinct columns before thresholding */
 } bml_hip_profile;
 void bml_hip_profile_begin(void);
 void bml_hip_profile_end(bml_hip_profile *out);

@@ -87,17 +87,35 @@ def test_x2_matches_oracle(hip, dtype, shape):
     assert rel_close(tr[0], rtr[0], 1e-13) and rel_close(tr[1], rtr[1], 1e-12)
 
 
-@pytest.mark.parametrize("group,table", [(8, 64), (16, 128), (32, 256), (64, 64), (64, 4096)])
-def test_x2_launch_shapes_agree(hip, monkeypatch, group, table):
-    """Every lanes-per-row / table-size choice gives the same bits; the tiny
-    tables push rows through the global-table pass."""
+@pytest.mark.parametrize("group,table,bucket", [
+    (8, 64, 2), (16, 128, 4), (32, 256, 8), (64, 64, 8), (64, 4096, 2), (64, 1000, 4),
+    (64, 1544, 8), (16, 200, 8), (8, 72, 4)])
+@pytest.mark.parametrize("remap", [0, 1])
+def test_x2_launch_shapes_agree(hip, monkeypatch, group, table, bucket, remap):
+    """Every lanes-per-row / table-size / bucket-width / block-order choice
+    gives the same bits; the tiny tables push rows through the global-table
+    pass, and table sizes need not be powers of two."""
     A = G.random_banded(N=1024, M=128, seed=7)
     ref, _ = orc.x2(A, 512, 1e-5)
     monkeypatch.setenv("BML_HIP_GROUP", str(group))
     monkeypatch.setenv("BML_HIP_TABLE", str(table))
+    monkeypatch.setenv("BML_HIP_BUCKET", str(bucket))
+    monkeypatch.setenv("BML_HIP_XCD_REMAP", str(remap))
     X2 = hip.zero_matrix(A.precision, A.N, 512)
     hip.multiply_x2(dev(hip, A), X2, 1e-5)
-    assert_ell_identical(X2.get(), ref, f"group {group} table {table}")
+    assert_ell_identical(X2.get(), ref, f"group {group} table {table} bucket {bucket}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("bucket", [2, 4, 8])
+def test_x2_buckets_all_precisions(hip, monkeypatch, dtype, bucket):
+    A = G.random_ell(700, 48, 20, 31, dtype)
+    ref, rtr = orc.x2(A, 700, 0.01)
+    monkeypatch.setenv("BML_HIP_BUCKET", str(bucket))
+    X2 = hip.zero_matrix(A.precision, A.N, 700)
+    tr = hip.multiply_x2(dev(hip, A), X2, 0.01)
+    assert_ell_identical(X2.get(), ref, f"bucket {bucket}")
+    assert rel_close(tr[0], rtr[0], 1e-13) and rel_close(tr[1], rtr[1], 1e-12)
 
 
 def test_x2_config1_shape(hip):

@@ -35,7 +35,7 @@ def test_sp2_matches_reference_driven_golden(hip):
 @pytest.mark.parametrize("L,hop", [(8, -0.1), (10, -0.25)])
 def test_sp2_matches_oracle(hip, L, hop):
     H = G.lattice_hamiltonian(L, L, L, M=16, hopping=hop, disorder=0.1 * abs(hop))
-    M = min(H.N, 512)
+    M = H.N
     ref, it, rhist = orc.sp2(H, M, H.N / 2, 1e-5, 1e-8, 25)
     X, r, hist = run_sp2(hip, H, M, H.N / 2, threshold=1e-5, tol=1e-8, maxit=25)
     assert r.iterations == it
