@@ -116,31 +116,42 @@ def cpu_baseline(H, N):
                       f"{MAXIT - iters} x median of the last 6 ({steady:.1f} ms)"}
 
 
-def traffic_probe(hip, dH, N):
-    """Kernels for the HBM-traffic counters: k_spgemm_lds<double> on the
-    density matrix after 20 SP2 iterations, then k_copy_rows moving that
-    matrix into one of wider rows (known bytes: it reads nnz and the live
-    entries, writes the same entries and nnz), to scale FETCH_SIZE and
-    WRITE_SIZE."""
-    X = hip.zero_matrix(dH.precision, N, M)
-    hip.sp2(dH, X, N / 2, threshold=THR, tol=TOL, maxit=20)
-    X2 = hip.zero_matrix(dH.precision, N, M)
-    hip.profile_begin()
-    for _ in range(3):
-        hip.multiply_x2(X, X2, THR)
-    prof = hip.profile_end()
-    W = hip.zero_matrix(dH.precision, N, M + 64)
-    for _ in range(3):
-        hip.copy(X, W)
+def copy_calibration(hip):
+    """Three k_copy_rows launches of known HBM bytes for the FETCH_SIZE /
+    WRITE_SIZE counters: an FP64 ELLPACK matrix with N = 32768 full rows of
+    M = 1024 entries (403 MB, more than the 32 MB of L2 plus the 256 MB
+    Infinity Cache, so every copy streams from and to HBM) copied into one
+    with wider rows.  Each copy reads nnz and the N*M (value, index) entries
+    and writes the same."""
+    from bml_amd.ell import Ell
+    n, m = 32768, 1024
+    A = Ell(np.ones((n, m)), np.tile(np.arange(m, dtype=np.int32), (n, 1)),
+            np.full(n, m, dtype=np.int32))
+    dA = hip.from_ell(A)
+    del A
+    W = hip.zero_matrix(dA.precision, n, m + 64)
     hip.synchronize()
-    mx, nnz = X.row_stats()
-    es = 8
-    print(json.dumps({
-        "x2_bytes_alg_per_launch": int(bytes_alg(prof, 2, 12) / max(prof.launches[2], 1)),
-        "x2_launches": prof.launches[2],
-        "copy_read_bytes": int(4 * N + (es + 4) * nnz),
-        "copy_write_bytes": int(4 * N + (es + 4) * nnz),
-        "nnz": int(nnz), "N": N}), flush=True)
+    for _ in range(3):
+        hip.copy(dA, W)
+    hip.synchronize()
+    W.free()
+    dA.free()
+    return {"kernel": "k_copy_rows<double>", "launches": 3, "bytes_each_way": 4 * n + 12 * n * m}
+
+
+def profiled_traffic():
+    """HBM bytes per row-product launch measured with rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE on this same command, scaled by the copy
+    calibration (summary committed under profiles/)."""
+    f = os.path.join(ROOT, "profiles", "sp2_lattice32_traffic.md")
+    out = {}
+    if os.path.exists(f):
+        for line in open(f):
+            if line.startswith("traffic_bytes_per_launch"):
+                for kv in line.split()[1:]:
+                    k, v = kv.split("=")
+                    out[int(k[1:])] = int(v)
+    return out
 
 
 def main():
@@ -149,9 +160,6 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--traffic-probe", action="store_true",
-                    help="for rocprofv3 --pmc runs: 3 FP64 X^2 launches on a converged "
-                         "density matrix of the workload and 3 row copies of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -166,10 +174,6 @@ def main():
     N = H.N
     dH = hip.from_ell(H)
 
-    if args.traffic_probe:
-        traffic_probe(hip, dH, N)
-        return
-
     for _ in range(args.warmup):
         time_solves(hip, dH, N, 1, True)
         time_solves(hip, dH, N, 1, False)
@@ -183,6 +187,11 @@ def main():
     # dominant kernel of the timed (mixed) region: the FP32 row product
     roof = roofline(p_mix, 1, 8, "k_spgemm_lds<float> (FP32 SP2 iterations)")
     roof64 = roofline(p_dbl, 2, 12, "k_spgemm_lds<double> (FP64 SP2 solves)")
+    traffic = profiled_traffic()
+    for r, p in ((roof, 1), (roof64, 2)):
+        if r is not None and p in traffic:
+            r["traffic"] = traffic[p]
+    calib = copy_calibration(hip)
 
     value = args.steps / t_mix
     fp64_solves = args.steps / t_dbl
@@ -216,6 +225,7 @@ def main():
         "max_abs_diff_mixed_vs_fp64": diff,
         "roofline": roof,
         "roofline_fp64_x2": roof64,
+        "copy_calibration": calib,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(H, N),
     }
     print(json.dumps(line), flush=True)

@@ -180,3 +180,34 @@ def test_add_norm_is_frobenius_of_difference():
     B = G.random_ell(50, 12, 5, 2)
     _, n = orc.add_norm(A.with_m(24), B, 1.0, 0.0, 0.0)
     assert abs(n - np.sum((A.to_dense() - B.to_dense()) ** 2)) < 1e-12
+
+
+def test_adjust_ab_is_ab_at_the_first_doubled_threshold_that_fits():
+    """multiply_adjust_AB doubles the threshold until every row of the
+    product fits C's M (bml_multiply_ellpack_typed.c:566-707): its result is
+    the plain thresholded product at the returned threshold, and at half of
+    it some row would not fit."""
+    A = G.random_ell(200, 24, 10, 11)
+    B = G.random_ell(200, 24, 10, 12)
+    MC, thr = 40, 1e-3
+    got, used = orc.adjust_ab(A, B, MC, thr)
+    k = int(round(np.log2(used / thr)))
+    assert k >= 1 and used == thr * 2.0 ** k
+    ref = orc.ab(A, B, 200, used)
+    assert int(ref.nnz.max()) <= MC
+    assert_ell_identical(got, ref.with_m(MC), "adjust_ab")
+    assert int(orc.ab(A, B, 200, used / 2).nnz.max()) > MC
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_scale_add_identity_is_dense_alpha_a_plus_beta_i(dtype):
+    """alpha*A + beta*I, every entry kept iff |v| > threshold
+    (bml_add_ellpack_typed.c:420-573), against the dense formula."""
+    A = G.random_ell(60, 12, 6, 21, dtype)
+    thr = 0.05
+    got = orc.scale_add_identity(A.with_m(13), 0.7, 0.3, thr)
+    D = 0.7 * A.to_dense() + 0.3 * np.eye(60)
+    ref = np.where(np.abs(D) > thr, D, 0.0)
+    tol = 1e-6 if np.dtype(dtype).itemsize <= 8 and dtype != np.float64 else 1e-15
+    np.testing.assert_allclose(got.to_dense(), ref, atol=tol)
+    assert ((got.to_dense() != 0) == (ref != 0)).all()
