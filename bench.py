@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 L, M, THR, TOL, MAXIT, REFINE = 32, 256, 1e-5, 1e-8, 60, 4
 HOP, DISORDER = -0.1, 0.01
 
@@ -81,7 +82,7 @@ def time_solves(hip, dH, N, k, mixed):
     results = []
     for _ in range(k):
         r, hist = hip.sp2(dH, X, N / 2, threshold=THR, tol=TOL, maxit=MAXIT, mixed=mixed,
-                          refine_iters=REFINE)
+                          refine_iters=REFINE, mixed_switch=False)
         results.append((r, hist))
     hip.synchronize()
     t1 = time.perf_counter()
@@ -154,12 +155,117 @@ def profiled_traffic():
     return out
 
 
+def ellblock_sp2(hip, G):
+    """Config 3: SP2 on ELLBLOCK, 16x16x8 sites x 32 orbitals (N = 65536,
+    2048 block rows of 32), FP64 MFMA vs mixed (FP32 MFMA or bf16-input
+    MFMA until the single-precision iterate stalls, then FP64).  Onsite
+    shift +-6 and coupling 0.02 instead of +-2 and 0.1: with the configured
+    values the random 32x32 onsite blocks close the gap (0.003 on a 5x5x4
+    cut) and the density matrix fills the whole lattice; these keep a gap of
+    1.9 and 25 blocks per block row above 1e-5.  MB = 160 holds the X^2
+    blocks touched before thresholding (the reference's overflow test)."""
+    H = G.block_lattice_hamiltonian(16, 16, 8, b=32, MB=8, onsite_shift=6.0, coupling=0.02)
+    MB = 160
+    dH = hip.from_blk(H, MB)
+    out = {"workload": "SP2 on ELLBLOCK, 16x16x8 sites x 32 orbitals (N=65536, 32x32 blocks), "
+                       "onsite shift +-6, coupling 0.02, MB=160, block threshold 1e-5, "
+                       "stop at ||X^2-X||_F<=1e-8 or 60 iterations", "N": H.N}
+    runs = [("fp64", hip.BLOCK_ENGINE_MFMA, False), ("mixed_fp32", hip.BLOCK_ENGINE_MFMA, True),
+            ("mixed_bf16", hip.BLOCK_ENGINE_MFMA_BF16, True)]
+    for name, engine, mixed in runs:
+        hip.set_block_engine(engine)
+        X = hip.block_matrix(H.precision, H.bsize, MB)
+        hip.eb_sp2(dH, X, H.N / 2, threshold=THR, tol=TOL, maxit=MAXIT, mixed=mixed)  # warm-up
+        hip.synchronize()
+        reps = 3
+        hip.profile_begin()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r, _h = hip.eb_sp2(dH, X, H.N / 2, threshold=THR, tol=TOL, maxit=MAXIT, mixed=mixed)
+        hip.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        prof = hip.profile_end()
+        entry = {"solves_per_s": round(1.0 / dt, 3), "iterations": r.iterations,
+                 "single_iterations": r.single_iterations, "converged": bool(r.converged),
+                 "idempotency": r.idempotency, "trace_error": abs(r.trace_x - H.N / 2)}
+        for p, key in ((2, "fp64"), (1, "bf16" if engine == hip.BLOCK_ENGINE_MFMA_BF16 else "fp32")):
+            n = prof.launches[p]
+            if not n:
+                continue
+            flops = 2.0 * 32 ** 3 * prof.products[p]
+            tf = flops / (prof.kernel_ms[p] * 1e-3) / 1e12
+            entry[f"x2_{key}"] = {"kernel": "k_eb_numeric", "launches": n,
+                                  "avg_launch_ms": round(prof.kernel_ms[p] / n, 4),
+                                  "block_pairs_per_launch": int(prof.products[p] / n),
+                                  "tflops": round(tf, 2), "mfma_peak_tflops": MFMA_PEAK_TFLOPS[key],
+                                  "mfma_frac": round(tf / MFMA_PEAK_TFLOPS[key], 4)}
+        out[name] = entry
+        X.free()
+    hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
+    out["mixed_fp32_over_fp64"] = round(out["mixed_fp32"]["solves_per_s"] / out["fp64"]["solves_per_s"], 3)
+    return out
+
+
+def batched_sp2(hip, G, plain_sample=32):
+    """Config 4: 512 independent SP2 solves, N log-uniform in [256, 8192]
+    (near-cubic lattices), batched into one block-diagonal matrix: FP64 and
+    mixed (FP32, last 4 iterations FP64), against the plain FP64 path (one
+    bml_hip_sp2_ellpack solve after another, timed on the first
+    plain_sample systems and scaled to 512).  Hopping -0.1 and M = 256 as in
+    config 2 (at -0.5 the rows overflow M); nocc = floor(N/2)."""
+    from bml_amd import generators as Gm
+    sizes = Gm.batch_sizes(512)
+    Hs = []
+    for n in sizes:
+        Lx, Ly, Lz = Gm.near_cubic(int(n))
+        Hs.append(Gm.lattice_hamiltonian(Lx, Ly, Lz, M=16, hopping=HOP, disorder=0.0))
+    Mb = 256
+    dHs = [hip.from_ell(H) for H in Hs]
+    nocc = [float(H.N // 2) for H in Hs]
+    rows = int(sum(H.N for H in Hs))
+    out = {"workload": f"512 SP2 solves, N log-uniform in [256, 8192] on near-cubic lattices "
+                       f"(sum N = {rows}), hopping {HOP}, onsite +-1, M={Mb}, threshold {THR}, "
+                       f"stop at ||X^2-X||_F<={TOL} or {MAXIT} iterations", "systems": len(Hs),
+           "rows": rows}
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        Xs = [hip.zero_matrix(H.precision, H.N, Mb) for H in Hs]
+        hip.sp2_batch(dHs, Xs, nocc, threshold=THR, tol=TOL, maxit=MAXIT, mixed=mixed)  # warm-up
+        hip.synchronize()
+        t0 = time.perf_counter()
+        res = hip.sp2_batch(dHs, Xs, nocc, threshold=THR, tol=TOL, maxit=MAXIT, mixed=mixed)
+        hip.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"solves_per_s": round(len(Hs) / dt, 2), "seconds": round(dt, 3),
+                     "converged": int(sum(r.converged for r in res)),
+                     "max_trace_error": max(abs(r.trace_x - c) for r, c in zip(res, nocc))}
+        for X in Xs:
+            X.free()
+    # plain FP64 path: one solve after another
+    sub = list(range(plain_sample))
+    Xp = [hip.zero_matrix(Hs[k].precision, Hs[k].N, Mb) for k in sub]
+    hip.sp2(dHs[0], Xp[0], nocc[0], threshold=THR, tol=TOL, maxit=MAXIT)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for k in sub:
+        hip.sp2(dHs[k], Xp[k], nocc[k], threshold=THR, tol=TOL, maxit=MAXIT)
+    hip.synchronize()
+    dt = time.perf_counter() - t0
+    share = sum(Hs[k].N for k in sub) / rows
+    out["plain_fp64"] = {"solves_per_s": round(len(sub) / dt, 2),
+                         "sample": f"first {len(sub)} systems ({100 * share:.1f}% of the rows)"}
+    out["mixed_batched_over_plain_fp64"] = round(out["mixed"]["solves_per_s"] /
+                                                 out["plain_fp64"]["solves_per_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="skip the ELLBLOCK (config 3) and batched (config 4) measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -226,6 +332,8 @@ def main():
         "roofline": roof,
         "roofline_fp64_x2": roof64,
         "copy_calibration": calib,
+        "ellblock_sp2": None if args.only_headline else ellblock_sp2(hip, G),
+        "batched_sp2": None if args.only_headline else batched_sp2(hip, G),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(H, N),
     }
     print(json.dumps(line), flush=True)

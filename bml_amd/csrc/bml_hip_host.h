@@ -38,6 +38,10 @@ int bmlh_finish_sync(const char *what, bmlhip_status *st);
 int bmlh_sync_mode(void);
 
 size_t bmlh_elem_size(int precision);
+/* block-product engine for products of A (BML_HIP_BLOCK_ENGINE_*, resolved) */
+int bmlh_block_engine_for(const bml_matrix_ellblock_hip_t *A);
+/* the real scalar s as one element of precision prec, into buf (16 bytes) */
+void bmlh_put_scalar(int prec, double s, void *buf);
 struct bml_hip_aux *bmlh_aux_new(void);
 void bmlh_set_stats(struct bml_hip_aux *aux, const bmlhip_status *st);
 

@@ -104,6 +104,31 @@ size_t bmlh_elem_size(int precision)
     return 0;
 }
 
+/* T-typed scalar s for precision prec into buf */
+void bmlh_put_scalar(int prec, double s, void *buf)
+{
+    switch (prec) {
+    case BML_HIP_SINGLE_REAL: {
+        float f = (float) s;
+        memcpy(buf, &f, 4);
+        break;
+    }
+    case BML_HIP_DOUBLE_REAL:
+        memcpy(buf, &s, 8);
+        break;
+    case BML_HIP_SINGLE_COMPLEX: {
+        float f[2] = { (float) s, 0.0f };
+        memcpy(buf, f, 8);
+        break;
+    }
+    case BML_HIP_DOUBLE_COMPLEX: {
+        double d[2] = { s, 0.0 };
+        memcpy(buf, d, 16);
+        break;
+    }
+    }
+}
+
 struct bml_hip_aux *bmlh_aux_new(void)
 {
     return (struct bml_hip_aux *) calloc(1, sizeof(struct bml_hip_aux));

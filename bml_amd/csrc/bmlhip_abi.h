@@ -65,6 +65,20 @@ int bmlhip_status_fetch(bmlhip_status *host);
  * MC; kind 1 = AB: no traces, the touched count is checked against MC (as
  * the reference's multiply_AB does).  group = lanes per row (8..64),
  * table = hash slots per row in LDS (power of two, >= 64). */
+/* Batched systems: one ELLPACK-R matrix holds nsys independent systems as
+ * consecutive row ranges (their column indices offset likewise, so the
+ * matrix is block diagonal and its products are the systems' products). */
+typedef struct bmlhip_batch {
+    int nsys;
+    const int *d_row_sys;  /* [N] system of each row */
+    const int *d_row_off;  /* [nsys + 1] first row of each system */
+    int *d_done;           /* [nsys] converged: its rows are skipped */
+    double *d_ab;          /* [2 nsys] alpha, beta of each system's pending add */
+    double *d_trx, *d_trx2, *d_norm2; /* [nsys] tr X, tr X^2, ||X - X^2||_F^2 */
+    double *d_rowtr;       /* [2 N] per-row trace partials */
+    double *d_rownorm;     /* [N] per-row norm partials */
+} bmlhip_batch;
+
 typedef struct bmlhip_spgemm_args {
     int prec, kind, N;
     int MA, MB, MC;
@@ -80,6 +94,7 @@ typedef struct bmlhip_spgemm_args {
     int bucket;  /* slots per probe: 2, 4 or 8 */
     int remap;   /* 1: XCD-aware block-to-row order */
     const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
+    const bmlhip_batch *batch; /* not NULL: per-system traces into batch->d_trx/d_trx2 */
 } bmlhip_spgemm_args;
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);
 
@@ -123,6 +138,37 @@ int bmlhip_sp2_decide(bmlhip_sp2_state *d_state, double nocc, double *d_hist, in
 /* After add (status scalars[2] holds ||X - X^2||^2): record the norm, set
  * done when it is <= tol, advance it. */
 int bmlhip_sp2_check(bmlhip_sp2_state *d_state, double tol, double *d_hist, int maxit);
+/* Capture the kernels queued on the library stream between begin and end
+ * into an executable graph (no allocation, memset or host sync may happen
+ * in between); launch replays them on the library stream. */
+int bmlhip_graph_begin(void);
+int bmlhip_graph_end(void **exec);
+int bmlhip_graph_launch(void *exec);
+void bmlhip_graph_destroy(void *exec);
+
+/* batched add with norm mode (1 or 2): alpha, beta per system from
+ * b->d_ab, per-system norms into b->d_norm2 */
+int bmlhip_add_batch(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
+                     int MB, const void *d_Bv, const int *d_Bi, const int *d_Bn,
+                     double threshold, int mode, const bmlhip_batch *b);
+/* per-system Gershgorin bounds of a batched matrix -> d_emin, d_emax [nsys] */
+int bmlhip_gershgorin_batch(int prec, int N, int M, const void *d_Av, const int *d_Ai,
+                            const int *d_An, const bmlhip_batch *b, double *d_emin,
+                            double *d_emax);
+/* per-system X0 = (emax I - X)/(emax - emin) in place (normalize) */
+int bmlhip_normalize_batch(int prec, int N, int M, void *d_Av, int *d_Ai, int *d_An,
+                           const bmlhip_batch *b, const double *d_emin, const double *d_emax);
+/* per-system branch decision / convergence test (after x2 / after add);
+ * d_alldone counts the finished systems; d_stats[0] receives the product's
+ * longest pre-threshold row, d_stats[1] the longest row of X after the add */
+int bmlhip_sp2_decide_batch(const bmlhip_batch *b, const double *d_nocc, int *d_branch,
+                            int *d_stats);
+int bmlhip_sp2_check_batch(const bmlhip_batch *b, double tol, int maxit, int *d_it,
+                           double *d_norm, int *d_alldone, int *d_stats);
+/* copy rows from (N, MA) to (N, MB) adding col_shift to every column index */
+int bmlhip_copy_rows_shift(int prec, int N, int MA, const void *d_Av, const int *d_Ai,
+                           const int *d_An, int MB, void *d_Bv, int *d_Bi, int *d_Bn,
+                           int col_shift);
 /* values of an ELLPACK-R matrix from one precision to another (same N, M) */
 int bmlhip_convert(int prec_in, int prec_out, size_t count, const void *d_in, void *d_out);
 /* A <- A + beta*I then threshold (add_identity); scale: values *= s first
@@ -145,6 +191,56 @@ int bmlhip_copy_rows(int prec, int N, int MA, const void *d_Av, const int *d_Ai,
 int bmlhip_identity(int prec, int N, int M, void *d_Av, int *d_Ai, int *d_An);
 /* nnz statistics of a matrix -> status max_nnz / total_nnz */
 int bmlhip_row_stats(int N, const int *d_An);
+
+/* ---- ELLBLOCK: one bt x bt tile per block slot ------------------------------ */
+
+typedef struct bmlhip_eb_args {
+    int prec;
+    int kind;           /* 0: x2 (traces -> scalars[0], [1]), 1: AB */
+    int NB, bt;
+    const int *d_bsize;
+    int MA, MB, MC;     /* block slots per block row of A, B, C */
+    const void *d_Av;
+    const int *d_Ai, *d_An;
+    const void *d_Bv;
+    const int *d_Bi, *d_Bn;
+    void *d_Cv;
+    int *d_Ci, *d_Cn;
+    double threshold;
+    int engine;         /* 0 exact, 1 MFMA, 2 MFMA with bf16 inputs */
+    const int *d_skip;  /* if not NULL and *d_skip != 0 the kernels do nothing */
+} bmlhip_eb_args;
+int bmlhip_eb_product(const bmlhip_eb_args *a);
+/* A <- alpha*A + beta*B blockwise; mode 2 adds |A - B|^2 over the union
+ * into scalars[2] (alpha, beta from d_ab when not NULL) */
+int bmlhip_eb_add(int prec, int NB, int bt, const int *d_bsize, int MA, void *d_Av,
+                  int *d_Ai, int *d_An, int MB, const void *d_Bv, const int *d_Bi,
+                  const int *d_Bn, double alpha, double beta, double threshold, int mode,
+                  const double *d_ab, const int *d_skip);
+int bmlhip_eb_threshold(int prec, int NB, int bt, const int *d_bsize, int MB, void *d_Av,
+                        int *d_Ai, int *d_An, double threshold);
+/* trace -> scalars[0] */
+int bmlhip_eb_trace(int prec, int NB, int bt, const int *d_bsize, int MB, const void *d_Av,
+                    const int *d_Ai, const int *d_An);
+/* copy block rows between matrices of different MB */
+int bmlhip_eb_copy(int prec, int NB, int bt, int MA, const void *d_Av, const int *d_Ai,
+                   const int *d_An, int MB, void *d_Bv, int *d_Bi, int *d_Bn);
+/* deterministic sum of n doubles (fixed tree) -> status->scalars[which] */
+int bmlhip_sum_to_status(const double *d_x, int n, int which);
+/* min of lo[0..n) -> scalars[0], max of hi[0..n) -> scalars[1] (the max
+ * starts at DBL_MIN, as bml's Gershgorin) */
+int bmlhip_minmax_to_status(const double *d_lo, const double *d_hi, int n);
+/* Gershgorin bounds of an ELLBLOCK matrix -> scalars[0] (min), [1] (max) */
+int bmlhip_eb_gershgorin(int prec, int NB, int bt, const int *d_bsize, int MB,
+                         const void *d_Av, const int *d_Ai, const int *d_An);
+/* values of the live blocks *= s (s_bits: one element of the precision) */
+int bmlhip_eb_scale(int prec, int NB, int bt, int MB, void *d_Av, const int *d_An,
+                    const void *s_bits);
+/* identity: one diagonal block per block row */
+int bmlhip_eb_identity(int prec, int NB, int bt, const int *d_bsize, int MB, void *d_Av,
+                       int *d_Ai, int *d_An);
+/* block-row statistics -> status max_nnz / total_nnz */
+int bmlhip_eb_row_stats(int NB, const int *d_An);
 
 #ifdef __cplusplus
 }

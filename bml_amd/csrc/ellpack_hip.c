@@ -55,31 +55,6 @@ static int max_nnz_of(const bml_matrix_ellpack_hip_t *A)
     return (A->aux && A->aux->stats_valid) ? A->aux->max_nnz : A->M;
 }
 
-/* T-typed scalar s for precision prec into buf */
-static void put_scalar(int prec, double s, void *buf)
-{
-    switch (prec) {
-    case BML_HIP_SINGLE_REAL: {
-        float f = (float) s;
-        memcpy(buf, &f, 4);
-        break;
-    }
-    case BML_HIP_DOUBLE_REAL:
-        memcpy(buf, &s, 8);
-        break;
-    case BML_HIP_SINGLE_COMPLEX: {
-        float f[2] = { (float) s, 0.0f };
-        memcpy(buf, f, 8);
-        break;
-    }
-    case BML_HIP_DOUBLE_COMPLEX: {
-        double d[2] = { s, 0.0 };
-        memcpy(buf, d, 16);
-        break;
-    }
-    }
-}
-
 static bml_matrix_ellpack_hip_t *alloc_ell(int prec, int N, int M, int mode, int zero,
                                            const char *what)
 {
@@ -621,7 +596,7 @@ void bml_scale_add_identity_ellpack_hip(bml_matrix_ellpack_hip_t *A, double alph
     if (!valid_ell(A, what))
         return;
     unsigned char s[16];
-    put_scalar(A->matrix_precision, alpha, s);
+    bmlh_put_scalar(A->matrix_precision, alpha, s);
     if (!bmlh_ok(bmlhip_scale_add_identity(A->matrix_precision, A->N, A->M, A->value, A->index,
                                            A->nnz, s, beta, threshold, 1), what))
         return;
@@ -706,7 +681,7 @@ void bml_normalize_ellpack_hip(bml_matrix_ellpack_hip_t *A, double mineval, doub
         return;
     const double span = maxeval - mineval;
     unsigned char s[16];
-    put_scalar(A->matrix_precision, -1.0 / span, s);
+    bmlh_put_scalar(A->matrix_precision, -1.0 / span, s);
     if (!bmlh_ok(bmlhip_scale_add_identity(A->matrix_precision, A->N, A->M, A->value, A->index,
                                            A->nnz, s, maxeval / span, 0.0, 1), what))
         return;

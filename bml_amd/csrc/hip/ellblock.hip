@@ -1,0 +1,1084 @@
+// gfx950 kernels for the ELLBLOCK thresholded multiply path.
+//
+// Storage: block slot (ib, jp) owns one BT x BT tile (BT = 8, 16 or 32),
+// the bsize[ib] x bsize[jb] block in its top-left corner, zeros elsewhere.
+// Zero padding makes every tile product a full BT x BT product: padded
+// columns of the left tile meet padded rows of the right one.
+//
+// Product C = A*B (x2 when A = B), one block row ib at a time, in two passes:
+//
+//  k_eb_symbolic  one wavefront per block row: walks the (jp, kp) block
+//                 pairs in the reference's order (jp, then kp), gives every
+//                 touched block column kb a slot in first-touch order (LDS
+//                 hash; a ballot prefix count ranks the new ones), counts the
+//                 pairs per slot, and writes the pairs grouped by slot (jp
+//                 order kept within a slot: a stable counting sort).
+//  k_eb_numeric   four wavefronts per block row: each takes one slot per
+//                 round, accumulates its pairs' tile products in registers
+//                 (exact VALU dot products in the reference's order, or
+//                 MFMA), takes the block's max |element|, and the four
+//                 decide keep/drop together so the kept blocks are stored
+//                 in slot order (the reference's output order).
+//
+// Keep rule (bml_multiply_ellblock_typed.c:406-408): the diagonal block, or
+// max |element| > threshold, compared in double.  A block row touching more
+// block columns than C holds is the reference's fatal overflow (:393-398).
+#include "../bmlhip_abi.h"
+#include "devtypes.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <type_traits>
+
+using namespace bmlhip;
+
+namespace {
+
+constexpr int EMPTY = -1;
+constexpr int NUM_WAVES = 4;  // wavefronts per block row in k_eb_numeric
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct EbParams {
+    int prec, kind, NB, bt;
+    const int *bsize;
+    int MA, MB, MC;
+    const void *Av;
+    const int *Ai, *An;
+    const void *Bv;
+    const int *Bi, *Bn;
+    void *Cv;
+    int *Ci, *Cn;
+    double thr;
+    int maxs, maxp;  // slots and pairs per block row held in the scratch lists
+    int *cnt;        // [NB] touched block columns
+    int *skb;        // [NB*maxs] block column of each slot
+    int *soff;       // [NB*(maxs+1)] start of each slot's pair list
+    int *pairs;      // [NB*maxp] jp << 16 | kp, grouped by slot
+    double *rowtr;   // [2*NB] per block row tr X, tr X^2 (x2)
+    bmlhip_status *st;
+    const int *skip;
+};
+
+__device__ __forceinline__ unsigned eb_hash(int kb, unsigned mask) {
+    return ((unsigned) kb * 0x9E3779B1u >> 7) & mask;
+}
+
+// ---------------------------------------------------------------------------
+// symbolic pass
+
+// Find kb in the LDS table (keys/sval, tsize slots) or insert it; returns the
+// table position, *isnew set when this lane inserted it.  The columns of one
+// chunk are distinct (a row of B holds each block column once), so no two
+// lanes of a chunk insert the same key.
+__device__ __forceinline__ unsigned find_or_insert(int *keys, int kb, unsigned mask, bool *isnew) {
+    unsigned h = eb_hash(kb, mask);
+    while (true) {
+        const int k = keys[h];
+        if (k == kb) return h;
+        if (k == EMPTY) {
+            const int old = atomicCAS(&keys[h], EMPTY, kb);
+            if (old == EMPTY) {
+                *isnew = true;
+                return h;
+            }
+            if (old == kb) return h;
+        }
+        h = (h + 1u) & mask;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
+    extern __shared__ int sm[];
+    if (p.skip && *p.skip) return;
+    const int ib = blockIdx.x;
+    const int lane = threadIdx.x;
+    const unsigned tsize = 1u << tbits, mask = tsize - 1u;
+    int *keys = sm;
+    int *sval = sm + tsize;
+    int *fill = sm + 2 * tsize;  // maxs counters
+    for (unsigned s = lane; s < tsize; s += 64) keys[s] = EMPTY;
+    for (int s = lane; s < p.maxs; s += 64) fill[s] = 0;
+    __syncthreads();
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const int nA = p.An[ib];
+    int count = 0;
+    // pass 1: slots in first-touch order and pairs per slot
+    for (int jp = 0; jp < nA; jp++) {
+        const int jb = p.Ai[(size_t) ib * p.MA + jp];
+        const int nb = p.Bn[jb];
+        for (int k0 = 0; k0 < nb; k0 += 64) {
+            const int kp = k0 + lane;
+            const bool act = kp < nb;
+            bool isnew = false;
+            unsigned h = 0;
+            int kb = 0;
+            if (act) {
+                kb = p.Bi[(size_t) jb * p.MB + kp];
+                h = find_or_insert(keys, kb, mask, &isnew);
+            }
+            const uint64_t nm = ballot64(isnew);
+            if (isnew) {
+                const int r = count + popc64(nm & ltmask);
+                sval[h] = r;
+                p.skb[(size_t) ib * p.maxs + r] = kb;
+            }
+            count += popc64(nm);
+            __syncthreads();  // new slot numbers visible to every lane
+            if (act) fill[sval[h]] += 1;  // the chunk's slots are distinct
+            __syncthreads();
+        }
+    }
+    // exclusive scan of the per-slot counts -> pair-list offsets
+    int running = 0;
+    int *off = p.soff + (size_t) ib * (p.maxs + 1);
+    for (int s0 = 0; s0 < count; s0 += 64) {
+        const int s = s0 + lane;
+        const int v = s < count ? fill[s] : 0;
+        int inc = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (s < count) {
+            off[s] = running + inc - v;
+            fill[s] = running + inc - v;  // next write position of the slot
+        }
+        running += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) off[count] = running;
+    __syncthreads();
+    // pass 2: pairs grouped by slot, jp order kept within a slot
+    int *pr = p.pairs + (size_t) ib * p.maxp;
+    for (int jp = 0; jp < nA; jp++) {
+        const int jb = p.Ai[(size_t) ib * p.MA + jp];
+        const int nb = p.Bn[jb];
+        for (int k0 = 0; k0 < nb; k0 += 64) {
+            const int kp = k0 + lane;
+            if (kp < nb) {
+                const int kb = p.Bi[(size_t) jb * p.MB + kp];
+                unsigned h = eb_hash(kb, mask);
+                while (keys[h] != kb) h = (h + 1u) & mask;
+                const int s = sval[h];
+                pr[fill[s]] = (int) (((unsigned) jp << 16) | (unsigned) kp);
+                fill[s] += 1;
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        p.cnt[ib] = count;
+        // work counters: block pairs and rows
+        atomicAdd((unsigned long long *) &p.st->prof_products[p.prec], (unsigned long long) running);
+        atomicAdd((unsigned long long *) &p.st->prof_rows[p.prec], 1ull);
+        if (count > p.MC) {
+            atomicAdd(&p.st->m_overflow_rows, 1);
+            atomicMin(&p.st->first_m_overflow, ib);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tile accumulators
+
+// Exact engine: lane l holds elements e = l + 64q of the tile.  Each element
+// takes its length-bk dot product left to right, then is added to the
+// accumulator: the order of the reference's block kernels.
+template <typename T, int BT>
+struct ExactAcc {
+    static constexpr int Q = (BT * BT + 63) / 64;
+    T a[Q];
+    __device__ void zero() {
+#pragma unroll
+        for (int q = 0; q < Q; q++) a[q] = Elem<T>::zero();
+    }
+    __device__ void mac(const T *L, const T *R, int bi, int bk, int bj) {
+        using E = Elem<T>;
+        const int lane = lane_id();
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int e = lane + 64 * q;
+            const int ii = e / BT, jj = e % BT;
+            if (e < BT * BT && ii < bi && jj < bj) {
+                T d = E::mul(L[ii * BT], R[jj]);
+                for (int k = 1; k < bk; k++) d = E::add(d, E::mul(L[ii * BT + k], R[k * BT + jj]));
+                a[q] = E::add(a[q], d);
+            }
+        }
+    }
+    __device__ double maxmag() const {
+        double m = 0.0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const double v = Elem<T>::mag(a[q]);
+            m = v > m ? v : m;
+        }
+        return m;
+    }
+    __device__ void store(T *C) const {
+        const int lane = lane_id();
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int e = lane + 64 * q;
+            if (e < BT * BT) C[e] = a[q];
+        }
+    }
+    // diagonal element kk held by this lane (in diag[kk]) for kk < BT
+    __device__ void diag(T *diag) const {
+        const int lane = lane_id();
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int e = lane + 64 * q;
+            if (e < BT * BT && e / BT == e % BT) diag[e / BT] = a[q];
+        }
+    }
+};
+
+// MFMA engine, single real, 32 x 32 tiles: v_mfma_f32_32x32x2f32 takes a
+// 32x2 slice of L (lane l: row l%32) and a 2x32 slice of R (lane l: column
+// l%32); the two k of an instruction come from the two lane halves.  Step s
+// pairs k = s (lanes 0-31) with k = 16 + s (lanes 32-63), so a lane reads 16
+// contiguous elements of its row of L and the lanes of a half read one
+// contiguous row of R per step.  Output: lane l, register v holds row
+// 8*(v/4) + 4*(l/32) + v%4, column l%32.
+struct MfmaAccF32 {
+    floatx16 a;
+    __device__ void zero() {
+#pragma unroll
+        for (int v = 0; v < 16; v++) a[v] = 0.0f;
+    }
+    __device__ void mac(const float *L, const float *R, int, int, int) {
+        const int lane = lane_id();
+        const int i = lane & 31, h = lane >> 5;
+        const float4 *l4 = reinterpret_cast<const float4 *>(L + i * 32 + h * 16);
+        float lv[16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 t = l4[q];
+            lv[4 * q] = t.x;
+            lv[4 * q + 1] = t.y;
+            lv[4 * q + 2] = t.z;
+            lv[4 * q + 3] = t.w;
+        }
+        float rv[16];
+#pragma unroll
+        for (int s = 0; s < 16; s++) rv[s] = R[(h * 16 + s) * 32 + i];
+#pragma unroll
+        for (int s = 0; s < 16; s++) a = __builtin_amdgcn_mfma_f32_32x32x2f32(lv[s], rv[s], a, 0, 0, 0);
+    }
+    __device__ double maxmag() const {
+        float m = 0.0f;
+#pragma unroll
+        for (int v = 0; v < 16; v++) m = fmaxf(m, fabsf(a[v]));
+        return (double) m;
+    }
+    __device__ static int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+    __device__ void store(float *C) const {
+        const int lane = lane_id();
+#pragma unroll
+        for (int v = 0; v < 16; v++) C[row(v, lane) * 32 + (lane & 31)] = a[v];
+    }
+    __device__ void diag(float *dg) const {
+        const int lane = lane_id();
+#pragma unroll
+        for (int v = 0; v < 16; v++)
+            if (row(v, lane) == (lane & 31)) dg[lane & 31] = a[v];
+    }
+};
+
+// MFMA engine, single real inputs rounded to bfloat16, FP32 accumulation:
+// v_mfma_f32_32x32x16_bf16 takes 8 consecutive k per lane (lanes 0-31 one
+// group of 8, lanes 32-63 the next); instruction q covers k = 16q .. 16q+15.
+// Same output layout as the FP32 engine.
+struct MfmaAccBF16 : MfmaAccF32 {
+    __device__ void mac(const float *L, const float *R, int, int, int) {
+        const int lane = lane_id();
+        const int i = lane & 31, h = lane >> 5;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int k0 = 16 * q + 8 * h;
+            const float4 *l4 = reinterpret_cast<const float4 *>(L + i * 32 + k0);
+            const float4 x0 = l4[0], x1 = l4[1];
+            bf16x8 av, bv;
+            av[0] = (__bf16) x0.x;
+            av[1] = (__bf16) x0.y;
+            av[2] = (__bf16) x0.z;
+            av[3] = (__bf16) x0.w;
+            av[4] = (__bf16) x1.x;
+            av[5] = (__bf16) x1.y;
+            av[6] = (__bf16) x1.z;
+            av[7] = (__bf16) x1.w;
+#pragma unroll
+            for (int e = 0; e < 8; e++) bv[e] = (__bf16) R[(k0 + e) * 32 + i];
+            a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, a, 0, 0, 0);
+        }
+    }
+};
+
+// MFMA engine, double real, 32 x 32 tiles as 2 x 2 tiles of 16 x 16:
+// v_mfma_f64_16x16x4f64 takes a 16x4 slice of L (lane l: row l%16, k group
+// l/16) and a 4x16 slice of R.  Step s pairs k = 8g + s for the four lane
+// groups g, so a lane reads 8 contiguous elements of its row of L.  Output:
+// lane l, register v holds row l/16 + 4v, column l%16 of its 16x16 tile.
+struct MfmaAccF64 {
+    doublex4 a[2][2];
+    __device__ void zero() {
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) a[ti][tj][v] = 0.0;
+    }
+    __device__ void mac(const double *L, const double *R, int, int, int) {
+        const int lane = lane_id();
+        const int c = lane & 15, g = lane >> 4;
+        double lv[2][8], rv[2][8];
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++) {
+            const double2 *l2 = reinterpret_cast<const double2 *>(L + (ti * 16 + c) * 32 + g * 8);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double2 t = l2[q];
+                lv[ti][2 * q] = t.x;
+                lv[ti][2 * q + 1] = t.y;
+            }
+        }
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+            for (int s = 0; s < 8; s++) rv[tj][s] = R[(g * 8 + s) * 32 + tj * 16 + c];
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+#pragma unroll
+            for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+                for (int tj = 0; tj < 2; tj++)
+                    a[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(lv[ti][s], rv[tj][s], a[ti][tj],
+                                                                     0, 0, 0);
+    }
+    __device__ double maxmag() const {
+        double m = 0.0;
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) m = fmax(m, fabs(a[ti][tj][v]));
+        return m;
+    }
+    __device__ void store(double *C) const {
+        const int lane = lane_id();
+        const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) C[(ti * 16 + g + 4 * v) * 32 + tj * 16 + c] = a[ti][tj][v];
+    }
+    __device__ void diag(double *dg) const {
+        const int lane = lane_id();
+        const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (g + 4 * v == c) dg[t * 16 + c] = a[t][t][v];
+    }
+};
+
+template <typename T, int ENGINE, int BT> struct AccOf { using type = ExactAcc<T, BT>; };
+template <> struct AccOf<float, 1, 32> { using type = MfmaAccF32; };
+template <> struct AccOf<float, 2, 32> { using type = MfmaAccBF16; };
+template <> struct AccOf<double, 1, 32> { using type = MfmaAccF64; };
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// numeric pass
+template <typename T, int ENGINE, int BT>
+__global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
+    using E = Elem<T>;
+    using Acc = typename AccOf<T, ENGINE, BT>::type;
+    __shared__ int keep_s[NUM_WAVES];
+    __shared__ T diag_s[NUM_WAVES][BT];
+    if (p.skip && *p.skip) return;
+    const int ib = blockIdx.x;
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int count = p.cnt[ib];
+    const int bi = p.bsize[ib];
+    const T *Av = (const T *) p.Av;
+    const T *Bv = (const T *) p.Bv;
+    T *Cv = (T *) p.Cv;
+    constexpr size_t B2 = (size_t) BT * BT;
+    const int *off = p.soff + (size_t) ib * (p.maxs + 1);
+    const int *pr = p.pairs + (size_t) ib * p.maxp;
+    T tx = E::zero(), tx2 = E::zero();
+    if (p.kind == 0 && threadIdx.x == 0) {
+        // tr X: the diagonal of every diagonal block met in the expand phase
+        for (int jp = 0; jp < p.An[ib]; jp++) {
+            const size_t sa = (size_t) ib * p.MA + jp;
+            if (p.Ai[sa] == ib)
+                for (int kk = 0; kk < bi; kk++) tx = E::add(tx, Av[sa * B2 + kk * BT + kk]);
+        }
+    }
+    int base = 0;
+    for (int r0 = 0; r0 < count; r0 += NUM_WAVES) {
+        const int s = r0 + w;
+        bool keep = false;
+        int kb = -1;
+        Acc acc;
+        acc.zero();
+        if (s < count) {
+            kb = p.skb[(size_t) ib * p.maxs + s];
+            const int bj = p.bsize[kb];
+            for (int t = off[s]; t < off[s + 1]; t++) {
+                const int jp = (int) ((unsigned) pr[t] >> 16), kp = pr[t] & 0xffff;
+                const size_t sa = (size_t) ib * p.MA + jp;
+                const int jb = p.Ai[sa];
+                acc.mac(Av + sa * B2, Bv + ((size_t) jb * p.MB + kp) * B2, bi, p.bsize[jb], bj);
+            }
+            const double nrm = wave_max(acc.maxmag());
+            keep = kb == ib || nrm > p.thr;
+            if (p.kind == 0 && kb == ib) acc.diag(diag_s[w]);
+        }
+        if (lane == 0) keep_s[w] = keep ? 1 : 0;
+        __syncthreads();
+        int pos = base, tot = 0;
+        for (int v = 0; v < NUM_WAVES; v++) {
+            pos += v < w ? keep_s[v] : 0;
+            tot += keep_s[v];
+        }
+        if (p.kind == 0 && kb == ib && s < count && lane == 0)
+            for (int kk = 0; kk < bi; kk++) tx2 = E::add(tx2, diag_s[w][kk]);
+        if (keep && pos < p.MC) {
+            const size_t sc = (size_t) ib * p.MC + pos;
+            acc.store(Cv + sc * B2);
+            if (lane == 0) p.Ci[sc] = kb;
+        }
+        base += tot;
+        __syncthreads();
+    }
+    // tx2 lives in the lane that owned the diagonal block: combine
+    __shared__ double tr2_s[NUM_WAVES];
+    if (lane == 0) tr2_s[w] = E::re(tx2);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int stored = base < p.MC ? base : p.MC;
+        p.Cn[ib] = stored;
+        atomicMax(&p.st->max_nnz, stored);
+        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        if (p.kind == 0) {
+            double t2 = 0.0;
+            for (int v = 0; v < NUM_WAVES; v++) t2 += tr2_s[v];
+            p.rowtr[ib] = E::re(tx);
+            p.rowtr[p.NB + ib] = t2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// add / add with norm: A <- alpha*A + beta*B, one wavefront per block row.
+// Union order: A's block columns, then B's new ones
+// (ellblock/bml_add_ellblock_typed.c:35-197); in mode 0 A is skipped when
+// alpha == 0 and B when beta == 0.  Block kept iff its max |element|, held in
+// T, is above the threshold.  Mode 2 also sums |A - B|^2 over the union into
+// rownorm.
+struct EbAddParams {
+    int NB, bt, MA, MB, mode;
+    const int *bsize;
+    void *Av;
+    int *Ai, *An;
+    const void *Bv;
+    const int *Bi, *Bn;
+    double alpha, beta, thr;
+    const double *ab;
+    const int *skip;
+    double *rownorm;
+    bmlhip_status *st;
+};
+
+template <typename T, int BT>
+__global__ __launch_bounds__(64) void k_eb_add(EbAddParams p) {
+    using E = Elem<T>;
+    extern __shared__ int usm[];
+    if (p.skip && *p.skip) return;
+    const int ib = blockIdx.x, lane = threadIdx.x;
+    const int cap = p.MA + p.MB;
+    int *ucol = usm, *ua = usm + cap, *ub = usm + 2 * cap;
+    const double alpha = p.ab ? p.ab[0] : p.alpha;
+    const double beta = p.ab ? p.ab[1] : p.beta;
+    const bool useA = p.mode != 0 || alpha > 0.0 || alpha < 0.0;
+    const bool useB = p.mode != 0 || beta > 0.0 || beta < 0.0;
+    const int nA = useA ? p.An[ib] : 0, nB = useB ? p.Bn[ib] : 0;
+    T *Av = (T *) p.Av;
+    const T *Bv = (const T *) p.Bv;
+    constexpr size_t B2 = (size_t) BT * BT;
+    // union, lane 0 (rows hold at most MA + MB block columns)
+    if (lane == 0) {
+        int n = 0;
+        for (int jp = 0; jp < nA; jp++) {
+            ucol[n] = p.Ai[(size_t) ib * p.MA + jp];
+            ua[n] = jp;
+            ub[n] = -1;
+            n++;
+        }
+        for (int kp = 0; kp < nB; kp++) {
+            const int kb = p.Bi[(size_t) ib * p.MB + kp];
+            int u = 0;
+            while (u < nA && ucol[u] != kb) u++;
+            if (u < nA) {
+                ub[u] = kp;
+            } else {
+                ucol[n] = kb;
+                ua[n] = -1;
+                ub[n] = kp;
+                n++;
+            }
+        }
+        usm[3 * cap] = n;
+    }
+    __syncthreads();
+    const int n = usm[3 * cap];
+    const int bi = p.bsize[ib];
+    constexpr int Q = (BT * BT + 63) / 64;
+    int ll = 0;
+    double nrm2 = 0.0;
+    for (int u = 0; u < n; u++) {
+        const int jb = ucol[u], sa = ua[u], sb = ub[u];
+        const int bj = p.bsize[jb];
+        T x[Q];
+        double m = 0.0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int e = lane + 64 * q;
+            x[q] = E::zero();
+            T y = E::zero();
+            if (e < BT * BT && e / BT < bi && e % BT < bj) {
+                if (sa >= 0) {
+                    const T a = Av[((size_t) ib * p.MA + sa) * B2 + e];
+                    x[q] = E::axpy(x[q], alpha, a);
+                    y = a;
+                }
+                if (sb >= 0) {
+                    const T b = Bv[((size_t) ib * p.MB + sb) * B2 + e];
+                    x[q] = E::axpy(x[q], beta, b);
+                    y = E::sub(y, b);
+                }
+            }
+            const double mg = E::mag(x[q]);
+            m = mg > m ? mg : m;
+            if (p.mode == 2) nrm2 += E::abs2(y);
+        }
+        m = wave_max(m);
+        const bool keep = E::above(E::from_real(m), p.thr);
+        if (keep) {
+            if (ll < p.MA) {
+                const size_t d = ((size_t) ib * p.MA + ll) * B2;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const int e = lane + 64 * q;
+                    if (e < BT * BT) Av[d + e] = x[q];
+                }
+                if (lane == 0) p.Ai[(size_t) ib * p.MA + ll] = jb;
+            } else if (lane == 0 && ll == p.MA) {
+                atomicAdd(&p.st->m_overflow_rows, 1);
+                atomicMin(&p.st->first_m_overflow, ib);
+            }
+            ll++;
+        }
+        __syncthreads();  // slot ll consumed before a later entry overwrites it
+    }
+    for (int off = 32; off > 0; off >>= 1) nrm2 += __shfl_down(nrm2, off, 64);
+    if (lane == 0) {
+        const int stored = ll < p.MA ? ll : p.MA;
+        p.An[ib] = stored;
+        atomicMax(&p.st->max_nnz, stored);
+        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        if (p.mode == 2) p.rownorm[ib] = nrm2;
+    }
+}
+
+// threshold (ellblock/bml_threshold_ellblock_typed.c:95-140): drop a block
+// whose max |element| (held in T) is not above thr, zero the elements of
+// the others that are not above thr, keep the order.  One wavefront per
+// block row.
+template <typename T, int BT>
+__global__ __launch_bounds__(64) void k_eb_threshold(int NB, int MB, const int *bsize, T *Av,
+                                                     int *Ai, int *An, double thr) {
+    using E = Elem<T>;
+    const int ib = blockIdx.x, lane = threadIdx.x;
+    constexpr size_t B2 = (size_t) BT * BT;
+    constexpr int Q = (BT * BT + 63) / 64;
+    const int bi = bsize[ib];
+    const int n = An[ib];
+    int ll = 0;
+    for (int jp = 0; jp < n; jp++) {
+        const size_t s = (size_t) ib * MB + jp;
+        const int jb = Ai[s];
+        const int bj = bsize[jb];
+        T x[Q];
+        double m = 0.0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int e = lane + 64 * q;
+            x[q] = e < BT * BT ? Av[s * B2 + e] : E::zero();
+            if (e < BT * BT && e / BT < bi && e % BT < bj) {
+                const double mg = E::mag(x[q]);
+                m = mg > m ? mg : m;
+            }
+        }
+        m = wave_max(m);
+        if (!E::above(E::from_real(m), thr)) continue;
+        const size_t d = (size_t) ib * MB + ll;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int e = lane + 64 * q;
+            if (e < BT * BT) {
+                const T v = E::above(x[q], thr) ? x[q] : E::zero();
+                Av[d * B2 + e] = v;
+            }
+        }
+        if (lane == 0) Ai[d] = jb;
+        ll++;
+        __syncthreads();
+    }
+    if (lane == 0) An[ib] = ll;
+}
+
+// trace (ellblock/bml_trace_ellblock_typed.c:27-58): the diagonal of the
+// first diagonal block of every block row, per block row in double.
+template <typename T, int BT>
+__global__ __launch_bounds__(256) void k_eb_trace_rows(int NB, int MB, const int *bsize,
+                                                       const T *Av, const int *Ai, const int *An,
+                                                       double *rowtr) {
+    const int ib = blockIdx.x * 256 + threadIdx.x;
+    if (ib >= NB) return;
+    constexpr size_t B2 = (size_t) BT * BT;
+    double s = 0.0;
+    for (int jp = 0; jp < An[ib]; jp++) {
+        const size_t sl = (size_t) ib * MB + jp;
+        if (Ai[sl] == ib) {
+            for (int kk = 0; kk < bsize[ib]; kk++) s += Elem<T>::re(Av[sl * B2 + kk * BT + kk]);
+            break;
+        }
+    }
+    rowtr[ib] = s;
+}
+
+// copy block rows between matrices of different MB
+template <typename T, int BT>
+__global__ __launch_bounds__(256) void k_eb_copy(int NB, int MA, const T *Av, const int *Ai,
+                                                 const int *An, int MB, T *Bv, int *Bi, int *Bn,
+                                                 bmlhip_status *st) {
+    const int ib = blockIdx.x;
+    constexpr size_t B2 = (size_t) BT * BT;
+    const int n = An[ib];
+    const int m = n < MB ? n : MB;
+    for (size_t e = threadIdx.x; e < (size_t) m * B2; e += 256)
+        Bv[(size_t) ib * MB * B2 + e] = Av[(size_t) ib * MA * B2 + e];
+    for (int jp = threadIdx.x; jp < m; jp += 256) Bi[(size_t) ib * MB + jp] = Ai[(size_t) ib * MA + jp];
+    if (threadIdx.x == 0) {
+        Bn[ib] = m;
+        if (n > MB) {
+            atomicAdd(&st->m_overflow_rows, 1);
+            atomicMin(&st->first_m_overflow, ib);
+        }
+    }
+}
+
+// Gershgorin rows (ellblock/bml_normalize_ellblock_typed.c:98-170): element
+// row i = (ib, ii): d = A(i,i), r = sum of |A(i,j)|, j != i, in block then
+// column order, held in T.  Written at ib*BT + ii; padding rows get values
+// that never win the min/max.
+template <typename T, int BT>
+__global__ __launch_bounds__(64) void k_eb_gersh_rows(int NB, int MB, const int *bsize,
+                                                      const T *Av, const int *Ai,
+                                                      const int *An, double *rowlo,
+                                                      double *rowhi) {
+    using E = Elem<T>;
+    const int ib = blockIdx.x, ii = threadIdx.x;
+    if (ii >= BT) return;
+    const size_t at = (size_t) ib * BT + ii;
+    if (ii >= bsize[ib]) {
+        rowlo[at] = 1.7976931348623157e308;
+        rowhi[at] = -1.7976931348623157e308;
+        return;
+    }
+    constexpr size_t B2 = (size_t) BT * BT;
+    T d = E::zero(), rad = E::zero();
+    for (int jp = 0; jp < An[ib]; jp++) {
+        const size_t sl = (size_t) ib * MB + jp;
+        const int jb = Ai[sl];
+        for (int jj = 0; jj < bsize[jb]; jj++) {
+            const T v = Av[sl * B2 + ii * BT + jj];
+            if (jb == ib && jj == ii)
+                d = v;
+            else
+                rad = E::acc_mag(rad, E::mag(v));
+        }
+    }
+    rowlo[at] = E::re_minus(d, rad);
+    rowhi[at] = E::re_plus(d, rad);
+}
+
+// x <- s*x over the live blocks (bml_scale_ellblock)
+template <typename T, int BT>
+__global__ __launch_bounds__(256) void k_eb_scale(int MB, T *Av, const int *An, T sv) {
+    const int ib = blockIdx.x;
+    constexpr size_t B2 = (size_t) BT * BT;
+    const size_t n = (size_t) An[ib] * B2;
+    T *row = Av + (size_t) ib * MB * B2;
+    for (size_t e = threadIdx.x; e < n; e += 256) row[e] = Elem<T>::mul(sv, row[e]);
+}
+
+// identity: one diagonal block per block row
+template <typename T, int BT>
+__global__ __launch_bounds__(256) void k_eb_identity(int MB, const int *bsize, T *Av, int *Ai,
+                                                     int *An) {
+    const int ib = blockIdx.x;
+    constexpr int B2 = BT * BT;
+    T *t = Av + (size_t) ib * MB * B2;
+    for (int e = threadIdx.x; e < B2; e += 256) {
+        const int ii = e / BT, jj = e % BT;
+        t[e] = (ii == jj && ii < bsize[ib]) ? Elem<T>::from_real(1.0) : Elem<T>::zero();
+    }
+    if (threadIdx.x == 0) {
+        Ai[(size_t) ib * MB] = ib;
+        An[ib] = 1;
+    }
+}
+
+int fail_msg(int code, const char *fmt, ...) {
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    bmlhip_set_errmsg(buf);
+    return code;
+}
+
+int report(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return 0;
+    return fail_msg(3, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename T, int ENGINE, int BT>
+int launch_numeric(const EbParams &p, hipStream_t s) {
+    if constexpr (ENGINE != 0 && !(BT == 32 && (std::is_same<T, float>::value ||
+                                                (ENGINE == 1 && std::is_same<T, double>::value)))) {
+        return fail_msg(2, "no MFMA engine for this element type and tile");
+    } else {
+        hipLaunchKernelGGL((k_eb_numeric<T, ENGINE, BT>), dim3(p.NB), dim3(64 * NUM_WAVES), 0, s,
+                           p);
+        return report("k_eb_numeric");
+    }
+}
+
+template <typename T>
+int eb_product_typed(const bmlhip_eb_args *a) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    if (a->MA >= 65536 || a->MB >= 65536)
+        return fail_msg(5, "ellblock product: more than 65535 block slots per row");
+    EbParams p;
+    memset(&p, 0, sizeof p);
+    p.prec = a->prec;
+    p.kind = a->kind;
+    p.NB = a->NB;
+    p.bt = a->bt;
+    p.bsize = a->d_bsize;
+    p.MA = a->MA;
+    p.MB = a->MB;
+    p.MC = a->MC;
+    p.Av = a->d_Av;
+    p.Ai = a->d_Ai;
+    p.An = a->d_An;
+    p.Bv = a->d_Bv;
+    p.Bi = a->d_Bi;
+    p.Bn = a->d_Bn;
+    p.Cv = a->d_Cv;
+    p.Ci = a->d_Ci;
+    p.Cn = a->d_Cn;
+    p.thr = a->threshold;
+    p.skip = a->d_skip;
+    p.st = bmlhip_status_dev();
+    const long long maxp = (long long) a->MA * a->MB;
+    const long long maxs = maxp < a->NB ? maxp : a->NB;
+    int tbits = 6;
+    while ((1ll << tbits) < 2 * maxs) tbits++;
+    const size_t lds = (size_t) (2 * (1ll << tbits) + maxs) * sizeof(int);
+    if (lds > 160 * 1024)
+        return fail_msg(5, "ellblock product: %lld block pairs per row exceed the LDS slot map",
+                        maxp);
+    p.maxs = (int) maxs;
+    p.maxp = (int) maxp;
+    const size_t NB = (size_t) a->NB;
+    p.rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (NB + 1));
+    p.cnt = (int *) bmlhip_scratch(3, sizeof(int) * (NB + 1));
+    p.skb = (int *) bmlhip_scratch(4, sizeof(int) * (NB * maxs + 1));
+    p.soff = (int *) bmlhip_scratch(5, sizeof(int) * (NB * (maxs + 1) + 1));
+    p.pairs = (int *) bmlhip_scratch(6, sizeof(int) * (NB * maxp + 1));
+    if (!p.rowtr || !p.cnt || !p.skb || !p.soff || !p.pairs)
+        return fail_msg(4, "ellblock product: scratch allocation failed");
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    if (a->NB == 0) return 0;
+    if (lds > 64 * 1024)
+        (void) hipFuncSetAttribute((const void *) k_eb_symbolic,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+    hipLaunchKernelGGL(k_eb_symbolic, dim3(a->NB), dim3(64), lds, s, p, tbits);
+    if ((rc = report("k_eb_symbolic"))) return rc;
+    const int prec_mark = a->prec;
+    bmlhip_prof_mark_begin(prec_mark);
+    if (std::is_same<T, float>::value && a->bt == 32 && a->engine == 1) {
+        rc = launch_numeric<T, 1, 32>(p, s);
+    } else if (std::is_same<T, float>::value && a->bt == 32 && a->engine == 2) {
+        rc = launch_numeric<T, 2, 32>(p, s);
+    } else if (std::is_same<T, double>::value && a->bt == 32 && a->engine == 1) {
+        rc = launch_numeric<T, 1, 32>(p, s);
+    } else {
+        switch (a->bt) {
+        case 8: rc = launch_numeric<T, 0, 8>(p, s); break;
+        case 16: rc = launch_numeric<T, 0, 16>(p, s); break;
+        case 32: rc = launch_numeric<T, 0, 32>(p, s); break;
+        default: rc = fail_msg(2, "tile edge %d not one of 8/16/32", a->bt);
+        }
+    }
+    bmlhip_prof_mark_end();
+    if (rc) return rc;
+    if (a->kind == 0) {
+        if ((rc = bmlhip_sum_to_status(p.rowtr, a->NB, 0))) return rc;
+        if ((rc = bmlhip_sum_to_status(p.rowtr + a->NB, a->NB, 1))) return rc;
+    }
+    return 0;
+}
+
+template <typename T>
+int eb_add_typed(int NB, int bt, const int *bsize, int MA, void *Av, int *Ai, int *An, int MB,
+                 const void *Bv, const int *Bi, const int *Bn, double alpha, double beta,
+                 double thr, int mode, const double *ab, const int *skip) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    EbAddParams p;
+    p.NB = NB;
+    p.bt = bt;
+    p.MA = MA;
+    p.MB = MB;
+    p.mode = mode;
+    p.bsize = bsize;
+    p.Av = Av;
+    p.Ai = Ai;
+    p.An = An;
+    p.Bv = Bv;
+    p.Bi = Bi;
+    p.Bn = Bn;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.thr = thr;
+    p.ab = ab;
+    p.skip = skip;
+    p.st = bmlhip_status_dev();
+    p.rownorm = (double *) bmlhip_scratch(0, sizeof(double) * ((size_t) NB + 1));
+    if (!p.rownorm) return fail_msg(4, "ellblock add: scratch allocation failed");
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    if (NB == 0) return 0;
+    const size_t lds = (size_t) (3 * (MA + MB) + 1) * sizeof(int);
+    if (lds > 64 * 1024)
+        return fail_msg(5, "ellblock add: %d + %d block slots per row exceed the LDS union list",
+                        MA, MB);
+    switch (bt) {
+    case 8: hipLaunchKernelGGL((k_eb_add<T, 8>), dim3(NB), dim3(64), lds, s, p); break;
+    case 16: hipLaunchKernelGGL((k_eb_add<T, 16>), dim3(NB), dim3(64), lds, s, p); break;
+    case 32: hipLaunchKernelGGL((k_eb_add<T, 32>), dim3(NB), dim3(64), lds, s, p); break;
+    default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+    }
+    if ((rc = report("k_eb_add"))) return rc;
+    if (mode == 2) return bmlhip_sum_to_status(p.rownorm, NB, 2);
+    return 0;
+}
+
+template <typename T>
+int eb_threshold_typed(int NB, int bt, const int *bsize, int MB, void *Av, int *Ai, int *An,
+                       double thr) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    if (NB == 0) return 0;
+    switch (bt) {
+    case 8: hipLaunchKernelGGL((k_eb_threshold<T, 8>), dim3(NB), dim3(64), 0, s, NB, MB, bsize, (T *) Av, Ai, An, thr); break;
+    case 16: hipLaunchKernelGGL((k_eb_threshold<T, 16>), dim3(NB), dim3(64), 0, s, NB, MB, bsize, (T *) Av, Ai, An, thr); break;
+    case 32: hipLaunchKernelGGL((k_eb_threshold<T, 32>), dim3(NB), dim3(64), 0, s, NB, MB, bsize, (T *) Av, Ai, An, thr); break;
+    default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+    }
+    return report("k_eb_threshold");
+}
+
+template <typename T>
+int eb_trace_typed(int NB, int bt, const int *bsize, int MB, const void *Av, const int *Ai,
+                   const int *An) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    double *rowtr = (double *) bmlhip_scratch(0, sizeof(double) * ((size_t) NB + 1));
+    if (!rowtr) return fail_msg(4, "ellblock trace: scratch allocation failed");
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    if (NB == 0) return 0;
+    const dim3 g((NB + 255) / 256);
+    switch (bt) {
+    case 8: hipLaunchKernelGGL((k_eb_trace_rows<T, 8>), g, dim3(256), 0, s, NB, MB, bsize, (const T *) Av, Ai, An, rowtr); break;
+    case 16: hipLaunchKernelGGL((k_eb_trace_rows<T, 16>), g, dim3(256), 0, s, NB, MB, bsize, (const T *) Av, Ai, An, rowtr); break;
+    case 32: hipLaunchKernelGGL((k_eb_trace_rows<T, 32>), g, dim3(256), 0, s, NB, MB, bsize, (const T *) Av, Ai, An, rowtr); break;
+    default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+    }
+    if ((rc = report("k_eb_trace_rows"))) return rc;
+    return bmlhip_sum_to_status(rowtr, NB, 0);
+}
+
+template <typename T>
+int eb_copy_typed(int NB, int bt, int MA, const void *Av, const int *Ai, const int *An, int MB,
+                  void *Bv, int *Bi, int *Bn) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    if (NB == 0) return 0;
+    bmlhip_status *st = bmlhip_status_dev();
+    switch (bt) {
+    case 8: hipLaunchKernelGGL((k_eb_copy<T, 8>), dim3(NB), dim3(256), 0, s, NB, MA, (const T *) Av, Ai, An, MB, (T *) Bv, Bi, Bn, st); break;
+    case 16: hipLaunchKernelGGL((k_eb_copy<T, 16>), dim3(NB), dim3(256), 0, s, NB, MA, (const T *) Av, Ai, An, MB, (T *) Bv, Bi, Bn, st); break;
+    case 32: hipLaunchKernelGGL((k_eb_copy<T, 32>), dim3(NB), dim3(256), 0, s, NB, MA, (const T *) Av, Ai, An, MB, (T *) Bv, Bi, Bn, st); break;
+    default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+    }
+    return report("k_eb_copy");
+}
+
+template <typename T>
+int eb_gershgorin_typed(int NB, int bt, const int *bsize, int MB, const void *Av, const int *Ai,
+                        const int *An) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    const size_t n = (size_t) NB * bt;
+    double *rows = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (n + 1));
+    if (!rows) return fail_msg(4, "ellblock gershgorin: scratch allocation failed");
+    if (NB > 0) {
+        switch (bt) {
+        case 8: hipLaunchKernelGGL((k_eb_gersh_rows<T, 8>), dim3(NB), dim3(64), 0, s, NB, MB, bsize, (const T *) Av, Ai, An, rows, rows + n); break;
+        case 16: hipLaunchKernelGGL((k_eb_gersh_rows<T, 16>), dim3(NB), dim3(64), 0, s, NB, MB, bsize, (const T *) Av, Ai, An, rows, rows + n); break;
+        case 32: hipLaunchKernelGGL((k_eb_gersh_rows<T, 32>), dim3(NB), dim3(64), 0, s, NB, MB, bsize, (const T *) Av, Ai, An, rows, rows + n); break;
+        default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+        }
+        if ((rc = report("k_eb_gersh_rows"))) return rc;
+    }
+    return bmlhip_minmax_to_status(rows, rows + n, (int) n);
+}
+
+template <typename T>
+int eb_scale_typed(int NB, int bt, int MB, void *Av, const int *An, const void *s_bits) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    T sv;
+    memcpy(&sv, s_bits, sizeof(T));
+    if (NB == 0) return 0;
+    switch (bt) {
+    case 8: hipLaunchKernelGGL((k_eb_scale<T, 8>), dim3(NB), dim3(256), 0, s, MB, (T *) Av, An, sv); break;
+    case 16: hipLaunchKernelGGL((k_eb_scale<T, 16>), dim3(NB), dim3(256), 0, s, MB, (T *) Av, An, sv); break;
+    case 32: hipLaunchKernelGGL((k_eb_scale<T, 32>), dim3(NB), dim3(256), 0, s, MB, (T *) Av, An, sv); break;
+    default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+    }
+    return report("k_eb_scale");
+}
+
+template <typename T>
+int eb_identity_typed(int NB, int bt, const int *bsize, int MB, void *Av, int *Ai, int *An) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    if (NB == 0) return 0;
+    switch (bt) {
+    case 8: hipLaunchKernelGGL((k_eb_identity<T, 8>), dim3(NB), dim3(256), 0, s, MB, bsize, (T *) Av, Ai, An); break;
+    case 16: hipLaunchKernelGGL((k_eb_identity<T, 16>), dim3(NB), dim3(256), 0, s, MB, bsize, (T *) Av, Ai, An); break;
+    case 32: hipLaunchKernelGGL((k_eb_identity<T, 32>), dim3(NB), dim3(256), 0, s, MB, bsize, (T *) Av, Ai, An); break;
+    default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+    }
+    return report("k_eb_identity");
+}
+
+}  // namespace
+
+#define DISPATCH(prec, call_tmpl)                                                  \
+    switch (prec) {                                                                \
+    case 1: return call_tmpl(float);                                               \
+    case 2: return call_tmpl(double);                                              \
+    case 3: return call_tmpl(cf32);                                                \
+    case 4: return call_tmpl(cf64);                                                \
+    default: return fail_msg(2, "unknown precision %d", prec);                      \
+    }
+
+extern "C" {
+
+int bmlhip_eb_product(const bmlhip_eb_args *a) {
+#define CALL(T) eb_product_typed<T>(a)
+    DISPATCH(a->prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_add(int prec, int NB, int bt, const int *bsize, int MA, void *Av, int *Ai,
+                  int *An, int MB, const void *Bv, const int *Bi, const int *Bn, double alpha,
+                  double beta, double thr, int mode, const double *ab, const int *skip) {
+#define CALL(T) eb_add_typed<T>(NB, bt, bsize, MA, Av, Ai, An, MB, Bv, Bi, Bn, alpha, beta, thr, mode, ab, skip)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_threshold(int prec, int NB, int bt, const int *bsize, int MB, void *Av, int *Ai,
+                        int *An, double thr) {
+#define CALL(T) eb_threshold_typed<T>(NB, bt, bsize, MB, Av, Ai, An, thr)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_trace(int prec, int NB, int bt, const int *bsize, int MB, const void *Av,
+                    const int *Ai, const int *An) {
+#define CALL(T) eb_trace_typed<T>(NB, bt, bsize, MB, Av, Ai, An)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_copy(int prec, int NB, int bt, int MA, const void *Av, const int *Ai,
+                   const int *An, int MB, void *Bv, int *Bi, int *Bn) {
+#define CALL(T) eb_copy_typed<T>(NB, bt, MA, Av, Ai, An, MB, Bv, Bi, Bn)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_gershgorin(int prec, int NB, int bt, const int *bsize, int MB, const void *Av,
+                         const int *Ai, const int *An) {
+#define CALL(T) eb_gershgorin_typed<T>(NB, bt, bsize, MB, Av, Ai, An)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_scale(int prec, int NB, int bt, int MB, void *Av, const int *An,
+                    const void *s_bits) {
+#define CALL(T) eb_scale_typed<T>(NB, bt, MB, Av, An, s_bits)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_identity(int prec, int NB, int bt, const int *bsize, int MB, void *Av, int *Ai,
+                       int *An) {
+#define CALL(T) eb_identity_typed<T>(NB, bt, bsize, MB, Av, Ai, An)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_row_stats(int NB, const int *An) { return bmlhip_row_stats(NB, An); }
+
+}  // extern "C"

@@ -79,6 +79,9 @@ struct SpParams {
     int *ovf;         // rows handed to the global-table pass
     const int *skip;  // when set and nonzero, the launch does nothing
     int remap;        // XCD-aware block-to-row order
+    // batched systems: row i belongs to system row_sys[i]; rows of systems
+    // with sys_done set are skipped (both NULL: one system)
+    const int *row_sys, *sys_done;
 };
 
 // Broadcast lane src's value to the caller's lanes.  With one group per
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
     }
     const int g = threadIdx.x / GROUP;
     const int row = vb * R + g;
-    const int i = row < p.N ? row : -1;
+    const int i = (row < p.N && !(p.row_sys && p.sys_done[p.row_sys[row]])) ? row : -1;
     const int cap = (int) tsize - 8;
     spgemm_row<T, GROUP, BK, false, uint16_t>(p, i, keys + g * stride, vals + (size_t) g * stride,
                                               ord + g * stride, tsize, cap);
@@ -517,6 +520,51 @@ __global__ __launch_bounds__(1024) void k_minmax(const double *x, const double *
     }
 }
 
+// Per-system sums of x over row ranges [off[s], off[s+1]) (fixed partition
+// and tree: deterministic); systems with done[s] set are left alone.
+__global__ __launch_bounds__(256) void k_seg_sum(const double *x, const int *off, const int *done,
+                                                 double *out) {
+    __shared__ double part[256];
+    const int s = blockIdx.x;
+    if (done && done[s]) return;
+    double acc = 0.0;
+    for (int k = off[s] + (int) threadIdx.x; k < off[s + 1]; k += 256) acc += x[k];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int) threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = part[0];
+}
+
+// Per-system min of lo and max of hi (the max starting at DBL_MIN, as bml's
+// Gershgorin).
+__global__ __launch_bounds__(256) void k_seg_minmax(const double *lo, const double *hi,
+                                                    const int *off, double *omin, double *omax) {
+    __shared__ double a_s[256], b_s[256];
+    const int s = blockIdx.x;
+    double a = 1.7976931348623157e308, b = 2.2250738585072014e-308;
+    for (int k = off[s] + (int) threadIdx.x; k < off[s + 1]; k += 256) {
+        a = lo[k] < a ? lo[k] : a;
+        b = hi[k] > b ? hi[k] : b;
+    }
+    a_s[threadIdx.x] = a;
+    b_s[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int) threadIdx.x < w) {
+            a_s[threadIdx.x] = a_s[threadIdx.x + w] < a_s[threadIdx.x] ? a_s[threadIdx.x + w] : a_s[threadIdx.x];
+            b_s[threadIdx.x] = b_s[threadIdx.x + w] > b_s[threadIdx.x] ? b_s[threadIdx.x + w] : b_s[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        omin[s] = a_s[0];
+        omax[s] = b_s[0];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // add / add_norm: A <- alpha*A + beta*B per row, union in first-touch order
 // (A's columns, then B's new ones), every entry kept iff |v| > threshold.
@@ -535,6 +583,10 @@ struct AddParams {
     bmlhip_status *st;
     const double *ab;  // when set: alpha = ab[0], beta = ab[1] (read on the device)
     const int *skip;   // when set and nonzero, the launch does nothing
+    // batched systems: alpha, beta of row i at ab_sys[2*row_sys[i]]; rows of
+    // systems with sys_done set are skipped
+    const int *row_sys, *sys_done;
+    const double *ab_sys;
 };
 
 template <typename T>
@@ -547,8 +599,9 @@ __device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int 
     const T *Bv = (const T *) p.Bv;
     const size_t rA = (size_t) i * p.MA, rB = (size_t) i * p.MB;
     int count = 0;
-    const double alpha = p.ab ? p.ab[0] : p.alpha;
-    const double beta = p.ab ? p.ab[1] : p.beta;
+    const double *ab = p.ab_sys ? p.ab_sys + 2 * p.row_sys[i] : p.ab;
+    const double alpha = ab ? ab[0] : p.alpha;
+    const double beta = ab ? ab[1] : p.beta;
     for (int part = 0; part < 2; part++) {
         const double s = part ? beta : alpha;
         if (p.mode == 0 && !(s > 0.0 || s < 0.0)) continue;
@@ -645,7 +698,9 @@ __global__ __launch_bounds__(64) void k_add_lds(AddParams p, int tbits) {
         ys[s] = Elem<T>::zero();
     }
     __syncthreads();
-    add_row<T>(p, blockIdx.x, keys, xs, ys, ord, tbits);
+    const int i = blockIdx.x;
+    if (p.row_sys && p.sys_done[p.row_sys[i]]) return;
+    add_row<T>(p, i, keys, xs, ys, ord, tbits);
 }
 
 template <typename T>
@@ -664,6 +719,7 @@ __global__ __launch_bounds__(64) void k_add_global(AddParams p, int tbits, unsig
     }
     __syncthreads();
     for (int i = blockIdx.x; i < p.N; i += gridDim.x) {
+        if (p.row_sys && p.sys_done[p.row_sys[i]]) continue;
         add_row<T>(p, i, keys, xs, ys, ord, tbits);
         __syncthreads();
     }
@@ -746,17 +802,13 @@ __global__ __launch_bounds__(256) void k_gersh_rows(int N, int M, const T *Av, c
     rowhi[i] = E::re_plus(d, rad);
 }
 
-// scale (optional) then add_identity with threshold: one thread per row,
-// sequential like the reference (bml_add_ellpack_typed.c:420-549)
+// scale (optional) then add_identity with threshold, row i, sequential like
+// the reference (bml_add_ellpack_typed.c:420-549)
 template <typename T>
-__global__ __launch_bounds__(256) void k_scale_add_identity(int N, int M, T *Av, int *Ai,
-                                                            int *An, T s, int do_scale,
-                                                            double beta, double thr,
-                                                            int do_identity,
-                                                            bmlhip_status *st) {
+__device__ void scale_add_identity_row(int i, int M, T *Av, int *Ai, int *An, T s, int do_scale,
+                                       double beta, double thr, int do_identity,
+                                       bmlhip_status *st) {
     using E = Elem<T>;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
     const size_t r = (size_t) i * M;
     const int n = An[i];
     if (do_scale)
@@ -797,10 +849,38 @@ __global__ __launch_bounds__(256) void k_scale_add_identity(int N, int M, T *Av,
     An[i] = kept;
 }
 
+// one thread per row
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale_add_identity(int N, int M, T *Av, int *Ai,
+                                                            int *An, T s, int do_scale,
+                                                            double beta, double thr,
+                                                            int do_identity,
+                                                            bmlhip_status *st) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    scale_add_identity_row<T>(i, M, Av, Ai, An, s, do_scale, beta, thr, do_identity, st);
+}
+
+// batched normalize: row i of system s = row_sys[i] gets
+// X0 = (emax I - X)/(emax - emin) with that system's bounds, as
+// bml_normalize_ellpack_typed.c:33-45 (scale by (T)(-1/(max-min)), then
+// add_identity(max/(max-min)) at threshold 0)
+template <typename T>
+__global__ __launch_bounds__(256) void k_normalize_batch(int N, int M, T *Av, int *Ai, int *An,
+                                                         const int *row_sys, const double *emin,
+                                                         const double *emax, bmlhip_status *st) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const int sy = row_sys[i];
+    const double d = emax[sy] - emin[sy];
+    scale_add_identity_row<T>(i, M, Av, Ai, An, Elem<T>::from_real(-1.0 / d), 1, emax[sy] / d,
+                              0.0, 1, st);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_rows(int N, int MA, const T *Av, const int *Ai,
                                                    const int *An, int MB, T *Bv, int *Bi,
-                                                   int *Bn, bmlhip_status *st) {
+                                                   int *Bn, bmlhip_status *st, int shift) {
     const int i = blockIdx.x * 4 + threadIdx.x / 64;
     if (i >= N) return;
     const int lane = threadIdx.x & 63;
@@ -808,7 +888,7 @@ __global__ __launch_bounds__(256) void k_copy_rows(int N, int MA, const T *Av, c
     const int m = n < MB ? n : MB;
     for (int k = lane; k < m; k += 64) {
         Bv[(size_t) i * MB + k] = Av[(size_t) i * MA + k];
-        Bi[(size_t) i * MB + k] = Ai[(size_t) i * MA + k];
+        Bi[(size_t) i * MB + k] = Ai[(size_t) i * MA + k] + shift;
     }
     if (lane == 0) {
         Bn[i] = m;
@@ -923,7 +1003,10 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     p.thr = a->threshold;
     p.skip = a->d_skip;
     p.st = bmlhip_status_dev();
-    p.rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
+    p.row_sys = a->batch ? a->batch->d_row_sys : nullptr;
+    p.sys_done = a->batch ? a->batch->d_done : nullptr;
+    p.rowtr = a->batch ? a->batch->d_rowtr
+                       : (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
     p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
     p.remap = a->remap;
     // global-table pass: one table per block, >= 2x the row bound
@@ -960,7 +1043,14 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     hipLaunchKernelGGL(k_spgemm_global<T>, dim3(gblocks), dim3(64), 0, s, p, (unsigned) gsize, gtab,
                        cursor);
     if ((rc = report("k_spgemm_global"))) return rc;
-    if (a->kind == 0) {
+    if (a->kind == 0 && a->batch) {
+        const bmlhip_batch *b = a->batch;
+        hipLaunchKernelGGL(k_seg_sum, dim3(b->nsys), dim3(256), 0, s, p.rowtr, b->d_row_off,
+                           b->d_done, b->d_trx);
+        hipLaunchKernelGGL(k_seg_sum, dim3(b->nsys), dim3(256), 0, s, p.rowtr + a->N,
+                           b->d_row_off, b->d_done, b->d_trx2);
+        if ((rc = report("k_seg_sum"))) return rc;
+    } else if (a->kind == 0) {
         hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, p.rowtr, a->N, &p.st->scalars[0]);
         hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, p.rowtr + a->N, a->N,
                            &p.st->scalars[1]);
@@ -972,7 +1062,8 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
 template <typename T>
 int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv, const int *Bi,
               const int *Bn, double alpha, double beta, double thr, int mode,
-              const double *abdev = nullptr, const int *skip = nullptr) {
+              const double *abdev = nullptr, const int *skip = nullptr,
+              const bmlhip_batch *batch = nullptr, double *rownorm = nullptr) {
     hipStream_t s = (hipStream_t) bmlhip_stream();
     AddParams p;
     p.N = N;
@@ -990,8 +1081,11 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
     p.thr = thr;
     p.ab = abdev;
     p.skip = skip;
+    p.row_sys = batch ? batch->d_row_sys : nullptr;
+    p.sys_done = batch ? batch->d_done : nullptr;
+    p.ab_sys = batch ? batch->d_ab : nullptr;
     p.st = bmlhip_status_dev();
-    p.rownorm = (double *) bmlhip_scratch(0, sizeof(double) * (size_t) (N + 1));
+    p.rownorm = rownorm ? rownorm : (double *) bmlhip_scratch(0, sizeof(double) * (size_t) (N + 1));
     if (!p.rownorm) return 4;
     int rc = bmlhip_status_reset();
     if (rc) return rc;
@@ -1010,7 +1104,11 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
             if ((rc = report("k_add_global"))) return rc;
         }
     }
-    if (mode) {
+    if (mode && batch) {
+        hipLaunchKernelGGL(k_seg_sum, dim3(batch->nsys), dim3(256), 0, s, p.rownorm,
+                           batch->d_row_off, batch->d_done, batch->d_norm2);
+        if ((rc = report("k_seg_sum"))) return rc;
+    } else if (mode) {
         hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, p.rownorm, N, &p.st->scalars[2]);
         if ((rc = report("k_sum"))) return rc;
     }
@@ -1074,14 +1172,39 @@ int gershgorin_typed(int N, int M, const void *Av, const int *Ai, const int *An)
 
 template <typename T>
 int copy_rows_typed(int N, int MA, const void *Av, const int *Ai, const int *An, int MB, void *Bv,
-                    int *Bi, int *Bn) {
+                    int *Bi, int *Bn, int shift = 0) {
     hipStream_t s = (hipStream_t) bmlhip_stream();
     int rc = bmlhip_status_reset();
     if (rc) return rc;
     if (N > 0)
         hipLaunchKernelGGL(k_copy_rows<T>, dim3((N + 3) / 4), dim3(256), 0, s, N, MA,
-                           (const T *) Av, Ai, An, MB, (T *) Bv, Bi, Bn, bmlhip_status_dev());
+                           (const T *) Av, Ai, An, MB, (T *) Bv, Bi, Bn, bmlhip_status_dev(),
+                           shift);
     return report("k_copy_rows");
+}
+
+template <typename T>
+int gershgorin_batch_typed(int N, int M, const void *Av, const int *Ai, const int *An,
+                           const bmlhip_batch *b, double *emin, double *emax) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    double *rows = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (N + 1));
+    if (!rows) return 4;
+    if (N > 0)
+        hipLaunchKernelGGL(k_gersh_rows<T>, dim3((N + 255) / 256), dim3(256), 0, s, N, M,
+                           (const T *) Av, Ai, An, rows, rows + N);
+    hipLaunchKernelGGL(k_seg_minmax, dim3(b->nsys), dim3(256), 0, s, rows, rows + N, b->d_row_off,
+                       emin, emax);
+    return report("k_gershgorin_batch");
+}
+
+template <typename T>
+int normalize_batch_typed(int N, int M, void *Av, int *Ai, int *An, const bmlhip_batch *b,
+                          const double *emin, const double *emax) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    if (N > 0)
+        hipLaunchKernelGGL(k_normalize_batch<T>, dim3((N + 255) / 256), dim3(256), 0, s, N, M,
+                           (T *) Av, Ai, An, b->d_row_sys, emin, emax, bmlhip_status_dev());
+    return report("k_normalize_batch");
 }
 
 template <typename T>
@@ -1163,6 +1286,48 @@ int bmlhip_identity(int prec, int N, int M, void *Av, int *Ai, int *An) {
 #define CALL(T) identity_typed<T>(N, M, Av, Ai, An)
     DISPATCH(prec, CALL)
 #undef CALL
+}
+
+int bmlhip_add_batch(int prec, int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
+                     const int *Bi, const int *Bn, double thr, int mode, const bmlhip_batch *b) {
+#define CALL(T) add_typed<T>(N, MA, Av, Ai, An, MB, Bv, Bi, Bn, 0.0, 0.0, thr, mode, nullptr, nullptr, b, b->d_rownorm)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_gershgorin_batch(int prec, int N, int M, const void *Av, const int *Ai, const int *An,
+                            const bmlhip_batch *b, double *emin, double *emax) {
+#define CALL(T) gershgorin_batch_typed<T>(N, M, Av, Ai, An, b, emin, emax)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_normalize_batch(int prec, int N, int M, void *Av, int *Ai, int *An,
+                           const bmlhip_batch *b, const double *emin, const double *emax) {
+#define CALL(T) normalize_batch_typed<T>(N, M, Av, Ai, An, b, emin, emax)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_copy_rows_shift(int prec, int N, int MA, const void *Av, const int *Ai, const int *An,
+                           int MB, void *Bv, int *Bi, int *Bn, int shift) {
+#define CALL(T) copy_rows_typed<T>(N, MA, Av, Ai, An, MB, Bv, Bi, Bn, shift)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_sum_to_status(const double *x, int n, int which) {
+    if (which < 0 || which > 3) return fail_msg(2, "status scalar %d out of range", which);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, (hipStream_t) bmlhip_stream(), x, n,
+                       &bmlhip_status_dev()->scalars[which]);
+    return report("k_sum");
+}
+
+int bmlhip_minmax_to_status(const double *lo, const double *hi, int n) {
+    // emax starts at DBL_MIN as in the reference
+    hipLaunchKernelGGL(k_minmax, dim3(1), dim3(1024), 0, (hipStream_t) bmlhip_stream(), lo, hi,
+                       n, &bmlhip_status_dev()->scalars[0], 2.2250738585072014e-308);
+    return report("k_minmax");
 }
 
 int bmlhip_row_stats(int N, const int *An) {

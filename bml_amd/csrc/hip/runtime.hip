@@ -10,6 +10,7 @@
 namespace {
 
 struct Runtime {
+    int capturing = 0;
     int profiling = 0;
     int prof_n = 0;
     hipEvent_t prof_ev[2 * 4096];
@@ -202,7 +203,7 @@ int bmlhip_prof_begin(void) {
 }
 
 void bmlhip_prof_mark_begin(int prec) {
-    if (!g.profiling || g.prof_n >= 4096) return;
+    if (!g.profiling || g.capturing || g.prof_n >= 4096) return;
     hipEvent_t *e = &g.prof_ev[2 * g.prof_n];
     if (hipEventCreate(&e[0]) != hipSuccess || hipEventCreate(&e[1]) != hipSuccess) return;
     g.prof_prec[g.prof_n] = prec;
@@ -210,7 +211,7 @@ void bmlhip_prof_mark_begin(int prec) {
 }
 
 void bmlhip_prof_mark_end(void) {
-    if (!g.profiling || g.prof_n >= 4096) return;
+    if (!g.profiling || g.capturing || g.prof_n >= 4096) return;
     (void) hipEventRecord(g.prof_ev[2 * g.prof_n + 1], g.stream);
     g.prof_n++;
 }
@@ -234,6 +235,33 @@ int bmlhip_prof_end(int launches[5], double kernel_ms[5]) {
     g.profiling = 0;
     g.prof_n = 0;
     return 0;
+}
+
+int bmlhip_graph_begin(void) {
+    CHECK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+    g.capturing = 1;
+    return 0;
+}
+
+int bmlhip_graph_end(void **exec) {
+    hipGraph_t graph = nullptr;
+    g.capturing = 0;
+    CHECK(hipStreamEndCapture(g.stream, &graph));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void) hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(e, "hipGraphInstantiate");
+    *exec = (void *) ge;
+    return 0;
+}
+
+int bmlhip_graph_launch(void *exec) {
+    CHECK(hipGraphLaunch((hipGraphExec_t) exec, g.stream));
+    return 0;
+}
+
+void bmlhip_graph_destroy(void *exec) {
+    if (exec) (void) hipGraphExecDestroy((hipGraphExec_t) exec);
 }
 
 int bmlhip_status_fetch(bmlhip_status *h) {

@@ -44,6 +44,34 @@ __global__ void k_sp2_check(bmlhip_sp2_state *s, const bmlhip_status *st, double
     if (nrm <= tol) s->done = 1;
 }
 
+// batched: one thread per system
+__global__ void k_sp2_decide_batch(int nsys, const int *done, const double *trx,
+                                   const double *trx2, const double *nocc, double *ab,
+                                   int *branch, const bmlhip_status *st, int *stats) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) stats[0] = st->max_touched;
+    if (s >= nsys || done[s]) return;
+    const int br = fabs(trx2[s] - nocc[s]) < fabs(2.0 * trx[s] - trx2[s] - nocc[s]) ? 0 : 1;
+    branch[s] = br;
+    ab[2 * s] = br ? 2.0 : 0.0;
+    ab[2 * s + 1] = br ? -1.0 : 1.0;
+}
+
+__global__ void k_sp2_check_batch(int nsys, int *done, const double *norm2, double tol,
+                                  int maxit, int *it, double *norm, int *alldone,
+                                  const bmlhip_status *st, int *stats) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) stats[1] = st->max_nnz;
+    if (s >= nsys || done[s]) return;
+    const double nrm = sqrt(norm2[s]);
+    norm[s] = nrm;
+    it[s] += 1;
+    if (nrm <= tol || it[s] >= maxit) {
+        done[s] = nrm <= tol ? 1 : 2;  // 1 converged, 2 iteration cap
+        atomicAdd(alldone, 1);
+    }
+}
+
 template <typename A, typename B>
 __device__ B cvt(A a);
 template <> __device__ double cvt<float, double>(float a) { return (double) a; }
@@ -80,6 +108,22 @@ int bmlhip_sp2_check(bmlhip_sp2_state *d_state, double tol, double *d_hist, int 
     hipLaunchKernelGGL(k_sp2_check, dim3(1), dim3(64), 0, (hipStream_t) bmlhip_stream(), d_state,
                        bmlhip_status_dev(), tol, d_hist, maxit);
     return launch_error("k_sp2_check");
+}
+
+int bmlhip_sp2_decide_batch(const bmlhip_batch *b, const double *nocc, int *branch,
+                            int *stats) {
+    hipLaunchKernelGGL(k_sp2_decide_batch, dim3((b->nsys + 255) / 256), dim3(256), 0,
+                       (hipStream_t) bmlhip_stream(), b->nsys, b->d_done, b->d_trx, b->d_trx2,
+                       nocc, b->d_ab, branch, bmlhip_status_dev(), stats);
+    return launch_error("k_sp2_decide_batch");
+}
+
+int bmlhip_sp2_check_batch(const bmlhip_batch *b, double tol, int maxit, int *it, double *norm,
+                           int *alldone, int *stats) {
+    hipLaunchKernelGGL(k_sp2_check_batch, dim3((b->nsys + 255) / 256), dim3(256), 0,
+                       (hipStream_t) bmlhip_stream(), b->nsys, b->d_done, b->d_norm2, tol, maxit,
+                       it, norm, alldone, bmlhip_status_dev(), stats);
+    return launch_error("k_sp2_check_batch");
 }
 
 int bmlhip_convert(int prec_in, int prec_out, size_t count, const void *in, void *out) {

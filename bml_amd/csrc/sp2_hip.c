@@ -23,12 +23,34 @@ void bml_hip_sp2_defaults(bml_hip_sp2_options *o)
     o->mixed = 0;
     o->refine_iters = 4;
     o->poll = 1;
+    o->graph = 1;
+    o->mixed_switch = 1;
 }
 
 static int single_of(int prec)
 {
     return prec == BML_HIP_DOUBLE_REAL ? BML_HIP_SINGLE_REAL
            : prec == BML_HIP_DOUBLE_COMPLEX ? BML_HIP_SINGLE_COMPLEX : 0;
+}
+
+/* Stall test of the single-precision phase: the idempotency measure read at
+ * every iteration no longer falls below its value two iterations earlier
+ * (SP2 alternates branches, and a converging iterate shrinks it at least
+ * every other step; once rounding dominates, it grows).  Needs poll == 1. */
+typedef struct {
+    int seen;
+    double prev1, prev2;
+} stall_track;
+
+static int stalled(const bml_hip_sp2_options *o, stall_track *t, double norm)
+{
+    if (!o->mixed_switch || o->poll != 1)
+        return 0;
+    const int stall = t->seen >= 2 && norm >= t->prev2;
+    t->prev2 = t->prev1;
+    t->prev1 = norm;
+    t->seen++;
+    return stall;
 }
 
 /* one SP2 iteration on (X, X2), all queued on the library stream */
@@ -139,11 +161,12 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
 
     bmlhip_sp2_state st;
     memset(&st, 0, sizeof st);
+    stall_track stall = { 0, 0.0, 0.0 };
     for (int it = 0; it < o.maxit; it++) {
         bml_matrix_ellpack_hip_t *cur = single ? Xs : X, *cur2 = single ? X2s : X2;
         if (!sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, what))
             goto out;
-        const int last_single = single && it + 1 >= o.maxit - o.refine_iters;
+        int last_single = single && it + 1 >= o.maxit - o.refine_iters;
         if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {
             if (!bmlh_ok(bmlhip_d2h(&st, d_state, sizeof st), what))
                 goto out;
@@ -156,6 +179,8 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
                 r.single_iterations = st.it;
             if (st.done)
                 break;
+            if (single && stalled(&o, &stall, st.norm))
+                last_single = 1;
         }
         if (last_single) {
             /* continue in double from the single-precision iterate */
@@ -198,6 +223,498 @@ out:
     bml_deallocate_ellpack_hip(X2);
     bml_deallocate_ellpack_hip(Xs);
     bml_deallocate_ellpack_hip(X2s);
+}
+
+/* ---- batched solves --------------------------------------------------------- */
+
+typedef struct {
+    bmlhip_batch b;
+    int *row_sys, *row_off, *done, *branch, *it, *alldone, *stats;
+    double *ab, *trx, *trx2, *norm2, *norm, *rowtr, *rownorm, *nocc, *emin, *emax;
+} batch_dev;
+
+static void batch_free(batch_dev *d)
+{
+    void *p[] = { d->row_sys, d->row_off, d->done, d->branch, d->it, d->alldone, d->stats,
+                  d->ab, d->trx, d->trx2, d->norm2, d->norm, d->rowtr, d->rownorm, d->nocc,
+                  d->emin, d->emax };
+    for (size_t k = 0; k < sizeof p / sizeof p[0]; k++)
+        bmlhip_free(p[k]);
+}
+
+static int batch_alloc(batch_dev *d, int nsys, int N, const char *what)
+{
+    memset(d, 0, sizeof *d);
+    const size_t S = (size_t) nsys, R = (size_t) N;
+    int ok = bmlh_ok(bmlhip_malloc((void **) &d->row_sys, R * sizeof(int)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->row_off, (S + 1) * sizeof(int)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->done, S * sizeof(int)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->branch, S * sizeof(int)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->it, S * sizeof(int)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->alldone, 2 * sizeof(int)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->stats, 2 * sizeof(int)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->ab, 2 * S * sizeof(double)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->trx, S * sizeof(double)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->trx2, S * sizeof(double)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->norm2, S * sizeof(double)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->norm, S * sizeof(double)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->rowtr, 2 * R * sizeof(double) + 8), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->rownorm, R * sizeof(double) + 8), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->nocc, S * sizeof(double)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->emin, S * sizeof(double)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->emax, S * sizeof(double)), what);
+    if (!ok)
+        return 0;
+    bmlhip_zero(d->done, S * sizeof(int));
+    bmlhip_zero(d->branch, S * sizeof(int));
+    bmlhip_zero(d->it, S * sizeof(int));
+    bmlhip_zero(d->alldone, 2 * sizeof(int));
+    bmlhip_zero(d->stats, 2 * sizeof(int));
+    bmlhip_zero(d->ab, 2 * S * sizeof(double));
+    bmlhip_zero(d->trx, S * sizeof(double));
+    bmlhip_zero(d->trx2, S * sizeof(double));
+    bmlhip_zero(d->norm2, S * sizeof(double));
+    bmlhip_zero(d->norm, S * sizeof(double));
+    d->b.nsys = nsys;
+    d->b.d_row_sys = d->row_sys;
+    d->b.d_row_off = d->row_off;
+    d->b.d_done = d->done;
+    d->b.d_ab = d->ab;
+    d->b.d_trx = d->trx;
+    d->b.d_trx2 = d->trx2;
+    d->b.d_norm2 = d->norm2;
+    d->b.d_rowtr = d->rowtr;
+    d->b.d_rownorm = d->rownorm;
+    return 1;
+}
+
+/* one batched iteration on (X, X2), all queued on the library stream */
+static int batch_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *X2,
+                           batch_dev *d, const bml_hip_sp2_options *o,
+                           const bmlh_spgemm_shape *sh, const char *what)
+{
+    bmlhip_spgemm_args a;
+    memset(&a, 0, sizeof a);
+    a.prec = X->matrix_precision;
+    a.kind = 0;
+    a.N = X->N;
+    a.MA = a.MB = X->M;
+    a.MC = X2->M;
+    a.d_Av = a.d_Bv = X->value;
+    a.d_Ai = a.d_Bi = X->index;
+    a.d_An = a.d_Bn = X->nnz;
+    a.d_Cv = X2->value;
+    a.d_Ci = X2->index;
+    a.d_Cn = X2->nnz;
+    a.threshold = o->threshold;
+    a.group = sh->group;
+    a.table = sh->table;
+    a.bucket = sh->bucket;
+    a.remap = sh->remap;
+    a.batch = &d->b;
+    return bmlh_ok(bmlhip_spgemm(&a), what) &&
+           bmlh_ok(bmlhip_sp2_decide_batch(&d->b, d->nocc, d->branch, d->stats), what) &&
+           bmlh_ok(bmlhip_add_batch(X->matrix_precision, X->N, X->M, X->value, X->index, X->nnz,
+                                    X2->M, X2->value, X2->index, X2->nnz, o->threshold, 2,
+                                    &d->b), what) &&
+           bmlh_ok(bmlhip_sp2_check_batch(&d->b, o->tol, o->maxit, d->it, d->norm, d->alldone,
+                                          d->stats), what);
+}
+
+void bml_hip_sp2_ellpack_batch(int nsys, bml_matrix_ellpack_hip_t *const *H,
+                               bml_matrix_ellpack_hip_t *const *X, const double *nocc,
+                               const bml_hip_sp2_options *opt, bml_hip_sp2_result *res)
+{
+    const char *what = "bml_hip_sp2_ellpack_batch";
+    bml_hip_sp2_options o;
+    if (opt)
+        o = *opt;
+    else
+        bml_hip_sp2_defaults(&o);
+    if (o.poll < 1)
+        o.poll = 1;
+    if (o.maxit < 1)
+        o.maxit = 1;
+    if (nsys < 1 || !H || !X || !nocc) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: nsys=%d", what, nsys);
+        return;
+    }
+    const int prec = X[0] ? X[0]->matrix_precision : 0, M = X[0] ? X[0]->M : 0;
+    long long Ntot = 0;
+    for (int s = 0; s < nsys; s++) {
+        if (!H[s] || !X[s] || H[s]->matrix_type != BML_HIP_TYPE_ELLPACK ||
+            X[s]->matrix_type != BML_HIP_TYPE_ELLPACK || H[s]->N != X[s]->N ||
+            H[s]->matrix_precision != prec || X[s]->matrix_precision != prec || X[s]->M != M ||
+            H[s] == X[s]) {
+            bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: system %d: H and X must be distinct "
+                       "ellpack_hip matrices of equal N, one precision and one M for every X",
+                       what, s);
+            return;
+        }
+        Ntot += H[s]->N;
+    }
+    if (Ntot >= (1ll << 24)) {
+        bmlh_error(BML_HIP_ERR_UNSUPPORTED, "%s: %lld rows in all exceed 2^24", what, Ntot);
+        return;
+    }
+    const int N = (int) Ntot;
+    const int sprec = single_of(prec);
+    if (o.mixed && !sprec) {
+        bmlh_error(BML_HIP_ERR_UNSUPPORTED, "%s: mixed precision needs double precision", what);
+        return;
+    }
+    int *row_off = malloc(sizeof(int) * ((size_t) nsys + 1));
+    int *row_sys = malloc(sizeof(int) * ((size_t) N + 1));
+    int *h_it = malloc(sizeof(int) * (size_t) nsys);
+    int *h_done = malloc(sizeof(int) * (size_t) nsys);
+    double *h_d = malloc(sizeof(double) * 5 * (size_t) nsys);
+    batch_dev d;
+    memset(&d, 0, sizeof d);
+    bml_matrix_ellpack_hip_t *Xc = NULL, *X2c = NULL, *Xs = NULL, *X2s = NULL;
+    void *graph = NULL;
+    bmlh_spgemm_shape gshape;
+    memset(&gshape, 0, sizeof gshape);
+    if (!row_off || !row_sys || !h_it || !h_done || !h_d) {
+        bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+        goto out;
+    }
+    row_off[0] = 0;
+    for (int s = 0; s < nsys; s++) {
+        row_off[s + 1] = row_off[s] + H[s]->N;
+        for (int i = row_off[s]; i < row_off[s + 1]; i++)
+            row_sys[i] = s;
+    }
+    if (!batch_alloc(&d, nsys, N, what))
+        goto out;
+    if (!bmlh_ok(bmlhip_h2d(d.row_off, row_off, sizeof(int) * ((size_t) nsys + 1)), what) ||
+        !bmlh_ok(bmlhip_h2d(d.row_sys, row_sys, sizeof(int) * (size_t) N), what) ||
+        !bmlh_ok(bmlhip_h2d(d.nocc, nocc, sizeof(double) * (size_t) nsys), what))
+        goto out;
+
+    /* pack H into one block-diagonal matrix with X's M, then per-system
+     * Gershgorin bounds and X0 = (emax I - H)/(emax - emin) */
+    Xc = bml_zero_matrix_ellpack_hip(prec, N, M, X[0]->distribution_mode);
+    X2c = bml_noinit_matrix_ellpack_hip(prec, N, M, X[0]->distribution_mode);
+    if (!Xc || !X2c)
+        goto out;
+    {
+        const size_t es = bmlh_elem_size(prec);
+        int hmax = 0;
+        for (int s = 0; s < nsys; s++) {
+            const size_t r0 = (size_t) row_off[s];
+            if (!bmlh_ok(bmlhip_copy_rows_shift(prec, H[s]->N, H[s]->M, H[s]->value,
+                                                H[s]->index, H[s]->nnz, M,
+                                                (char *) Xc->value + r0 * M * es,
+                                                Xc->index + r0 * M, Xc->nnz + r0, row_off[s]),
+                         what))
+                goto out;
+            const int hm = H[s]->aux->stats_valid ? H[s]->aux->max_nnz : H[s]->M;
+            if (hm > hmax)
+                hmax = hm;
+        }
+        bmlhip_status st;
+        if (!bmlh_finish_sync(what, &st))
+            goto out;
+        Xc->aux->stats_valid = 1;
+        Xc->aux->max_nnz = hmax + 1;
+    }
+    if (!bmlh_ok(bmlhip_gershgorin_batch(prec, N, M, Xc->value, Xc->index, Xc->nnz, &d.b,
+                                         d.emin, d.emax), what) ||
+        !bmlh_ok(bmlhip_normalize_batch(prec, N, M, Xc->value, Xc->index, Xc->nnz, &d.b,
+                                        d.emin, d.emax), what))
+        goto out;
+
+    int single = 0;
+    if (o.mixed && o.maxit > o.refine_iters) {
+        Xs = bml_noinit_matrix_ellpack_hip(sprec, N, M, Xc->distribution_mode);
+        X2s = bml_noinit_matrix_ellpack_hip(sprec, N, M, Xc->distribution_mode);
+        if (!Xs || !X2s)
+            goto out;
+        if (!bmlh_ok(bmlhip_convert(prec, sprec, (size_t) N * M, Xc->value, Xs->value), what) ||
+            !bmlh_ok(bmlhip_d2d(Xs->index, Xc->index, (size_t) N * M * sizeof(int)), what) ||
+            !bmlh_ok(bmlhip_d2d(Xs->nnz, Xc->nnz, (size_t) N * sizeof(int)), what))
+            goto out;
+        *Xs->aux = *Xc->aux;
+        single = 1;
+    }
+
+    int touched = 0, hint_max = 0, stats[2] = { 0, 0 }, alldone[2] = { 0, 0 };
+    for (int it = 0; it < o.maxit; it++) {
+        bml_matrix_ellpack_hip_t *cur = single ? Xs : Xc, *cur2 = single ? X2s : X2c;
+        const int mx = cur->aux->stats_valid ? cur->aux->max_nnz : cur->M;
+        bmlh_spgemm_shape sh;
+        bmlh_pick_spgemm(cur->matrix_precision, N, mx, mx, touched, hint_max, &sh);
+        if (o.graph) {
+            /* replay the iteration as a graph; capture anew when the launch
+             * shape or the precision changes (the first pass runs eagerly so
+             * every scratch buffer is sized before capture) */
+            const int same = graph && sh.group == gshape.group && sh.table == gshape.table &&
+                             sh.bucket == gshape.bucket && sh.remap == gshape.remap;
+            if (!same) {
+                bmlhip_graph_destroy(graph);
+                graph = NULL;
+                if (!batch_iteration(cur, cur2, &d, &o, &sh, what))
+                    goto out;
+                if (!bmlh_ok(bmlhip_graph_begin(), what))
+                    goto out;
+                const int okc = batch_iteration(cur, cur2, &d, &o, &sh, what);
+                void *ge = NULL;
+                const int oke = bmlh_ok(bmlhip_graph_end(&ge), what);
+                if (!okc || !oke)
+                    goto out;
+                graph = ge;
+                gshape = sh;
+                /* the captured kernels were not run: the eager pass above was
+                 * this iteration */
+            } else if (!bmlh_ok(bmlhip_graph_launch(graph), what)) {
+                goto out;
+            }
+        } else if (!batch_iteration(cur, cur2, &d, &o, &sh, what)) {
+            goto out;
+        }
+        const int last_single = single && it + 1 >= o.maxit - o.refine_iters;
+        if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {
+            if (!bmlh_ok(bmlhip_d2h(alldone, d.alldone, sizeof alldone), what) ||
+                !bmlh_ok(bmlhip_d2h(stats, d.stats, sizeof stats), what))
+                goto out;
+            touched = stats[0];
+            hint_max = mx;
+            cur->aux->stats_valid = 1;
+            cur->aux->max_nnz = stats[1];
+            if (alldone[0] >= nsys)
+                break;
+        }
+        if (last_single) {
+            if (!bmlh_ok(bmlhip_convert(sprec, prec, (size_t) N * M, Xs->value, Xc->value), what) ||
+                !bmlh_ok(bmlhip_d2d(Xc->index, Xs->index, (size_t) N * M * sizeof(int)), what) ||
+                !bmlh_ok(bmlhip_d2d(Xc->nnz, Xs->nnz, (size_t) N * sizeof(int)), what))
+                goto out;
+            *Xc->aux = *Xs->aux;
+            single = 0;
+            bmlhip_graph_destroy(graph);
+            graph = NULL;
+        }
+    }
+    if (single) {
+        if (!bmlh_ok(bmlhip_convert(sprec, prec, (size_t) N * M, Xs->value, Xc->value), what) ||
+            !bmlh_ok(bmlhip_d2d(Xc->index, Xs->index, (size_t) N * M * sizeof(int)), what) ||
+            !bmlh_ok(bmlhip_d2d(Xc->nnz, Xs->nnz, (size_t) N * sizeof(int)), what))
+            goto out;
+    }
+    /* unpack every system's rows into its X */
+    {
+        const size_t es = bmlh_elem_size(prec);
+        for (int s = 0; s < nsys; s++) {
+            const size_t r0 = (size_t) row_off[s];
+            if (!bmlh_ok(bmlhip_copy_rows_shift(prec, X[s]->N, M, (char *) Xc->value + r0 * M * es,
+                                                Xc->index + r0 * M, Xc->nnz + r0, M, X[s]->value,
+                                                X[s]->index, X[s]->nnz, -row_off[s]), what))
+                goto out;
+            X[s]->aux->stats_valid = 0;
+        }
+        bmlhip_status st;
+        if (!bmlh_finish_sync(what, &st))
+            goto out;
+    }
+    if (res) {
+        if (!bmlh_ok(bmlhip_d2h(h_it, d.it, sizeof(int) * (size_t) nsys), what) ||
+            !bmlh_ok(bmlhip_d2h(h_done, d.done, sizeof(int) * (size_t) nsys), what) ||
+            !bmlh_ok(bmlhip_d2h(h_d, d.trx, sizeof(double) * (size_t) nsys), what) ||
+            !bmlh_ok(bmlhip_d2h(h_d + nsys, d.trx2, sizeof(double) * (size_t) nsys), what) ||
+            !bmlh_ok(bmlhip_d2h(h_d + 2 * nsys, d.norm, sizeof(double) * (size_t) nsys), what) ||
+            !bmlh_ok(bmlhip_d2h(h_d + 3 * nsys, d.emin, sizeof(double) * (size_t) nsys), what) ||
+            !bmlh_ok(bmlhip_d2h(h_d + 4 * nsys, d.emax, sizeof(double) * (size_t) nsys), what))
+            goto out;
+        for (int s = 0; s < nsys; s++) {
+            bml_hip_sp2_result *r = &res[s];
+            memset(r, 0, sizeof *r);
+            r->iterations = h_it[s];
+            r->single_iterations = o.mixed ? (h_it[s] < o.maxit - o.refine_iters
+                                                  ? h_it[s] : o.maxit - o.refine_iters)
+                                           : 0;
+            r->converged = h_done[s] == 1;
+            r->trace_x = h_d[s];
+            r->trace_x2 = h_d[nsys + s];
+            r->idempotency = h_d[2 * nsys + s];
+            r->emin = h_d[3 * nsys + s];
+            r->emax = h_d[4 * nsys + s];
+        }
+    }
+out:
+    bmlhip_graph_destroy(graph);
+    batch_free(&d);
+    bml_deallocate_ellpack_hip(Xc);
+    bml_deallocate_ellpack_hip(X2c);
+    bml_deallocate_ellpack_hip(Xs);
+    bml_deallocate_ellpack_hip(X2s);
+    free(row_off);
+    free(row_sys);
+    free(h_it);
+    free(h_done);
+    free(h_d);
+}
+
+/* ---- ELLBLOCK ------------------------------------------------------------- */
+
+static int eb_sp2_iteration(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hip_t *X2,
+                            bmlhip_sp2_state *d_state, double *d_hist,
+                            const bml_hip_sp2_options *o, double nocc, const char *what)
+{
+    bmlhip_eb_args a;
+    memset(&a, 0, sizeof a);
+    a.prec = X->matrix_precision;
+    a.kind = 0;
+    a.NB = X->NB;
+    a.bt = X->bt;
+    a.d_bsize = X->bsize;
+    a.MA = a.MB = X->MB;
+    a.MC = X2->MB;
+    a.d_Av = a.d_Bv = X->value;
+    a.d_Ai = a.d_Bi = X->indexb;
+    a.d_An = a.d_Bn = X->nnzb;
+    a.d_Cv = X2->value;
+    a.d_Ci = X2->indexb;
+    a.d_Cn = X2->nnzb;
+    a.threshold = o->threshold;
+    a.engine = bmlh_block_engine_for(X);
+    a.d_skip = &d_state->done;
+    return bmlh_ok(bmlhip_eb_product(&a), what) &&
+           bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
+           bmlh_ok(bmlhip_eb_add(X->matrix_precision, X->NB, X->bt, X->bsize, X->MB, X->value,
+                                 X->indexb, X->nnzb, X2->MB, X2->value, X2->indexb, X2->nnzb,
+                                 0.0, 0.0, o->threshold, 2, d_state->ab, &d_state->done),
+                   what) &&
+           bmlh_ok(bmlhip_sp2_check(d_state, o->tol, d_hist, o->maxit), what);
+}
+
+/* values, block columns and counts of an ELLBLOCK matrix into one of the
+ * same shape in another precision */
+static int eb_convert(const bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_t *B,
+                      const char *what)
+{
+    const size_t slots = (size_t) A->NB * A->MB;
+    return bmlh_ok(bmlhip_convert(A->matrix_precision, B->matrix_precision,
+                                  slots * A->bt * A->bt, A->value, B->value), what) &&
+           bmlh_ok(bmlhip_d2d(B->indexb, A->indexb, slots * sizeof(int)), what) &&
+           bmlh_ok(bmlhip_d2d(B->nnzb, A->nnzb, (size_t) A->NB * sizeof(int)), what);
+}
+
+void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellblock_hip_t *X,
+                          double nocc, const bml_hip_sp2_options *opt,
+                          bml_hip_sp2_result *res, double *history)
+{
+    const char *what = "bml_hip_sp2_ellblock";
+    bml_hip_sp2_options o;
+    if (opt)
+        o = *opt;
+    else
+        bml_hip_sp2_defaults(&o);
+    if (o.poll < 1)
+        o.poll = 1;
+    if (o.maxit < 1)
+        o.maxit = 1;
+    if (!H || !X || H->matrix_type != BML_HIP_TYPE_ELLBLOCK ||
+        X->matrix_type != BML_HIP_TYPE_ELLBLOCK || H->NB != X->NB || H->bt != X->bt ||
+        H->matrix_precision != X->matrix_precision || H == X ||
+        memcmp(H->bsize_host, X->bsize_host, sizeof(int) * (size_t) H->NB) != 0) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: H and X must be distinct ellblock_hip matrices "
+                   "of equal block structure and precision", what);
+        return;
+    }
+    const int prec = X->matrix_precision, NB = X->NB, MB = X->MB;
+    const int sprec = single_of(prec);
+    if (o.mixed && !sprec) {
+        bmlh_error(BML_HIP_ERR_UNSUPPORTED, "%s: mixed precision needs a double-precision X",
+                   what);
+        return;
+    }
+    bml_hip_sp2_result r;
+    memset(&r, 0, sizeof r);
+    double *g = bml_gershgorin_ellblock_hip(H);
+    if (!g)
+        return;
+    r.emin = g[0];
+    r.emax = g[1];
+    free(g);
+    bml_copy_ellblock_hip(H, X);
+    if (bmlh_failed())
+        return;
+    bml_normalize_ellblock_hip(X, r.emin, r.emax);
+    if (bmlh_failed())
+        return;
+
+    bml_matrix_ellblock_hip_t *X2 = NULL, *Xs = NULL, *X2s = NULL;
+    bmlhip_sp2_state *d_state = NULL;
+    double *d_hist = NULL;
+    X2 = bml_block_matrix_ellblock_hip(prec, NB, MB, X->M, X->bsize_host, X->distribution_mode);
+    if (!X2)
+        goto out;
+    if (!bmlh_ok(bmlhip_malloc((void **) &d_state, sizeof(bmlhip_sp2_state)), what) ||
+        !bmlh_ok(bmlhip_malloc((void **) &d_hist, sizeof(double) * 4 * (size_t) o.maxit), what))
+        goto out;
+    bmlhip_zero(d_state, sizeof(bmlhip_sp2_state));
+    bmlhip_zero(d_hist, sizeof(double) * 4 * (size_t) o.maxit);
+
+    int single = 0;
+    if (o.mixed && o.maxit > o.refine_iters) {
+        Xs = bml_block_matrix_ellblock_hip(sprec, NB, MB, X->M, X->bsize_host,
+                                           X->distribution_mode);
+        X2s = bml_block_matrix_ellblock_hip(sprec, NB, MB, X->M, X->bsize_host,
+                                            X->distribution_mode);
+        if (!Xs || !X2s || !eb_convert(X, Xs, what))
+            goto out;
+        single = 1;
+    }
+
+    bmlhip_sp2_state st;
+    memset(&st, 0, sizeof st);
+    stall_track stall = { 0, 0.0, 0.0 };
+    for (int it = 0; it < o.maxit; it++) {
+        bml_matrix_ellblock_hip_t *cur = single ? Xs : X, *cur2 = single ? X2s : X2;
+        if (!eb_sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, what))
+            goto out;
+        int last_single = single && it + 1 >= o.maxit - o.refine_iters;
+        if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {
+            if (!bmlh_ok(bmlhip_d2h(&st, d_state, sizeof st), what))
+                goto out;
+            if (single)
+                r.single_iterations = st.it;
+            if (st.done)
+                break;
+            if (single && stalled(&o, &stall, st.norm))
+                last_single = 1;
+        }
+        if (last_single) {
+            if (!eb_convert(Xs, X, what))
+                goto out;
+            single = 0;
+        }
+    }
+    if (!bmlh_ok(bmlhip_d2h(&st, d_state, sizeof st), what))
+        goto out;
+    if (single && !eb_convert(Xs, X, what))
+        goto out;
+    {
+        bmlhip_status s;
+        if (!bmlh_finish_sync(what, &s))
+            goto out;
+    }
+    X->aux->stats_valid = 0;
+    r.iterations = st.it;
+    r.converged = st.done;
+    r.trace_x = st.trx;
+    r.trace_x2 = st.trx2;
+    r.idempotency = st.norm;
+    if (history)
+        bmlh_ok(bmlhip_d2h(history, d_hist, sizeof(double) * 4 * (size_t) o.maxit), what);
+    if (res)
+        *res = r;
+out:
+    bmlhip_free(d_state);
+    bmlhip_free(d_hist);
+    bml_deallocate_ellblock_hip(X2);
+    bml_deallocate_ellblock_hip(Xs);
+    bml_deallocate_ellblock_hip(X2s);
 }
 
 void bml_hip_profile_begin(void)

@@ -53,7 +53,8 @@ class _Blk(C.Structure):
 
 class Sp2Options(C.Structure):
     _fields_ = [("threshold", C.c_double), ("tol", C.c_double), ("maxit", C.c_int),
-                ("mixed", C.c_int), ("refine_iters", C.c_int), ("poll", C.c_int)]
+                ("mixed", C.c_int), ("refine_iters", C.c_int), ("poll", C.c_int),
+                ("graph", C.c_int), ("mixed_switch", C.c_int)]
 
 
 class Sp2Result(C.Structure):
@@ -114,12 +115,20 @@ _SIGS = {
     "bml_ellpack_hip_row_stats": (None, [_P, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     "bml_hip_sp2_defaults": (None, [C.POINTER(Sp2Options)]),
     "bml_hip_sp2_ellpack": (None, [_P, _P, _D, C.POINTER(Sp2Options), C.POINTER(Sp2Result), _P]),
+    "bml_hip_sp2_ellpack_batch": (None, [_I, _P, _P, _P, C.POINTER(Sp2Options), _P]),
     "bml_hip_profile_begin": (None, []),
     "bml_hip_profile_end": (None, [C.POINTER(Profile)]),
 }
-_OPTIONAL = {
+_SIGS.update({
+    "bml_hip_set_block_engine": (None, [_I]),
+    "bml_hip_set_block_sizes": (None, [_P, _I, _I]),
+    "bml_hip_get_block_sizes": (_P, [_I, _I]),
+    "bml_hip_get_nb": (_I, []),
+    "bml_hip_get_mb": (_I, []),
     "bml_block_matrix_ellblock_hip": (_P, [_I, _I, _I, _I, _P, _I]),
+    "bml_zero_matrix_ellblock_hip": (_P, [_I, _I, _I, _I]),
     "bml_deallocate_ellblock_hip": (None, [_P]),
+    "bml_clear_ellblock_hip": (None, [_P]),
     "bml_copy_ellblock_hip_new": (_P, [_P]),
     "bml_copy_ellblock_hip": (None, [_P, _P]),
     "bml_ellblock_hip_set_blocks": (None, [_P, _P, _P, _P, _I, _I]),
@@ -131,7 +140,15 @@ _OPTIONAL = {
     "bml_add_ellblock_hip": (None, [_P, _P, _D, _D, _D]),
     "bml_threshold_ellblock_hip": (None, [_P, _D]),
     "bml_trace_ellblock_hip": (_D, [_P]),
-}
+    "bml_gershgorin_ellblock_hip": (_P, [_P]),
+    "bml_normalize_ellblock_hip": (None, [_P, _D, _D]),
+    "bml_scale_inplace_ellblock_hip": (None, [_P, _P]),
+    "bml_add_identity_ellblock_hip": (None, [_P, _D, _D]),
+    "bml_hip_sp2_ellblock": (None, [_P, _P, _D, C.POINTER(Sp2Options), C.POINTER(Sp2Result), _P]),
+})
+_OPTIONAL: dict = {}
+
+BLOCK_ENGINE_AUTO, BLOCK_ENGINE_EXACT, BLOCK_ENGINE_MFMA, BLOCK_ENGINE_MFMA_BF16 = -1, 0, 1, 2
 
 
 def header_symbols() -> list[str]:
@@ -383,16 +400,35 @@ def normalize(A: EllpackHip, mineval: float, maxeval: float) -> None:
 
 def sp2(H: EllpackHip, X: EllpackHip, nocc: float, threshold: float = 1e-5,
         tol: float = 1e-8, maxit: int = 60, mixed: bool = False, refine_iters: int = 4,
-        poll: int = 1):
+        poll: int = 1, mixed_switch: bool = True):
     """X <- SP2 density matrix of H; returns (Sp2Result, history[it, 4])."""
     o = Sp2Options()
     lib().bml_hip_sp2_defaults(C.byref(o))
     o.threshold, o.tol, o.maxit = threshold, tol, maxit
     o.mixed, o.refine_iters, o.poll = int(mixed), refine_iters, poll
+    o.mixed_switch = int(mixed_switch)
     r = Sp2Result()
     hist = np.zeros((maxit, 4))
     _call("bml_hip_sp2_ellpack", H.ptr, X.ptr, nocc, C.byref(o), C.byref(r), hist.ctypes.data)
     return r, hist[: r.iterations]
+
+
+def sp2_batch(Hs, Xs, nocc, threshold: float = 1e-5, tol: float = 1e-8, maxit: int = 60,
+              mixed: bool = False, refine_iters: int = 4, poll: int = 1, graph: bool = True):
+    """Independent SP2 solves of the ellpack_hip pairs (Hs[s], Xs[s]) in one
+    batch; returns the list of Sp2Result."""
+    n = len(Hs)
+    o = Sp2Options()
+    lib().bml_hip_sp2_defaults(C.byref(o))
+    o.threshold, o.tol, o.maxit = threshold, tol, maxit
+    o.mixed, o.refine_iters, o.poll, o.graph = int(mixed), refine_iters, poll, int(graph)
+    hp = (C.c_void_p * n)(*[h.ptr.value for h in Hs])
+    xp = (C.c_void_p * n)(*[x.ptr.value for x in Xs])
+    oc = np.ascontiguousarray(np.broadcast_to(np.asarray(nocc, np.float64), (n,)))
+    res = (Sp2Result * n)()
+    _call("bml_hip_sp2_ellpack_batch", n, C.cast(hp, C.c_void_p), C.cast(xp, C.c_void_p),
+          oc.ctypes.data, C.byref(o), C.cast(res, C.c_void_p))
+    return list(res)
 
 
 def profile_begin() -> None:
@@ -403,3 +439,173 @@ def profile_end() -> Profile:
     p = Profile()
     _call("bml_hip_profile_end", C.byref(p))
     return p
+
+
+# ---- ELLBLOCK ---------------------------------------------------------------
+
+class EllblockHip:
+    """Handle to a device-resident ellblock_hip matrix."""
+
+    def __init__(self, ptr):
+        if not ptr:
+            _check()
+            raise BmlError(2, "allocation returned NULL")
+        self.ptr = C.c_void_p(ptr)
+
+    @property
+    def _s(self) -> _Blk:
+        return C.cast(self.ptr, C.POINTER(_Blk)).contents
+
+    @property
+    def N(self) -> int:
+        return self._s.N
+
+    @property
+    def NB(self) -> int:
+        return self._s.NB
+
+    @property
+    def MB(self) -> int:
+        return self._s.MB
+
+    @property
+    def bt(self) -> int:
+        return self._s.bt
+
+    @property
+    def precision(self) -> int:
+        return self._s.matrix_precision
+
+    @property
+    def dtype(self):
+        return np.dtype(DTYPES[self.precision])
+
+    @property
+    def bsize(self) -> np.ndarray:
+        return np.ctypeslib.as_array(C.cast(self._s.bsize_host, C.POINTER(C.c_int)),
+                                     shape=(self.NB,)).copy()
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().bml_deallocate_ellblock_hip(self.ptr)
+            self.ptr = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            if _lib is not None and self.ptr:
+                self.free()
+        except Exception:
+            pass
+
+    def set(self, A: Blk) -> "EllblockHip":
+        if A.NB != self.NB or A.dtype != self.dtype:
+            raise BmlError(2, "shape/precision mismatch")
+        v = np.ascontiguousarray(A.value)
+        ix = np.ascontiguousarray(A.indexb, np.int32)
+        nn = np.ascontiguousarray(A.nnzb, np.int32)
+        _call("bml_ellblock_hip_set_blocks", self.ptr, v.ctypes.data, ix.ctypes.data,
+              nn.ctypes.data, A.MB, A.bmax)
+        return self
+
+    def get(self, bmax: int | None = None) -> Blk:
+        bsize = self.bsize
+        bmax = int(bmax or bsize.max())
+        out = Blk.zeros(bsize, self.MB, self.dtype, bmax)
+        _call("bml_ellblock_hip_get_blocks", self.ptr, out.value.ctypes.data,
+              out.indexb.ctypes.data, out.nnzb.ctypes.data, bmax)
+        for ib in range(self.NB):
+            out.indexb[ib, out.nnzb[ib]:] = 0
+        return out
+
+
+def set_block_engine(engine: int) -> None:
+    _call("bml_hip_set_block_engine", engine)
+
+
+def block_matrix(precision: int, bsize, MB: int, M: int = 0) -> EllblockHip:
+    b = np.ascontiguousarray(bsize, np.int32)
+    return EllblockHip(_call("bml_block_matrix_ellblock_hip", precision, len(b), MB,
+                             M or int(b.sum()), b.ctypes.data, 0))
+
+
+def from_blk(A: Blk, MB: int | None = None) -> EllblockHip:
+    return block_matrix(A.precision, A.bsize, MB or A.MB).set(A)
+
+
+def eb_export_to_dense(A: EllblockHip, order: int = 0) -> np.ndarray:
+    p = _call("bml_export_to_dense_ellblock_hip", A.ptr, order)
+    n = A.N * A.N
+    buf = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_ubyte)), shape=(n * A.dtype.itemsize,))
+    out = np.frombuffer(buf.tobytes(), A.dtype).reshape(A.N, A.N)
+    _libc.free(p)
+    return out
+
+
+def eb_copy_new(A: EllblockHip) -> EllblockHip:
+    return EllblockHip(_call("bml_copy_ellblock_hip_new", A.ptr))
+
+
+def eb_copy(A: EllblockHip, B: EllblockHip) -> None:
+    _call("bml_copy_ellblock_hip", A.ptr, B.ptr)
+
+
+def eb_clear(A: EllblockHip) -> None:
+    _call("bml_clear_ellblock_hip", A.ptr)
+
+
+def eb_multiply_x2(X: EllblockHip, X2: EllblockHip, threshold: float):
+    """X2 <- X*X blockwise; returns (tr X, tr X^2)."""
+    return _take2(_call("bml_multiply_x2_ellblock_hip", X.ptr, X2.ptr, threshold))
+
+
+def eb_multiply(A: EllblockHip, B: EllblockHip, Cm: EllblockHip, alpha: float = 1.0,
+                beta: float = 0.0, threshold: float = 0.0) -> None:
+    _call("bml_multiply_ellblock_hip", A.ptr, B.ptr, Cm.ptr, alpha, beta, threshold)
+
+
+def eb_multiply_AB(A: EllblockHip, B: EllblockHip, Cm: EllblockHip, threshold: float) -> None:
+    _call("bml_multiply_AB_ellblock_hip", A.ptr, B.ptr, Cm.ptr, threshold)
+
+
+def eb_add(A: EllblockHip, B: EllblockHip, alpha: float, beta: float, threshold: float) -> None:
+    _call("bml_add_ellblock_hip", A.ptr, B.ptr, alpha, beta, threshold)
+
+
+def eb_threshold(A: EllblockHip, thr: float) -> None:
+    _call("bml_threshold_ellblock_hip", A.ptr, thr)
+
+
+def eb_trace(A: EllblockHip) -> float:
+    return _call("bml_trace_ellblock_hip", A.ptr)
+
+
+def eb_gershgorin(A: EllblockHip):
+    return _take2(_call("bml_gershgorin_ellblock_hip", A.ptr))
+
+
+def eb_normalize(A: EllblockHip, mineval: float, maxeval: float) -> None:
+    _call("bml_normalize_ellblock_hip", A.ptr, mineval, maxeval)
+
+
+def eb_scale_inplace(scale, A: EllblockHip) -> None:
+    s = np.array([scale], A.dtype)
+    _call("bml_scale_inplace_ellblock_hip", s.ctypes.data, A.ptr)
+
+
+def eb_add_identity(A: EllblockHip, beta: float, threshold: float) -> None:
+    _call("bml_add_identity_ellblock_hip", A.ptr, beta, threshold)
+
+
+def eb_sp2(H: EllblockHip, X: EllblockHip, nocc: float, threshold: float = 1e-5,
+           tol: float = 1e-8, maxit: int = 60, mixed: bool = False, refine_iters: int = 4,
+           poll: int = 1, mixed_switch: bool = True):
+    """X <- SP2 density matrix of the ELLBLOCK H; returns (Sp2Result, history[it, 4])."""
+    o = Sp2Options()
+    lib().bml_hip_sp2_defaults(C.byref(o))
+    o.threshold, o.tol, o.maxit = threshold, tol, maxit
+    o.mixed, o.refine_iters, o.poll = int(mixed), refine_iters, poll
+    o.mixed_switch = int(mixed_switch)
+    r = Sp2Result()
+    hist = np.zeros((maxit, 4))
+    _call("bml_hip_sp2_ellblock", H.ptr, X.ptr, nocc, C.byref(o), C.byref(r), hist.ctypes.data)
+    return r, hist[: r.iterations]

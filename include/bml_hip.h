@@ -86,6 +86,30 @@ typedef struct bml_matrix_ellpack_hip_t {
     struct bml_hip_aux *aux; /* library-private bookkeeping */
 } bml_matrix_ellpack_hip_t;
 
+/* ELLBLOCK matrix resident in HBM.  Block slot (ib, jp) owns one bt x bt
+ * tile, bt = 8, 16 or 32 (the smallest that holds every block size), with
+ * the bsize[ib] x bsize[jb] block in its top-left corner and zeros
+ * elsewhere: value[((ib*MB + jp)*bt + ii)*bt + jj].  The tile replaces the
+ * per-block pointers into a memory pool of bml_matrix_ellblock_t
+ * (ellblock/bml_types_ellblock.h:11-56); the first seven members keep that
+ * struct's order. */
+typedef struct bml_matrix_ellblock_hip_t {
+    int matrix_type;       /* BML_HIP_TYPE_ELLBLOCK */
+    int matrix_precision;  /* bml_matrix_precision_t */
+    int distribution_mode; /* sequential only */
+    int N;                 /* sum of bsize */
+    int M;                 /* bml's nominal non-zeroes per row */
+    int NB;                /* block rows */
+    int MB;                /* block slots per block row */
+    int bt;                /* tile edge */
+    void *value;           /* device: NB*MB*bt*bt elements */
+    int *indexb;           /* device: NB*MB block columns */
+    int *nnzb;             /* device: NB live blocks per block row */
+    int *bsize;            /* device: NB block sizes */
+    int *bsize_host;       /* host copy of bsize */
+    struct bml_hip_aux *aux; /* library-private bookkeeping */
+} bml_matrix_ellblock_hip_t;
+
 /* ---- runtime ------------------------------------------------------------ */
 
 /* Select the device and create the library stream; called implicitly by the
@@ -199,6 +223,102 @@ void bml_ellpack_hip_row_stats(const bml_matrix_ellpack_hip_t *A, int *max_nnz,
                                long long *total_nnz);
 
 
+/* ---- ELLBLOCK ----------------------------------------------------------- */
+
+/* Block products: the engine that multiplies bt x bt tiles.
+ *   EXACT   each element's length-bsize dot product left to right, then
+ *           added, in the reference's order (bml_multiply_block1..6,
+ *           ellblock/bml_multiply_ellblock_typed.c:28-168): bit-identical
+ *           to the reference's block kernels; any precision and tile;
+ *   MFMA    matrix cores (v_mfma_f32_32x32x2f32 for single real,
+ *           v_mfma_f64_16x16x4f64 for double real) on 32 x 32 tiles;
+ *   MFMA_BF16  single real with inputs rounded to bfloat16 and FP32
+ *           accumulation (v_mfma_f32_32x32x16_bf16); double-precision
+ *           products under this setting use MFMA;
+ *   AUTO    MFMA for real precisions with 32 x 32 tiles, else EXACT.
+ * No bml counterpart: the reference picks its kernel by block size
+ * (libxsmm above 6, :380). */
+#define BML_HIP_BLOCK_ENGINE_AUTO -1
+#define BML_HIP_BLOCK_ENGINE_EXACT 0
+#define BML_HIP_BLOCK_ENGINE_MFMA 1
+#define BML_HIP_BLOCK_ENGINE_MFMA_BF16 2
+void bml_hip_set_block_engine(int engine);
+
+/* Block-size registry: bml_set_block_sizes / bml_get_block_sizes /
+ * bml_get_nb / bml_get_mb (ellblock/bml_allocate_ellblock.c:24-91).  Set at
+ * most once (the reference asserts); unset, the default is blocks of 4 rows,
+ * nb = ceil(N/4), mb = max(4, ceil(M/4)) capped at nb, the last block taking
+ * the remainder. */
+void bml_hip_set_block_sizes(const int *bsize, int nb, int mb);
+const int *bml_hip_get_block_sizes(int N, int M);
+int bml_hip_get_nb(void);
+int bml_hip_get_mb(void);
+
+/* bml_block_matrix_ellblock (ellblock/bml_allocate_ellblock_typed.c:176-243):
+ * zero matrix with NB block rows of the given sizes (each <= 32) and MB
+ * slots per block row; registers the sizes when none are set. */
+bml_matrix_ellblock_hip_t *bml_block_matrix_ellblock_hip(
+    int precision, int NB, int MB, int M, const int *bsize, int distrib_mode);
+/* bml_zero_matrix_ellblock (bml_allocate_ellblock_typed.c:245-262): block
+ * sizes from the registry */
+bml_matrix_ellblock_hip_t *bml_zero_matrix_ellblock_hip(int precision, int N, int M,
+                                                        int distrib_mode);
+/* bml_deallocate_ellblock (ellblock/bml_allocate_ellblock.c) */
+void bml_deallocate_ellblock_hip(bml_matrix_ellblock_hip_t *A);
+/* bml_clear_ellblock (bml_allocate_ellblock_typed.c:127-145) */
+void bml_clear_ellblock_hip(bml_matrix_ellblock_hip_t *A);
+/* bml_copy_ellblock_new / bml_copy_ellblock (ellblock/bml_copy_ellblock_typed.c);
+ * B may have a different MB as long as every block row fits. */
+bml_matrix_ellblock_hip_t *bml_copy_ellblock_hip_new(const bml_matrix_ellblock_hip_t *A);
+void bml_copy_ellblock_hip(const bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_t *B);
+/* Upload/download blocks as bmax x bmax host tiles per slot
+ * (value[((ib*src_MB + jp)*bmax + ii)*bmax + jj]), block columns and
+ * counts; bmax >= every block size. */
+void bml_ellblock_hip_set_blocks(bml_matrix_ellblock_hip_t *A, const void *value,
+                                 const int *indexb, const int *nnzb, int src_MB, int bmax);
+void bml_ellblock_hip_get_blocks(const bml_matrix_ellblock_hip_t *A, void *value,
+                                 int *indexb, int *nnzb, int bmax);
+/* bml_multiply_x2_ellblock (ellblock/bml_multiply_ellblock_typed.c:246-442):
+ * X2 <- X*X blockwise; a product block is kept iff it is the diagonal block
+ * or its max |element| > threshold (no elementwise threshold); fatal if a
+ * block row touches more than X2->MB blocks; returns malloc'd
+ * double[2] = {tr X, tr X^2}. */
+void *bml_multiply_x2_ellblock_hip(bml_matrix_ellblock_hip_t *X,
+                                   bml_matrix_ellblock_hip_t *X2, double threshold);
+/* bml_multiply_ellblock (bml_multiply_ellblock_typed.c:183-231):
+ * C <- alpha*A*B + beta*C. */
+void bml_multiply_ellblock_hip(bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_t *B,
+                               bml_matrix_ellblock_hip_t *C, double alpha, double beta,
+                               double threshold);
+/* bml_multiply_AB_ellblock (bml_multiply_ellblock_typed.c:455-621): C <- A*B */
+void bml_multiply_AB_ellblock_hip(bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_t *B,
+                                  bml_matrix_ellblock_hip_t *C, double threshold);
+/* bml_add_ellblock (ellblock/bml_add_ellblock_typed.c:35-197): A <- alpha*A +
+ * beta*B blockwise, a block kept iff its max |element| > threshold. */
+void bml_add_ellblock_hip(bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_t *B,
+                          double alpha, double beta, double threshold);
+/* bml_threshold_ellblock (ellblock/bml_threshold_ellblock_typed.c:95-140):
+ * drop blocks whose max |element| <= threshold, zero the small elements of
+ * the others. */
+void bml_threshold_ellblock_hip(bml_matrix_ellblock_hip_t *A, double threshold);
+/* bml_trace_ellblock (ellblock/bml_trace_ellblock_typed.c:27-58) */
+double bml_trace_ellblock_hip(const bml_matrix_ellblock_hip_t *A);
+/* bml_gershgorin_ellblock (ellblock/bml_normalize_ellblock_typed.c:98-170):
+ * returns malloc'd double[2] = {min eigenvalue bound, max bound}. */
+void *bml_gershgorin_ellblock_hip(const bml_matrix_ellblock_hip_t *A);
+/* bml_normalize_ellblock (bml_normalize_ellblock_typed.c:33-46):
+ * A <- (maxeval*I - A)/(maxeval - mineval). */
+void bml_normalize_ellblock_hip(bml_matrix_ellblock_hip_t *A, double mineval, double maxeval);
+/* bml_scale_inplace_ellblock (ellblock/bml_scale_ellblock_typed.c:81-87):
+ * scale_factor points to one element of A's precision. */
+void bml_scale_inplace_ellblock_hip(const void *scale_factor, bml_matrix_ellblock_hip_t *A);
+/* bml_add_identity_ellblock (ellblock/bml_add_ellblock_typed.c:354-370):
+ * A <- A + beta*I through the blockwise add. */
+void bml_add_identity_ellblock_hip(bml_matrix_ellblock_hip_t *A, double beta, double threshold);
+/* bml_export_to_dense_ellblock (ellblock/bml_export_ellblock_typed.c):
+ * returns a malloc'd N*N host array. */
+void *bml_export_to_dense_ellblock_hip(const bml_matrix_ellblock_hip_t *A, int order);
+
 /* ---- SP2 purification ----------------------------------------------------
  * No bml counterpart: client codes (PROGRESS, LATTE) build SP2 from
  * bml_gershgorin, bml_normalize, bml_multiply_x2 and bml_add_norm; this runs
@@ -212,6 +332,11 @@ typedef struct bml_hip_sp2_options {
                           refine_iters iterations in double */
     int refine_iters;  /* double-precision iterations at the end (mixed) */
     int poll;          /* read the convergence flag every poll iterations */
+    int graph;         /* batched solves: replay each iteration's kernels as a
+                          captured hipGraph (1) or launch them one by one (0) */
+    int mixed_switch;  /* mixed: also move to double as soon as the single-
+                          precision iterate stops improving (1, needs poll 1),
+                          or only for the last refine_iters iterations (0) */
 } bml_hip_sp2_options;
 
 typedef struct bml_hip_sp2_result {
@@ -233,6 +358,22 @@ void bml_hip_sp2_defaults(bml_hip_sp2_options *opt);
 void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_hip_t *X,
                          double nocc, const bml_hip_sp2_options *opt,
                          bml_hip_sp2_result *res, double *history);
+/* nsys independent SP2 solves at once (H[s], X[s] of any N; every X[s] with
+ * the same M and precision): the systems are packed into one block-diagonal
+ * ELLPACK-R matrix, every iteration runs one launch per phase over all rows
+ * (rows of finished systems skip their work), and each system keeps its own
+ * branch decisions, traces and convergence test.  res receives nsys
+ * results.  No bml counterpart. */
+void bml_hip_sp2_ellpack_batch(int nsys, bml_matrix_ellpack_hip_t *const *H,
+                               bml_matrix_ellpack_hip_t *const *X, const double *nocc,
+                               const bml_hip_sp2_options *opt, bml_hip_sp2_result *res);
+/* The same on ELLBLOCK matrices (block products on the engine chosen by
+ * bml_hip_set_block_engine; in mixed precision the single-precision
+ * iterations use MFMA or, with BML_HIP_BLOCK_ENGINE_MFMA_BF16, bfloat16
+ * inputs).  H and X share their block sizes; X->MB bounds the fill. */
+void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellblock_hip_t *X,
+                          double nocc, const bml_hip_sp2_options *opt,
+                          bml_hip_sp2_result *res, double *history);
 
 /* ---- profiling window -----------------------------------------------------
  * No bml counterpart (bml has no timers).  Between begin and end every

@@ -1,0 +1,252 @@
+"""ELLBLOCK HIP backend against the reference (golden vectors) and the oracle.
+
+The exact block engine forms every element's dot product in the reference's
+order (bml_multiply_block1..6, ellblock/bml_multiply_ellblock_typed.c:28-168),
+so its results are compared bit for bit.  The MFMA engines sum in the matrix
+cores' order: FP64 within 1e-12 of max |X^2| (the parity bar for floating
+point), FP32 within 1e-5, BF16 inputs within 2^-7 of the summed magnitudes.
+"""
+import numpy as np
+import pytest
+
+import orc
+from bml_amd import generators as G
+from bml_amd.ell import Blk
+from helpers import DTYPES, assert_blk_identical, blk_of, load_cases, rel_close
+
+pytestmark = pytest.mark.gpu
+
+BLK = load_cases("ellblock.npz")
+BLK_CASES = sorted(BLK)
+
+
+def widen(X: Blk, MB: int) -> Blk:
+    out = Blk.zeros(X.bsize, MB, X.dtype, X.bmax)
+    out.value[:, : X.MB] = X.value
+    out.indexb[:, : X.MB] = X.indexb
+    out.nnzb[:] = X.nnzb
+    return out
+
+
+@pytest.fixture
+def exact(hip):
+    hip.set_block_engine(hip.BLOCK_ENGINE_EXACT)
+    yield hip
+    hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
+
+
+def ttol(dtype):
+    return 1e-12 if np.dtype(dtype) in (np.float64, np.complex128) else 2e-5
+
+
+@pytest.mark.parametrize("case", BLK_CASES)
+def test_golden_vectors(exact, case):
+    """x2, AB, multiply, add, threshold and trace reproduce the reference's
+    outputs (bml 2.4.0 C code, block sizes 1..6) bit for bit."""
+    hip = exact
+    d = BLK[case]
+    A, B, Cm = blk_of(d, "A"), blk_of(d, "B"), blk_of(d, "C")
+    thr = float(d["threshold"])
+    NB = A.NB
+    dA = hip.from_blk(widen(A, NB))
+    dB = hip.from_blk(widen(B, NB))
+    X2 = hip.block_matrix(A.precision, A.bsize, NB)
+    tr = hip.eb_multiply_x2(dA, X2, thr)
+    assert_blk_identical(X2.get(A.bmax), blk_of(d, "x2"), f"{case} x2")
+    _, otr = orc.eb_x2(widen(A, NB), NB, thr)
+    assert rel_close(tr[0], otr[0], ttol(A.dtype)) and rel_close(tr[1], otr[1], ttol(A.dtype))
+    Cab = hip.block_matrix(A.precision, A.bsize, NB)
+    hip.eb_multiply_AB(dA, dB, Cab, thr)
+    assert_blk_identical(Cab.get(A.bmax), blk_of(d, "ab"), f"{case} ab")
+    for tag in ("m1", "m3"):
+        if tag + "_args" not in d:
+            continue
+        alpha, beta, same = d[tag + "_args"]
+        dC = hip.from_blk(widen(Cm, NB))
+        hip.eb_multiply(dA, dA if same else dB, dC, float(alpha), float(beta), thr)
+        assert_blk_identical(dC.get(A.bmax), blk_of(d, tag), f"{case} {tag}")
+    if "add_value" in d:
+        dS = hip.from_blk(widen(A, NB))
+        hip.eb_add(dS, dB, 1.2, 0.8, thr)
+        assert_blk_identical(dS.get(A.bmax), blk_of(d, "add"), f"{case} add")
+    dT = hip.from_blk(widen(A, NB))
+    hip.eb_threshold(dT, 0.9)
+    assert_blk_identical(dT.get(A.bmax), blk_of(d, "thr"), f"{case} threshold")
+    assert rel_close(hip.eb_trace(dA), float(d["trace"][0]), ttol(A.dtype))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("bsize", [[3, 5, 7, 2, 8, 1, 4, 6, 8, 2],
+                                   [16, 9, 12, 16, 3, 16, 1, 11],
+                                   [32, 17, 30, 32, 5, 32, 24]])
+def test_exact_engine_matches_oracle(exact, dtype, bsize):
+    """All three tile edges (8, 16, 32) and uneven block sizes, bit for bit."""
+    hip = exact
+    NB = len(bsize)
+    A = G.random_blk(bsize, NB, 3, 40 + NB, dtype)
+    B = G.random_blk(bsize, NB, 3, 41 + NB, dtype)
+    thr = 0.5
+    ref, otr = orc.eb_x2(A, NB, thr)
+    dA, dB = hip.from_blk(A), hip.from_blk(B)
+    X2 = hip.block_matrix(A.precision, A.bsize, NB)
+    tr = hip.eb_multiply_x2(dA, X2, thr)
+    assert_blk_identical(X2.get(A.bmax), ref, "x2")
+    assert rel_close(tr[0], otr[0], 1e-12) and rel_close(tr[1], otr[1], ttol(dtype))
+    Cab = hip.block_matrix(A.precision, A.bsize, NB)
+    hip.eb_multiply_AB(dA, dB, Cab, thr)
+    assert_blk_identical(Cab.get(A.bmax), orc.eb_ab(A, B, NB, thr), "ab")
+    dS = hip.from_blk(A)
+    hip.eb_add(dS, dB, -0.7, 1.3, 0.3)
+    assert_blk_identical(dS.get(A.bmax), orc.eb_add(A, B, -0.7, 1.3, 0.3), "add")
+    dT = hip.from_blk(A)
+    hip.eb_threshold(dT, 0.6)
+    assert_blk_identical(dT.get(A.bmax), orc.eb_threshold(A, 0.6), "threshold")
+    assert rel_close(hip.eb_trace(dA), orc.eb_trace(A), 1e-12)
+    # dense export matches the host matrix
+    np.testing.assert_array_equal(hip.eb_export_to_dense(dA), A.to_dense())
+
+
+def blocks_close(got: Blk, ref: Blk, rtol: float, what: str):
+    np.testing.assert_array_equal(got.nnzb, ref.nnzb, err_msg=f"{what}: nnzb")
+    scale = max(np.abs(ref.value).max(), 1e-300)
+    for ib in range(ref.NB):
+        n = int(ref.nnzb[ib])
+        np.testing.assert_array_equal(got.indexb[ib, :n], ref.indexb[ib, :n],
+                                      err_msg=f"{what}: block row {ib}")
+        err = np.abs(got.value[ib, :n] - ref.value[ib, :n]).max(initial=0.0)
+        assert err <= rtol * scale, f"{what}: block row {ib}: {err / scale:.3e} of max|X^2|"
+
+
+@pytest.mark.parametrize("dtype,engine,rtol", [(np.float64, 1, 1e-12), (np.float32, 1, 1e-5),
+                                               (np.float32, 2, 2.0 ** -7)])
+def test_mfma_engines_match_oracle(hip, dtype, engine, rtol):
+    """32 x 32 blocks on the matrix cores, against the exact oracle."""
+    H = G.block_lattice_hamiltonian(4, 4, 2, b=32, MB=32, dtype=dtype)
+    NB = H.NB
+    ref, otr = orc.eb_x2(H, 32, 0.05)
+    hip.set_block_engine(engine)
+    try:
+        dH = hip.from_blk(H)
+        X2 = hip.block_matrix(H.precision, H.bsize, 32)
+        tr = hip.eb_multiply_x2(dH, X2, 0.05)
+        got = X2.get()
+    finally:
+        hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
+    if engine == 2:
+        # products of bf16-rounded inputs: relative to the summed magnitudes
+        absH = Blk(np.abs(H.value), H.indexb, H.nnzb, H.bsize)
+        bound, _ = orc.eb_x2(absH, 32, 0.0)
+        D = np.abs(got.to_dense() - ref.to_dense())
+        assert (D <= rtol * bound.to_dense() + 1e-30).all()
+        assert rel_close(tr[1], otr[1], 1e-2)
+    else:
+        blocks_close(got, ref, rtol, f"engine {engine}")
+        assert rel_close(tr[0], otr[0], 1e-12)
+        assert rel_close(tr[1], otr[1], 1e-12 if dtype == np.float64 else 1e-5)
+    assert X2.NB == NB
+
+
+def test_x2_matches_dense_product(hip):
+    """At threshold 0 with room for every block, blockwise X^2 is the dense
+    product (no bml code involved)."""
+    H = G.block_lattice_hamiltonian(3, 3, 2, b=32, MB=32)
+    D = H.to_dense()
+    dH = hip.from_blk(H)
+    X2 = hip.block_matrix(H.precision, H.bsize, 18)
+    tr = hip.eb_multiply_x2(dH, X2, 0.0)
+    ref = D @ D
+    assert np.abs(hip.eb_export_to_dense(X2) - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert rel_close(tr[0], np.trace(D), 1e-13) and rel_close(tr[1], np.trace(ref), 1e-12)
+
+
+def test_block_overflow_raises(hip):
+    """More touched block columns than X2's slots: the reference's fatal
+    'Number of non-zeroes per row > M'."""
+    A = G.random_blk([4] * 12, 12, 6, 5)
+    X2 = hip.block_matrix(A.precision, A.bsize, 2)
+    with pytest.raises(hip.BmlError) as e:
+        hip.eb_multiply_x2(hip.from_blk(A), X2, 0.0)
+    assert e.value.code == hip.ERR_M_OVERFLOW
+    ok = hip.block_matrix(A.precision, A.bsize, 12)
+    hip.eb_multiply_x2(hip.from_blk(A), ok, 0.0)
+
+
+def test_copy_and_registry(hip):
+    A = G.random_blk([5, 3, 8, 8], 4, 2, 9)
+    dA = hip.from_blk(A)
+    W = hip.block_matrix(A.precision, A.bsize, 6)
+    hip.eb_copy(dA, W)
+    assert_blk_identical(W.get(A.bmax), A, "copy to wider rows")
+    assert_blk_identical(hip.eb_copy_new(dA).get(A.bmax), A, "copy_new")
+    hip.eb_clear(W)
+    assert W.get(A.bmax).nnzb.sum() == 0
+    # block sizes are registered by the first allocation and set once only
+    assert hip._call("bml_hip_get_nb") > 0 and hip._call("bml_hip_get_mb") > 0
+    b = np.array([4, 4], np.int32)
+    with pytest.raises(hip.BmlError):
+        hip._call("bml_hip_set_block_sizes", b.ctypes.data, 2, 2)
+
+
+def test_gershgorin_and_normalize(hip):
+    """Gershgorin bounds and X0 = (emax I - H)/(emax - emin) of an ELLBLOCK
+    H (ellblock/bml_normalize_ellblock_typed.c:33-170) against the dense
+    formulas."""
+    H = G.block_lattice_hamiltonian(3, 2, 2, b=8, MB=8)
+    D = H.to_dense()
+    dH = hip.from_blk(H)
+    lo, hi = hip.eb_gershgorin(dH)
+    r = np.abs(D).sum(axis=1) - np.abs(np.diag(D))
+    assert abs(lo - (np.diag(D) - r).min()) <= 1e-13 and abs(hi - (np.diag(D) + r).max()) <= 1e-13
+    w = np.linalg.eigvalsh(D)
+    assert lo <= w.min() + 1e-12 and hi >= w.max() - 1e-12
+    hip.eb_normalize(dH, lo, hi)
+    np.testing.assert_allclose(hip.eb_export_to_dense(dH), (hi * np.eye(len(D)) - D) / (hi - lo),
+                               atol=1e-15)
+
+
+def test_sp2_is_the_spectral_projector(hip):
+    """SP2 on ELLBLOCK at threshold 0 -> the projector onto the lowest nocc
+    eigenvectors (numpy eigh), through both the exact and the MFMA engines."""
+    H = G.block_lattice_hamiltonian(2, 2, 2, b=32, MB=8, coupling=0.3)
+    D = H.to_dense()
+    N = len(D)
+    w, V = np.linalg.eigh(D)
+    P = V[:, : N // 2] @ V[:, : N // 2].T
+    for engine in (hip.BLOCK_ENGINE_EXACT, hip.BLOCK_ENGINE_MFMA):
+        hip.set_block_engine(engine)
+        try:
+            dH = hip.from_blk(H)
+            X = hip.block_matrix(H.precision, H.bsize, 8)
+            r, hist = hip.eb_sp2(dH, X, N / 2, threshold=0.0, tol=1e-9, maxit=80)
+        finally:
+            hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
+        assert r.converged and r.idempotency <= 1e-9, (engine, r.idempotency)
+        assert np.abs(hip.eb_export_to_dense(X) - P).max() < 1e-8
+        assert abs(hip.eb_trace(X) - N / 2) < 1e-8
+
+
+def test_sp2_mixed_precision(hip):
+    """Single-precision iterations (FP32 MFMA, or bfloat16 inputs) until
+    they stop improving, then double precision: converges (idempotent, exact
+    electron count).  SP2 steps are polynomials in X, so the double-precision
+    steps cannot rotate the occupied subspace: the density matrix keeps the
+    single-precision phase's error, about eps * ||H|| / gap (gapped system:
+    onsite shift 6, coupling 0.05)."""
+    H = G.block_lattice_hamiltonian(2, 2, 2, b=32, MB=8, onsite_shift=6.0, coupling=0.05)
+    N = H.N
+    dH = hip.from_blk(H)
+    Xd = hip.block_matrix(H.precision, H.bsize, 8)
+    rd, _ = hip.eb_sp2(dH, Xd, N / 2, threshold=1e-9, tol=1e-9, maxit=60)
+    assert rd.converged
+    ref = hip.eb_export_to_dense(Xd)
+    for engine, err in ((hip.BLOCK_ENGINE_MFMA, 1e-5), (hip.BLOCK_ENGINE_MFMA_BF16, 2e-3)):
+        hip.set_block_engine(engine)
+        try:
+            Xm = hip.block_matrix(H.precision, H.bsize, 8)
+            rm, hist = hip.eb_sp2(dH, Xm, N / 2, threshold=1e-9, tol=1e-9, maxit=60, mixed=True)
+        finally:
+            hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
+        assert rm.converged and rm.idempotency <= 1e-9, (engine, rm.idempotency)
+        assert 5 <= rm.single_iterations < rm.iterations
+        assert abs(rm.trace_x - N / 2) < 1e-7
+        assert np.abs(hip.eb_export_to_dense(Xm) - ref).max() < err

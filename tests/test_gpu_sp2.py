@@ -64,7 +64,7 @@ def test_sp2_mixed_precision_refines_to_double(hip, dtype):
     M = 256
     Xd, rd, hd = run_sp2(hip, H, M, N / 2, threshold=1e-5, tol=1e-8, maxit=30)
     Xm, rm, hm = run_sp2(hip, H, M, N / 2, threshold=1e-5, tol=1e-8, maxit=30, mixed=True,
-                         refine_iters=4)
+                         refine_iters=4, mixed_switch=False)
     assert rm.single_iterations == 26 and rm.iterations == 30
     # the double-precision refinement brings the trace back to nocc
     assert abs(rm.trace_x - N / 2) <= 1e-6 * N
@@ -72,3 +72,63 @@ def test_sp2_mixed_precision_refines_to_double(hip, dtype):
     assert rm.idempotency <= 10 * max(rd.idempotency, 1e-7)
     diff = np.abs(Xm.to_dense() - Xd.to_dense()).max()
     assert diff < 1e-4
+
+
+def _batch_systems(seed=11, M=512):
+    shapes = [(4, 4, 4), (6, 6, 4), (4, 4, 6), (8, 8, 8), (6, 6, 6), (4, 4, 2)]
+    return [G.lattice_hamiltonian(*L, M=M, hopping=-0.1, disorder=0.01, seed=seed + k)
+            for k, L in enumerate(shapes)]
+
+
+def test_sp2_mixed_switches_when_single_precision_stalls(hip):
+    """With mixed_switch the single-precision phase ends as soon as its
+    idempotency measure stops falling; double-precision iterations then
+    converge to the FP64 solve's density matrix."""
+    H = G.lattice_hamiltonian(8, 8, 8, M=16, hopping=-0.1, disorder=0.01)
+    N = H.N
+    Xd, rd, _ = run_sp2(hip, H, 512, N / 2, threshold=1e-9, tol=1e-9, maxit=60)
+    Xm, rm, hm = run_sp2(hip, H, 512, N / 2, threshold=1e-9, tol=1e-9, maxit=60, mixed=True)
+    assert rd.converged and rm.converged
+    assert rm.single_iterations < 56 and rm.iterations - rm.single_iterations >= 1
+    assert rm.idempotency <= 1e-9
+    # the double-precision steps cannot rotate the occupied subspace: the
+    # single-precision phase's error (~1e-7 * ||H|| / gap) remains
+    assert np.abs(Xm.to_dense() - Xd.to_dense()).max() < 1e-6
+
+
+def test_batch_matches_single_solves(hip):
+    """A batch of systems of uneven N gives each system the density matrix
+    of its own solve (FP64), graph replay or not."""
+    Hs = _batch_systems()
+    dHs = [hip.from_ell(H) for H in Hs]
+    singles = []
+    for dH, H in zip(dHs, Hs):
+        X = hip.zero_matrix(H.precision, H.N, 512)
+        r, _ = hip.sp2(dH, X, H.N / 2, threshold=1e-7, tol=1e-9, maxit=50)
+        singles.append((r, X.get().to_dense()))
+    outs = {}
+    for graph in (False, True):
+        Xs = [hip.zero_matrix(H.precision, H.N, 512) for H in Hs]
+        res = hip.sp2_batch(dHs, Xs, [H.N / 2 for H in Hs], threshold=1e-7, tol=1e-9, maxit=50,
+                            graph=graph)
+        outs[graph] = [X.get() for X in Xs]
+        for (r1, D1), rb, X in zip(singles, res, Xs):
+            assert rb.iterations == r1.iterations and rb.converged == r1.converged
+            assert abs(rb.trace_x - r1.trace_x) <= 1e-10
+            assert np.abs(X.get().to_dense() - D1).max() <= 1e-12
+    # graph replays run the same kernels as the eager launches: same bits
+    for a, b in zip(outs[False], outs[True]):
+        assert_ell_identical(a, b, "graph vs eager")
+
+
+def test_batch_mixed_precision(hip):
+    Hs = _batch_systems()
+    dHs = [hip.from_ell(H) for H in Hs]
+    Xd = [hip.zero_matrix(H.precision, H.N, 512) for H in Hs]
+    Xm = [hip.zero_matrix(H.precision, H.N, 512) for H in Hs]
+    nocc = [H.N / 2 for H in Hs]
+    rd = hip.sp2_batch(dHs, Xd, nocc, threshold=1e-6, tol=1e-9, maxit=40)
+    rm = hip.sp2_batch(dHs, Xm, nocc, threshold=1e-6, tol=1e-9, maxit=40, mixed=True)
+    for a, b, r in zip(Xd, Xm, rm):
+        assert np.abs(a.get().to_dense() - b.get().to_dense()).max() < 1e-4
+        assert r.idempotency < 1e-3
