@@ -56,7 +56,35 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
         g <<= 1;
     while (t > 64 && (64 / g) * (t + 8 + g) * slot > budget)
         t -= 64;
+    /* rows of B longer than three chunks: several wavefronts (at most 3
+     * chunks each) share the row's table, sized for the row alone up to
+     * what one block may hold.  Up to three chunks one wavefront is faster
+     * (the 32^3 lattice X^2: 6.3 ms against 16 ms with three wavefronts): a
+     * step's fixed work and the barrier outweigh the occupancy gain. */
+    const int nch = (max_b + 63) / 64;
+    int waves = 1;
+    if (g == 64 && nch >= 4) {
+        /* as few wavefronts as hold the chunks (3 each): on the 64^3 complex
+         * X^2 (5 chunks) 2 wavefronts ran in 203 ms, 3 in 211, 5 in 243,
+         * one wavefront with a 70 KB table in 323 */
+        waves = (nch + 2) / 3;
+        if (waves < 2)
+            waves = 2;
+        if (waves == 7)
+            waves = 8;
+        if (waves > 8)
+            waves = 8;
+        long long tm = est * 3 / 2;
+        if (tm < est + max_b)
+            tm = est + max_b;
+        tm = (tm + 64 + 63) & ~63ll;
+        const long long mw_budget = 120 * 1024;
+        while (tm > 64 && (tm + 72) * slot > mw_budget)
+            tm -= 64;
+        t = tm;
+    }
     const char *e;
+    out->waves = (e = getenv("BML_HIP_WAVES")) ? atoi(e) : waves;
     out->group = (e = getenv("BML_HIP_GROUP")) ? atoi(e) : g;
     out->table = (e = getenv("BML_HIP_TABLE")) ? atoi(e) : (int) t;
     out->bucket = (e = getenv("BML_HIP_BUCKET")) ? atoi(e) : 4;

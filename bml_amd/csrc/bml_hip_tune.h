@@ -9,6 +9,7 @@ typedef struct bmlh_spgemm_shape {
     int table;  /* LDS hash slots per row */
     int bucket; /* slots per probe */
     int remap;  /* XCD-aware block order */
+    int waves;  /* wavefronts per row (long rows) */
 } bmlh_spgemm_shape;
 
 /* Launch shape for a product whose left rows hold up to max_a entries and

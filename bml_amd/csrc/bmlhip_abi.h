@@ -93,6 +93,7 @@ typedef struct bmlhip_spgemm_args {
     int table;   /* LDS hash-table slots per row (rounded up to a multiple of 8) */
     int bucket;  /* slots per probe: 2, 4 or 8 */
     int remap;   /* 1: XCD-aware block-to-row order */
+    int waves;   /* group 64: wavefronts sharing one row's table (1 = one) */
     const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
     const bmlhip_batch *batch; /* not NULL: per-system traces into batch->d_trx/d_trx2 */
 } bmlhip_spgemm_args;

@@ -427,6 +427,7 @@ static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
     a.table = sh.table;
     a.bucket = sh.bucket;
     a.remap = sh.remap;
+    a.waves = sh.waves;
     return bmlh_ok(bmlhip_spgemm(&a), what);
 }
 

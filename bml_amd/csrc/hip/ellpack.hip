@@ -415,6 +415,253 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
 template <int GROUP>
 constexpr unsigned table_pad() { return 8u + (unsigned) GROUP; }
 
+// Several wavefronts per row (long rows of B).  The W wavefronts of a block
+// share one row's table; at each step wavefront w takes its own range of the
+// chunks of row j of B (q = ceil(chunks / W) consecutive chunks, so the
+// wavefronts' ranges follow kp order).  The columns of one row of B are
+// distinct, so within a step the wavefronts touch distinct slots; one
+// barrier per step orders the steps: every wavefront's probes and inserts
+// of step t are done before any wavefront ranks or adds (the new-column
+// counts of the step are exchanged through LDS, double-buffered), and the
+// adds of step t come before the adds of step t+1.  Sums are therefore
+// formed in the reference's order, as in the one-wavefront kernel.
+template <typename T, int W, int BK>
+__global__ __launch_bounds__(64 * W) void k_spgemm_mw(SpParams p, unsigned tsize) {
+    using E = Elem<T>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int cnt_s[2][W];
+    if (p.skip && *p.skip) return;
+    const unsigned stride = tsize + table_pad<64>();
+    T *vals = (T *) smem;
+    int *keys = (int *) (smem + (size_t) stride * sizeof(T));
+    uint16_t *ord = (uint16_t *) (smem + (size_t) stride * (sizeof(T) + sizeof(int)));
+    for (unsigned s = threadIdx.x; s < stride; s += 64 * W) {
+        keys[s] = EMPTY;
+        vals[s] = E::zero();
+    }
+    __syncthreads();
+    const int row = blockIdx.x;
+    if (row >= p.N || (p.row_sys && p.sys_done[p.row_sys[row]])) return;  // uniform
+    const int i = row;
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const T *Av = (const T *) p.Av;
+    const T *Bv = (const T *) p.Bv;
+    const int *Ai = p.Ai, *Bi = p.Bi, *Bn = p.Bn;
+    const int nA = p.An[i];
+    const unsigned rowA = (unsigned) i * (unsigned) p.MA;
+    const unsigned MB = (unsigned) p.MB;
+    const unsigned nbk = tsize / BK, dummy = tsize, priv = tsize + 8u + (unsigned) lane;
+    const int cap = (int) tsize - 8;
+
+    int count = 0;
+    bool tovf = false;
+    long long prods = 0;
+    T trx = E::zero();
+
+    // first kp of this wavefront's chunks for a row of nb entries
+    auto first_kp = [&](int nb) {
+        const int nch = (nb + 63) / 64;
+        const int q = (nch + W - 1) / W;
+        return w * q * 64;
+    };
+    auto last_kp = [&](int nb) {
+        const int nch = (nb + 63) / 64;
+        const int q = (nch + W - 1) / W;
+        const int e = (w + 1) * q * 64;
+        return e < nb ? e : nb;
+    };
+    auto load_b = [&](BRow<T> &r, int j, int nb, bool act) {
+        const unsigned base = (unsigned) j * MB;
+        const int k0 = first_kp(nb), k1 = last_kp(nb);
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const int kp = k0 + c * 64 + lane;
+            const unsigned at = (act & (kp < k1)) ? base + (unsigned) kp : 0u;
+            r.col[c] = Bi[at];
+            r.val[c] = Bv[at];
+        }
+    };
+
+    BRow<T> buf0, buf1;
+    for (int c0 = 0; c0 < nA; c0 += 64) {
+        const int jl = c0 + lane;
+        const unsigned at = jl < nA ? rowA + (unsigned) jl : 0u;
+        const T av = Av[at];
+        const int aj = jl < nA ? Ai[at] : 0;
+        const int an = Bn[aj];
+        const int steps = nA - c0 < 64 ? nA - c0 : 64;
+        T a = bcast<64>(av, 0);
+        int j = bcast<64>(aj, 0);
+        int nb = bcast<64>(an, 0);
+        load_b(buf0, j, nb, !tovf);
+        // step t: row j (in cur) times a, prefetch step t+1 into nxt; the two
+        // buffers are named, not indexed, so they stay in registers
+        auto step = [&](int t, BRow<T> &cur, BRow<T> &nxt) {
+            const int t1 = t + 1 < 64 ? t + 1 : t;
+            const T a1 = bcast<64>(av, t1);
+            const int j1 = bcast<64>(aj, t1);
+            const int nb1 = bcast<64>(an, t1);
+            bool gact = !tovf;
+            // the table must keep an empty slot, and a wavefront holds at
+            // most NCH chunks of a row: otherwise the row goes to the
+            // global-table pass
+            if (gact && (count + nb > cap || ((nb + 63) / 64 + W - 1) / W > NCH)) {
+                tovf = true;
+                gact = false;
+            }
+            load_b(nxt, j1, nb1, t + 1 < steps && gact);
+            if (gact) {
+                prods += nb;
+                if (j == i) trx = E::add(trx, a);
+            }
+            const int k0 = first_kp(nb), k1 = last_kp(nb);
+            // probe / insert this wavefront's chunks
+            unsigned h[NCH], slot[NCH];
+            bool act[NCH], isnew[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                act[c] = gact & (k0 + c * 64 + lane < k1);
+                h[c] = act[c] ? home_bucket<false>(cur.col[c], nbk) * BK : dummy;
+                slot[c] = priv;
+                isnew[c] = false;
+            }
+            while (true) {
+                Bucket<BK> kb[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; c++) kb[c] = load_bucket<BK>(keys, h[c]);
+                unsigned atc[NCH];
+                bool ins[NCH], anyins = false;
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+                    const int col = cur.col[c];
+                    const bool pend = h[c] != dummy;
+                    unsigned sel = BK;
+                    bool m = false;
+#pragma unroll
+                    for (int e = BK - 1; e >= 0; e--) {
+                        const bool me = kb[c].k[e] == col, ee = kb[c].k[e] == EMPTY;
+                        sel = (me | ee) ? (unsigned) e : sel;
+                        m = me ? true : (ee ? false : m);
+                    }
+                    const bool stop = sel < (unsigned) BK;
+                    atc[c] = h[c] + sel;
+                    const bool hit = pend & stop & m;
+                    ins[c] = pend & stop & !m;
+                    slot[c] = hit ? atc[c] : slot[c];
+                    unsigned hn = h[c] + BK;
+                    hn = hn >= tsize ? hn - tsize : hn;
+                    h[c] = (hit | !pend) ? dummy : (ins[c] ? h[c] : hn);
+                    anyins = anyins | ins[c];
+                }
+                if (__any(anyins)) {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++) {
+                        const int old = atomicCAS(&keys[ins[c] ? atc[c] : priv], EMPTY, cur.col[c]);
+                        const bool won = ins[c] & (old == EMPTY);
+                        isnew[c] = isnew[c] | won;
+                        slot[c] = won ? atc[c] : slot[c];
+                        h[c] = won ? dummy : h[c];
+                    }
+                }
+                bool anyp = false;
+#pragma unroll
+                for (int c = 0; c < NCH; c++) anyp = anyp | (h[c] != dummy);
+                if (!__any(anyp)) break;
+            }
+            int nw = 0;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) nw += popc64(ballot64(isnew[c]));
+            if (lane == 0) cnt_s[t & 1][w] = nw;
+            __syncthreads();
+            int base = count, tot = 0;
+#pragma unroll
+            for (int v = 0; v < W; v++) {
+                const int cv = cnt_s[t & 1][v];
+                base += v < w ? cv : 0;
+                tot += cv;
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                const uint64_t nm = ballot64(isnew[c]);
+                if (isnew[c]) ord[base + popc64(nm & ltmask)] = (uint16_t) slot[c];
+                base += popc64(nm);
+            }
+            count += tot;
+            T v[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++) v[c] = vals[slot[c]];
+#pragma unroll
+            for (int c = 0; c < NCH; c++) vals[slot[c]] = E::add(v[c], E::mul(a, cur.val[c]));
+            a = a1;
+            j = j1;
+            nb = nb1;
+        };
+        for (int t = 0; t < steps; t += 2) {
+            step(t, buf0, buf1);
+            if (t + 1 < steps) step(t + 1, buf1, buf0);
+        }
+    }
+    __syncthreads();  // every wavefront's last adds are in
+    if (w != 0) return;
+    // epilogue (wavefront 0): thresholded compaction in first-touch order
+    const int MC = p.MC;
+    const size_t rowC = (size_t) i * MC;
+    T *Cv = (T *) p.Cv;
+    int kept = 0;
+    T d2 = E::zero();
+    if (!tovf) {
+        for (int r0 = 0; r0 < count; r0 += 64) {
+            const int r = r0 + lane;
+            const bool act = r < count;
+            int s = 0, col = 0;
+            T v = E::zero();
+            if (act) {
+                s = (int) ord[r];
+                col = keys[s];
+                v = vals[s];
+            }
+            const bool isdiag = act && col == i;
+            const bool keep = isdiag || (act && E::above(v, p.thr));
+            const uint64_t km = ballot64(keep);
+            const int pos = kept + popc64(km & ltmask);
+            if (keep && pos < MC) {
+                p.Ci[rowC + pos] = col;
+                Cv[rowC + pos] = v;
+            }
+            const uint64_t dm = ballot64(isdiag);
+            if (dm) d2 = shfl_any(v, __ffsll((unsigned long long) dm) - 1);
+            kept += popc64(km);
+        }
+    } else if (lane == 0) {
+        const int q = atomicAdd(&p.st->table_overflow_rows, 1);
+        p.ovf[q] = i;
+        atomicAdd((unsigned long long *) &p.st->prof_ovf_rows[PrecOf<T>::value], 1ull);
+    }
+    if (!tovf && lane == 0) {
+        const int stored = kept < MC ? kept : MC;
+        p.Cn[i] = stored;
+        const bool over = p.kind == 1 ? count > MC : kept > MC;
+        if (over) {
+            atomicAdd(&p.st->m_overflow_rows, 1);
+            atomicMin(&p.st->first_m_overflow, i);
+        }
+        atomicMax(&p.st->max_nnz, stored);
+        atomicMax(&p.st->max_touched, count);
+        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        const int pi = PrecOf<T>::value;
+        atomicAdd((unsigned long long *) &p.st->prof_products[pi], (unsigned long long) prods);
+        atomicAdd((unsigned long long *) &p.st->prof_rows[pi], 1ull);
+        atomicAdd((unsigned long long *) &p.st->prof_in_nnz[pi], (unsigned long long) nA);
+        atomicAdd((unsigned long long *) &p.st->prof_out_nnz[pi], (unsigned long long) stored);
+        atomicAdd((unsigned long long *) &p.st->prof_touched[pi], (unsigned long long) count);
+        if (p.kind == 0) {
+            p.rowtr[i] = E::re(trx);
+            p.rowtr[p.N + i] = E::re(d2);
+        }
+    }
+}
+
 // First pass: table in LDS, 64/GROUP rows per 64-thread block.
 template <typename T, int GROUP, int BK>
 __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
@@ -966,6 +1213,23 @@ int launch_spgemm_lds(const SpParams &p, unsigned tsize, hipStream_t s) {
     return report("k_spgemm_lds");
 }
 
+template <typename T, int W>
+int launch_spgemm_mw(const SpParams &p, unsigned tsize, hipStream_t s) {
+    const size_t lds = (size_t) (tsize + table_pad<64>()) * (sizeof(T) + sizeof(int) + 2);
+    if (lds > 160 * 1024 - 64) {
+        return fail_msg(2, "LDS table of %zu bytes exceeds 160 KiB", lds);
+    }
+    if (lds > 64 * 1024)
+        (void) hipFuncSetAttribute((const void *) k_spgemm_mw<T, W, 4>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+    if (p.N > 0) {
+        bmlhip_prof_mark_begin(PrecOf<T>::value);
+        hipLaunchKernelGGL((k_spgemm_mw<T, W, 4>), dim3(p.N), dim3(64 * W), lds, s, p, tsize);
+        bmlhip_prof_mark_end();
+    }
+    return report("k_spgemm_mw");
+}
+
 template <typename T, int GROUP>
 int launch_spgemm_bk(const SpParams &p, unsigned tsize, int bucket, hipStream_t s) {
     switch (bucket) {
@@ -1031,7 +1295,17 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     int rc = bmlhip_status_reset();
     if (rc) return rc;
     const unsigned ts = (unsigned) tsize;
-    switch (a->group) {
+    if (a->group == 64 && a->waves > 1) {
+        switch (a->waves) {
+        case 2: rc = launch_spgemm_mw<T, 2>(p, ts, s); break;
+        case 3: rc = launch_spgemm_mw<T, 3>(p, ts, s); break;
+        case 4: rc = launch_spgemm_mw<T, 4>(p, ts, s); break;
+        case 5: rc = launch_spgemm_mw<T, 5>(p, ts, s); break;
+        case 6: rc = launch_spgemm_mw<T, 6>(p, ts, s); break;
+        case 8: rc = launch_spgemm_mw<T, 8>(p, ts, s); break;
+        default: return fail_msg(2, "waves %d not one of 1..6 or 8", a->waves);
+        }
+    } else switch (a->group) {
     case 8: rc = launch_spgemm_bk<T, 8>(p, ts, a->bucket, s); break;
     case 16: rc = launch_spgemm_bk<T, 16>(p, ts, a->bucket, s); break;
     case 32: rc = launch_spgemm_bk<T, 32>(p, ts, a->bucket, s); break;

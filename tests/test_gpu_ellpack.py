@@ -101,6 +101,7 @@ def test_x2_launch_shapes_agree(hip, monkeypatch, group, table, bucket, remap):
     monkeypatch.setenv("BML_HIP_TABLE", str(table))
     monkeypatch.setenv("BML_HIP_BUCKET", str(bucket))
     monkeypatch.setenv("BML_HIP_XCD_REMAP", str(remap))
+    monkeypatch.setenv("BML_HIP_WAVES", "1")
     X2 = hip.zero_matrix(A.precision, A.N, 512)
     hip.multiply_x2(dev(hip, A), X2, 1e-5)
     assert_ell_identical(X2.get(), ref, f"group {group} table {table} bucket {bucket}")
@@ -207,3 +208,21 @@ def test_empty_rows_and_zero_threshold(hip):
     tr = hip.multiply_x2(dev(hip, A), X2, 0.0)
     assert_ell_identical(X2.get(), ref, "empty rows")
     assert tr[0] == rtr[0] and rel_close(tr[1], rtr[1], 1e-14)
+
+
+@pytest.mark.parametrize("dtype,waves", [(np.float64, None), (np.float64, 1), (np.float64, 2),
+                                         (np.float64, 3), (np.float64, 4), (np.float64, 8),
+                                         (np.float32, None), (np.complex64, None),
+                                         (np.complex128, None), (np.complex128, 6)])
+def test_x2_long_rows_several_wavefronts(hip, monkeypatch, dtype, waves):
+    """Rows of ~300 entries (5 chunks of 64): several wavefronts share one
+    row's table, one barrier per step; the result keeps the reference's
+    bits and first-touch order."""
+    A = G.random_ell(1500, 320, 300, 77, dtype)
+    ref, rtr = orc.x2(A, 1500, 0.3)
+    if waves is not None:
+        monkeypatch.setenv("BML_HIP_WAVES", str(waves))
+    X2 = hip.zero_matrix(A.precision, A.N, 1500)
+    tr = hip.multiply_x2(dev(hip, A), X2, 0.3)
+    assert_ell_identical(X2.get(), ref, f"waves {waves}")
+    assert rel_close(tr[0], rtr[0], 1e-12) and rel_close(tr[1], rtr[1], 1e-10)
