@@ -258,6 +258,89 @@ def batched_sp2(hip, G, plain_sample=32):
     return out
 
 
+def x2_config1(hip, G, cpu=True):
+    """Config 1: one bml_multiply_x2 (threshold 1e-5) and one bml_multiply
+    (alpha 1, beta 0, A != B: the AB + add path, C with M = 1024) of the
+    random banded N = 4096 matrix (~32 nnz/row within |i-j| <= 256), X^2
+    into M2 = 256;
+    with the reference's CPU x2 on the host beside it."""
+    A = G.random_banded()
+    dA = hip.from_ell(A)
+    dB = hip.copy_new(dA)
+    X2 = hip.zero_matrix(A.precision, A.N, 256)
+    # the AB path checks the pre-threshold row against C's M, as the
+    # reference does: C gets M = 1024 as in the reference probe
+    Cm = hip.zero_matrix(A.precision, A.N, 1024)
+    hip.multiply_x2(dA, X2, THR)
+    hip.multiply(dA, dB, Cm, 1.0, 0.0, THR)
+    reps = 20
+    hip.synchronize()
+    hip.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hip.multiply_x2(dA, X2, THR)
+    hip.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    prof = hip.profile_end()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hip.multiply(dA, dB, Cm, 1.0, 0.0, THR)
+    hip.synchronize()
+    dtm = (time.perf_counter() - t0) / reps
+    n = prof.launches[2]
+    out = {"workload": "random banded N=4096, ~32 nnz/row, |i-j|<=256, X^2 threshold 1e-5 into M2=256",
+           "x2_ms": round(dt * 1e3, 4), "x2_kernel_ms": round(prof.kernel_ms[2] / n, 4),
+           "x2_gflops": round(2.0 * prof.products[2] / n / dt / 1e9, 2),
+           "multiply_AB_plus_add_ms": round(dtm * 1e3, 4),
+           "bytes_alg_gbs": round(bytes_alg(prof, 2, 12) / n / (prof.kernel_ms[2] / n * 1e-3) / 1e9, 1)}
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if cpu and os.path.exists(drv):
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores", OMP_STACKSIZE="64M")
+        with tempfile.TemporaryDirectory() as tmp:
+            f = os.path.join(tmp, "a.ell")
+            A.write(f)
+            r = subprocess.run([drv, "bench_x2", f, "256", THR.hex(), "20"], capture_output=True,
+                               text=True, env=env, timeout=300)
+        if r.returncode == 0:
+            ms = float(r.stdout.split()[1])
+            out["cpu_reference_x2_ms"] = ms
+            out["cpu_reference_threads"] = threads
+            out["gpu_over_cpu_x2"] = round(ms / (dt * 1e3), 1)
+    return out
+
+
+def x2_config5(hip, G):
+    """Config 5: thresholded X^2 of the double complex Hermitian 64^3
+    lattice (Peierls phases, N = 262144), X = the third SP2 iterate
+    (~280 nnz/row), M = 512, threshold 1e-6."""
+    H = G.peierls_hamiltonian(64, M=16)
+    dH = hip.from_ell(H)
+    X = hip.zero_matrix(H.precision, H.N, 512)
+    hip.sp2(dH, X, H.N / 2, threshold=1e-6, tol=1e-8, maxit=3)
+    X2 = hip.zero_matrix(H.precision, H.N, 512)
+    hip.multiply_x2(X, X2, 1e-6)
+    hip.synchronize()
+    reps = 3
+    hip.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hip.multiply_x2(X, X2, 1e-6)
+    hip.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    prof = hip.profile_end()
+    roof = roofline(prof, 4, 20, "k_spgemm_mw<cf64> (double complex X^2)")
+    mx, tot = X.row_stats()
+    m2, tot2 = X2.row_stats()
+    return {"workload": "double complex Hermitian 64^3 lattice (Peierls phases), N=262144, "
+                        "X = SP2 iterate 3, M=512, threshold 1e-6",
+            "x_nnz_per_row": round(tot / H.N, 1), "x2_nnz_per_row": round(tot2 / H.N, 1),
+            "x2_max_nnz": m2, "x2_ms": round(dt * 1e3, 2),
+            "x2_gflops": round(8.0 * prof.products[4] / reps / dt / 1e9, 1),
+            "roofline": roof}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,6 +415,8 @@ def main():
         "roofline": roof,
         "roofline_fp64_x2": roof64,
         "copy_calibration": calib,
+        "x2_config1": None if args.only_headline else x2_config1(hip, G, cpu=not args.no_cpu),
+        "x2_config5": None if args.only_headline else x2_config5(hip, G),
         "ellblock_sp2": None if args.only_headline else ellblock_sp2(hip, G),
         "batched_sp2": None if args.only_headline else batched_sp2(hip, G),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(H, N),

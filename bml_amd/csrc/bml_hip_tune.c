@@ -8,8 +8,8 @@
  *
  * The LDS table holds a row's distinct columns.  est, the largest distinct
  * count per row, comes from the previous product of the same matrix when
- * known (scaled by the squared growth of its rows since: a product row can
- * grow as fast as (row length)^2), else from the bound min(N, max_a*max_b).
+ * known (scaled by the growth of its rows since), else from the bound
+ * min(N, max_a*max_b).
  * The table gets max(3/2 est, est + max_b) + 64 slots in 4-slot buckets: a
  * row stops (and goes to the global-table pass) once count + (row length of
  * B) could pass the table, and at ~2/3 load the probe rounds per step (the
@@ -39,15 +39,18 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
         bound = N;
     long long est = bound;
     if (touched_hint > 0) {
+        /* the rows grew since the hint was taken: a row's distinct count
+         * grows about as its length */
         double grow = hint_max_a > 0 && max_a > hint_max_a ? (double) max_a / hint_max_a : 1.0;
-        est = (long long) (touched_hint * grow * grow) + 1;
+        est = (long long) (touched_hint * grow) + 1;
         if (est > bound)
             est = bound;
     }
-    long long t = est * 3 / 2;
-    if (t < est + max_b)
-        t = est + max_b;
-    t = (t + 64 + 63) & ~63ll;
+    long long need = est * 3 / 2;
+    if (need < est + max_b)
+        need = est + max_b;
+    need = (need + 64 + 63) & ~63ll;
+    long long t = need;
     const long long slot = (long long) bmlh_elem_size(precision) + 4 + 2;
     const char *eb = getenv("BML_HIP_LDS_BUDGET");
     const long long budget = eb ? atoll(eb) : 32 * 1024;
@@ -56,17 +59,18 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
         g <<= 1;
     while (t > 64 && (64 / g) * (t + 8 + g) * slot > budget)
         t -= 64;
-    /* rows of B longer than three chunks: several wavefronts (at most 3
-     * chunks each) share the row's table, sized for the row alone up to
-     * what one block may hold.  Up to three chunks one wavefront is faster
-     * (the 32^3 lattice X^2: 6.3 ms against 16 ms with three wavefronts): a
-     * step's fixed work and the barrier outweigh the occupancy gain. */
+    /* A row whose table does not fit one wavefront's budget: several
+     * wavefronts (at most 3 chunks of the rows of B each) share one table
+     * of up to 120 KB, so the row still fits in LDS.  When the table fits,
+     * one wavefront is faster: on the 32^3 lattice X^2, 6.3 ms against
+     * 16 ms with three wavefronts (a step's fixed work and the barrier
+     * outweigh the occupancy gain).  On the 64^3 complex X^2 (73 KB tables,
+     * 5 chunks) two wavefronts ran in 203 ms, 3 in 211, 5 in 243, one
+     * wavefront with the whole table in 323, and the capped table (rows in
+     * the global-table pass) in 3200. */
     const int nch = (max_b + 63) / 64;
     int waves = 1;
-    if (g == 64 && nch >= 4) {
-        /* as few wavefronts as hold the chunks (3 each): on the 64^3 complex
-         * X^2 (5 chunks) 2 wavefronts ran in 203 ms, 3 in 211, 5 in 243,
-         * one wavefront with a 70 KB table in 323 */
+    if (g == 64 && nch >= 2 && (need + 72) * slot > budget) {
         waves = (nch + 2) / 3;
         if (waves < 2)
             waves = 2;
@@ -74,10 +78,7 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
             waves = 8;
         if (waves > 8)
             waves = 8;
-        long long tm = est * 3 / 2;
-        if (tm < est + max_b)
-            tm = est + max_b;
-        tm = (tm + 64 + 63) & ~63ll;
+        long long tm = need;
         const long long mw_budget = 120 * 1024;
         while (tm > 64 && (tm + 72) * slot > mw_budget)
             tm -= 64;
