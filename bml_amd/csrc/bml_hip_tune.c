@@ -50,15 +50,32 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
     if (need < est + max_b)
         need = est + max_b;
     need = (need + 64 + 63) & ~63ll;
-    long long t = need;
     const long long slot = (long long) bmlh_elem_size(precision) + 4 + 2;
     const char *eb = getenv("BML_HIP_LDS_BUDGET");
     const long long budget = eb ? atoll(eb) : 32 * 1024;
     /* more rows per wavefront only while their tables fit */
-    while (g < 64 && (64 / g) * (t + 8 + g) * slot > budget)
+    while (g < 64 && (64 / g) * (need + 8 + g) * slot > budget)
         g <<= 1;
-    while (t > 64 && (64 / g) * (t + 8 + g) * slot > budget)
-        t -= 64;
+    /* The kernel is latency bound, so its speed follows the wavefronts
+     * resident per CU, which the LDS sets (160 KiB per CU, allocated in
+     * 1 KiB granules here).  Take the largest table that still keeps the
+     * most wavefronts with at least `need` slots: on the 32^3 lattice X^2
+     * (1177 distinct columns per row, need 1856) 1792 slots (6 wavefronts
+     * per CU) ran 4.8 ms, 1856 slots (5) 5.4 ms. */
+    const int R = 64 / g;
+    const long long pad = 8 + g;
+    long long t = 0;
+    for (int k = 8; k >= 5 && !t; k--) {
+        const long long per = (160 * 1024 / k) / 1024 * 1024;
+        const long long tk = (per / (R * slot) - pad) / 64 * 64;
+        if (tk >= need - 64)
+            t = tk;
+    }
+    if (!t) {
+        t = need;
+        while (t > 64 && R * (t + pad) * slot > budget)
+            t -= 64;
+    }
     /* A row whose table does not fit one wavefront's budget: several
      * wavefronts (at most 3 chunks of the rows of B each) share one table
      * of up to 120 KB, so the row still fits in LDS.  When the table fits,

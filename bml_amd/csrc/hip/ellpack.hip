@@ -155,6 +155,24 @@ __device__ __forceinline__ Bucket<BK> load_bucket(const int *keys, unsigned at) 
     return b;
 }
 
+// Where col sits in a bucket (BK if absent), and how many of its slots are
+// filled.  Buckets fill front to back and nothing is deleted while a row is
+// built, so a present column is among the filled slots and the first empty
+// slot is at index `filled` (BK: the bucket is full).  EMPTY (-1) is the
+// only key with the sign bit set.
+template <int BK>
+__device__ __forceinline__ void bucket_scan(const Bucket<BK> &b, int col, unsigned &hit_at,
+                                            unsigned &filled) {
+    unsigned e_hit = BK, empties = 0;
+#pragma unroll
+    for (int e = BK - 1; e >= 0; e--) {
+        e_hit = b.k[e] == col ? (unsigned) e : e_hit;
+        empties += (unsigned) b.k[e] >> 31;
+    }
+    hit_at = e_hit;
+    filled = BK - empties;
+}
+
 // One row per group.  keys/vals/ord point at the group's table (LDS or
 // global): tsize slots (nbk = tsize/BK buckets), then a dummy bucket at slot
 // tsize that finished lanes read, then (from slot tsize + 8) one private
@@ -230,21 +248,13 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
             bool ins[NCH], anyins = false;
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
-                const int col = r.col[c];
                 const bool pend = h[c] != dummy;
-                // first slot of the bucket holding col or EMPTY (BK: none)
-                unsigned sel = BK;
-                bool m = false;
-#pragma unroll
-                for (int e = BK - 1; e >= 0; e--) {
-                    const bool me = kb[c].k[e] == col, ee = kb[c].k[e] == EMPTY;
-                    sel = (me | ee) ? (unsigned) e : sel;
-                    m = me ? true : (ee ? false : m);
-                }
-                const bool stop = sel < (unsigned) BK;
-                at[c] = h[c] + sel;
-                const bool hit = pend & stop & m;
-                ins[c] = pend & stop & !m;
+                unsigned eh, nf;
+                bucket_scan<BK>(kb[c], r.col[c], eh, nf);
+                const bool found = eh < (unsigned) BK;
+                at[c] = h[c] + (found ? eh : nf);
+                const bool hit = pend & found;
+                ins[c] = pend & !found & (nf < (unsigned) BK);
                 slot[c] = hit ? at[c] : slot[c];
                 unsigned hn = h[c] + BK;
                 hn = hn >= tsize ? hn - tsize : hn;
@@ -534,20 +544,13 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_mw(SpParams p, unsigned tsize
                 bool ins[NCH], anyins = false;
 #pragma unroll
                 for (int c = 0; c < NCH; c++) {
-                    const int col = cur.col[c];
                     const bool pend = h[c] != dummy;
-                    unsigned sel = BK;
-                    bool m = false;
-#pragma unroll
-                    for (int e = BK - 1; e >= 0; e--) {
-                        const bool me = kb[c].k[e] == col, ee = kb[c].k[e] == EMPTY;
-                        sel = (me | ee) ? (unsigned) e : sel;
-                        m = me ? true : (ee ? false : m);
-                    }
-                    const bool stop = sel < (unsigned) BK;
-                    atc[c] = h[c] + sel;
-                    const bool hit = pend & stop & m;
-                    ins[c] = pend & stop & !m;
+                    unsigned eh, nf;
+                    bucket_scan<BK>(kb[c], cur.col[c], eh, nf);
+                    const bool found = eh < (unsigned) BK;
+                    atc[c] = h[c] + (found ? eh : nf);
+                    const bool hit = pend & found;
+                    ins[c] = pend & !found & (nf < (unsigned) BK);
                     slot[c] = hit ? atc[c] : slot[c];
                     unsigned hn = h[c] + BK;
                     hn = hn >= tsize ? hn - tsize : hn;
