@@ -78,6 +78,8 @@ struct SpParams {
     bmlhip_status *st;
     int *ovf;         // rows handed to the global-table pass
     const int *skip;  // when set and nonzero, the launch does nothing
+    int4 *rowstat;    // [N]: (kept, distinct columns, products, nnz of A's row),
+                      // kept = -1 for a skipped row; reduced by k_row_stats
     int remap;        // XCD-aware block-to-row order
     // batched systems: row i belongs to system row_sys[i]; rows of systems
     // with sys_done set are skipped (both NULL: one system)
@@ -395,23 +397,10 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
         }
     }
     if (rowact && gl == 0) {
-        const int stored = kept < MC ? kept : MC;
-        p.Cn[i] = stored;
-        // kind 1 (AB) checks the touched count, as the reference does
-        const bool over = p.kind == 1 ? count > MC : kept > MC;
-        if (over) {
-            atomicAdd(&p.st->m_overflow_rows, 1);
-            atomicMin(&p.st->first_m_overflow, i);
-        }
-        atomicMax(&p.st->max_nnz, stored);
-        atomicMax(&p.st->max_touched, count);
-        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
-        const int pi = PrecOf<T>::value;
-        atomicAdd((unsigned long long *) &p.st->prof_products[pi], (unsigned long long) prods);
-        atomicAdd((unsigned long long *) &p.st->prof_rows[pi], 1ull);
-        atomicAdd((unsigned long long *) &p.st->prof_in_nnz[pi], (unsigned long long) nA);
-        atomicAdd((unsigned long long *) &p.st->prof_out_nnz[pi], (unsigned long long) stored);
-        atomicAdd((unsigned long long *) &p.st->prof_touched[pi], (unsigned long long) count);
+        p.Cn[i] = kept < MC ? kept : MC;
+        // plain stores: same-address atomics from every row serialise (eight
+        // per row cost 1.3 ms over 32768 rows)
+        p.rowstat[i] = make_int4(kept, count, (int) prods, nA);
         if (p.kind == 0) {
             p.rowtr[i] = E::re(trx);
             p.rowtr[p.N + i] = E::re(d2);
@@ -451,7 +440,11 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_mw(SpParams p, unsigned tsize
     }
     __syncthreads();
     const int row = blockIdx.x;
-    if (row >= p.N || (p.row_sys && p.sys_done[p.row_sys[row]])) return;  // uniform
+    if (row >= p.N) return;
+    if (p.row_sys && p.sys_done[p.row_sys[row]]) {  // uniform
+        if (threadIdx.x == 0) p.rowstat[row] = make_int4(-1, 0, 0, 0);
+        return;
+    }
     const int i = row;
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const uint64_t ltmask = (1ull << lane) - 1ull;
@@ -642,22 +635,8 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_mw(SpParams p, unsigned tsize
         atomicAdd((unsigned long long *) &p.st->prof_ovf_rows[PrecOf<T>::value], 1ull);
     }
     if (!tovf && lane == 0) {
-        const int stored = kept < MC ? kept : MC;
-        p.Cn[i] = stored;
-        const bool over = p.kind == 1 ? count > MC : kept > MC;
-        if (over) {
-            atomicAdd(&p.st->m_overflow_rows, 1);
-            atomicMin(&p.st->first_m_overflow, i);
-        }
-        atomicMax(&p.st->max_nnz, stored);
-        atomicMax(&p.st->max_touched, count);
-        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
-        const int pi = PrecOf<T>::value;
-        atomicAdd((unsigned long long *) &p.st->prof_products[pi], (unsigned long long) prods);
-        atomicAdd((unsigned long long *) &p.st->prof_rows[pi], 1ull);
-        atomicAdd((unsigned long long *) &p.st->prof_in_nnz[pi], (unsigned long long) nA);
-        atomicAdd((unsigned long long *) &p.st->prof_out_nnz[pi], (unsigned long long) stored);
-        atomicAdd((unsigned long long *) &p.st->prof_touched[pi], (unsigned long long) count);
+        p.Cn[i] = kept < MC ? kept : MC;
+        p.rowstat[i] = make_int4(kept, count, (int) prods, nA);
         if (p.kind == 0) {
             p.rowtr[i] = E::re(trx);
             p.rowtr[p.N + i] = E::re(d2);
@@ -695,6 +674,7 @@ __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
     const int g = threadIdx.x / GROUP;
     const int row = vb * R + g;
     const int i = (row < p.N && !(p.row_sys && p.sys_done[p.row_sys[row]])) ? row : -1;
+    if (row < p.N && i < 0 && threadIdx.x % GROUP == 0) p.rowstat[row] = make_int4(-1, 0, 0, 0);
     const int cap = (int) tsize - 8;
     spgemm_row<T, GROUP, BK, false, uint16_t>(p, i, keys + g * stride, vals + (size_t) g * stride,
                                               ord + g * stride, tsize, cap);
@@ -727,6 +707,88 @@ __global__ __launch_bounds__(64) void k_spgemm_global(SpParams p, unsigned tsize
         spgemm_row<T, 64, 2, true, int>(p, p.ovf[q], keys, vals, ord, tsize, (int) tsize - 2);
         __syncthreads();
     }
+}
+
+// Reduces the per-row records (kept, distinct, products, nnz of A's row) of
+// a row product or an add into the status block: rows over MC and the
+// first of them (kind 1, AB: the distinct count is checked, as the reference
+// does), largest and total stored row, largest distinct count, and with
+// prof > 0 the profiling sums of that precision.  One set of atomics per
+// block.
+__global__ __launch_bounds__(256) void k_reduce_rowstat(const int4 *rs, int N, int MC, int kind,
+                                                        int prof, const int *skip,
+                                                        bmlhip_status *st) {
+    if (skip && *skip) return;
+    int ovf = 0, first = INT_MAX, mx = 0, mt = 0;
+    unsigned long long tot = 0, prods = 0, rows = 0, innz = 0, dist = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        const int4 r = rs[i];
+        if (r.x < 0) continue;
+        const int stored = r.x < MC ? r.x : MC;
+        if (kind == 1 ? r.y > MC : r.x > MC) {
+            ovf++;
+            first = i < first ? i : first;
+        }
+        mx = stored > mx ? stored : mx;
+        mt = r.y > mt ? r.y : mt;
+        tot += (unsigned long long) stored;
+        prods += (unsigned long long) (unsigned) r.z;
+        rows++;
+        innz += (unsigned long long) r.w;
+        dist += (unsigned long long) r.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ovf += __shfl_down(ovf, off, 64);
+        const int f = __shfl_down(first, off, 64);
+        first = f < first ? f : first;
+        const int a = __shfl_down(mx, off, 64);
+        mx = a > mx ? a : mx;
+        const int b = __shfl_down(mt, off, 64);
+        mt = b > mt ? b : mt;
+        tot += __shfl_down(tot, off, 64);
+        prods += __shfl_down(prods, off, 64);
+        rows += __shfl_down(rows, off, 64);
+        innz += __shfl_down(innz, off, 64);
+        dist += __shfl_down(dist, off, 64);
+    }
+    __shared__ int si[4][4];
+    __shared__ unsigned long long sl[4][5];
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        si[w][0] = ovf; si[w][1] = first; si[w][2] = mx; si[w][3] = mt;
+        sl[w][0] = tot; sl[w][1] = prods; sl[w][2] = rows; sl[w][3] = innz; sl[w][4] = dist;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) {
+            ovf += si[k][0];
+            first = si[k][1] < first ? si[k][1] : first;
+            mx = si[k][2] > mx ? si[k][2] : mx;
+            mt = si[k][3] > mt ? si[k][3] : mt;
+            tot += sl[k][0]; prods += sl[k][1]; rows += sl[k][2]; innz += sl[k][3]; dist += sl[k][4];
+        }
+        if (ovf) {
+            atomicAdd(&st->m_overflow_rows, ovf);
+            atomicMin(&st->first_m_overflow, first);
+        }
+        if (rows) {
+            atomicMax(&st->max_nnz, mx);
+            atomicAdd((unsigned long long *) &st->total_nnz, tot);
+        }
+        if (prof > 0 && rows) {
+            atomicMax(&st->max_touched, mt);
+            atomicAdd((unsigned long long *) &st->prof_products[prof], prods);
+            atomicAdd((unsigned long long *) &st->prof_rows[prof], rows);
+            atomicAdd((unsigned long long *) &st->prof_in_nnz[prof], innz);
+            atomicAdd((unsigned long long *) &st->prof_out_nnz[prof], tot);
+            atomicAdd((unsigned long long *) &st->prof_touched[prof], dist);
+        }
+    }
+}
+
+inline int rowstat_blocks(int N) {
+    const int b = (N + 2047) / 2048;
+    return b < 1 ? 1 : (b > 240 ? 240 : b);
 }
 
 // Deterministic sum of n doubles (fixed partition, fixed tree).
@@ -837,6 +899,7 @@ struct AddParams {
     // systems with sys_done set are skipped
     const int *row_sys, *sys_done;
     const double *ab_sys;
+    int4 *rowstat;  // [N]: (kept, 0, 0, 0), kept = -1 for a skipped row
 };
 
 template <typename T>
@@ -921,14 +984,8 @@ __device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int 
     // fixed-order wavefront reduction of the row's norm
     for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, 64);
     if (lane == 0) {
-        const int stored = kept < p.MA ? kept : p.MA;
-        p.An[i] = stored;
-        if (kept > p.MA) {
-            atomicAdd(&p.st->m_overflow_rows, 1);
-            atomicMin(&p.st->first_m_overflow, i);
-        }
-        atomicMax(&p.st->max_nnz, stored);
-        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        p.An[i] = kept < p.MA ? kept : p.MA;
+        p.rowstat[i] = make_int4(kept, 0, 0, 0);
         if (p.mode) p.rownorm[i] = nrm;
     }
 }
@@ -949,7 +1006,10 @@ __global__ __launch_bounds__(64) void k_add_lds(AddParams p, int tbits) {
     }
     __syncthreads();
     const int i = blockIdx.x;
-    if (p.row_sys && p.sys_done[p.row_sys[i]]) return;
+    if (p.row_sys && p.sys_done[p.row_sys[i]]) {
+        if (threadIdx.x == 0) p.rowstat[i] = make_int4(-1, 0, 0, 0);
+        return;
+    }
     add_row<T>(p, i, keys, xs, ys, ord, tbits);
 }
 
@@ -969,7 +1029,10 @@ __global__ __launch_bounds__(64) void k_add_global(AddParams p, int tbits, unsig
     }
     __syncthreads();
     for (int i = blockIdx.x; i < p.N; i += gridDim.x) {
-        if (p.row_sys && p.sys_done[p.row_sys[i]]) continue;
+        if (p.row_sys && p.sys_done[p.row_sys[i]]) {
+            if (threadIdx.x == 0) p.rowstat[i] = make_int4(-1, 0, 0, 0);
+            continue;
+        }
         add_row<T>(p, i, keys, xs, ys, ord, tbits);
         __syncthreads();
     }
@@ -1252,6 +1315,9 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
         (size_t) a->N * (size_t) a->MC >= (1ull << 32))
         return fail_msg(5, "row product: N=%d with M up to %d exceeds the 2^24 columns / 2^32 "
                            "entries the kernels address", a->N, a->MA > a->MB ? a->MA : a->MB);
+    // a row's product count is kept in 32 bits
+    if ((size_t) a->MA * (size_t) a->MB >= (1ull << 32))
+        return fail_msg(5, "row product: MA*MB = %d*%d exceeds 2^32", a->MA, a->MB);
     SpParams p;
     p.N = a->N;
     p.MA = a->MA;
@@ -1275,6 +1341,7 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     p.rowtr = a->batch ? a->batch->d_rowtr
                        : (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
     p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
+    p.rowstat = (int4 *) bmlhip_scratch(7, sizeof(int4) * (size_t) (a->N + 1));
     p.remap = a->remap;
     // global-table pass: one table per block, >= 2x the row bound
     long long bound = (long long) a->MA * a->MB;
@@ -1288,7 +1355,7 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     if (gblocks < 1) gblocks = 1;
     unsigned char *gtab = (unsigned char *) bmlhip_scratch(2, (size_t) gblocks * per_block + 256);
     int *cursor = &p.st->cursor;
-    if (!p.rowtr || !p.ovf || !gtab) {
+    if (!p.rowtr || !p.ovf || !p.rowstat || !gtab) {
         return fail_msg(4, "scratch allocation failed");
     }
     // LDS table: a multiple of 8 slots, at least 64, below 2^16 buckets
@@ -1320,6 +1387,10 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     hipLaunchKernelGGL(k_spgemm_global<T>, dim3(gblocks), dim3(64), 0, s, p, (unsigned) gsize, gtab,
                        cursor);
     if ((rc = report("k_spgemm_global"))) return rc;
+    hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(a->N)), dim3(256), 0, s,
+                       (const int4 *) p.rowstat, a->N, a->MC, a->kind, (int) PrecOf<T>::value,
+                       p.skip, p.st);
+    if ((rc = report("k_reduce_rowstat"))) return rc;
     if (a->kind == 0 && a->batch) {
         const bmlhip_batch *b = a->batch;
         hipLaunchKernelGGL(k_seg_sum, dim3(b->nsys), dim3(256), 0, s, p.rowtr, b->d_row_off,
@@ -1363,7 +1434,8 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
     p.ab_sys = batch ? batch->d_ab : nullptr;
     p.st = bmlhip_status_dev();
     p.rownorm = rownorm ? rownorm : (double *) bmlhip_scratch(0, sizeof(double) * (size_t) (N + 1));
-    if (!p.rownorm) return 4;
+    p.rowstat = (int4 *) bmlhip_scratch(7, sizeof(int4) * (size_t) (N + 1));
+    if (!p.rownorm || !p.rowstat) return 4;
     int rc = bmlhip_status_reset();
     if (rc) return rc;
     // a row's union has at most MA + MB columns; table at most 3/4 full
@@ -1380,6 +1452,9 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
             hipLaunchKernelGGL(k_add_global<T>, dim3(blocks), dim3(64), 0, s, p, tbits, tabs);
             if ((rc = report("k_add_global"))) return rc;
         }
+        hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(N)), dim3(256), 0, s,
+                           (const int4 *) p.rowstat, N, MA, 0, 0, skip, p.st);
+        if ((rc = report("k_reduce_rowstat"))) return rc;
     }
     if (mode && batch) {
         hipLaunchKernelGGL(k_seg_sum, dim3(batch->nsys), dim3(256), 0, s, p.rownorm,
