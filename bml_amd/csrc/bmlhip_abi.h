@@ -34,6 +34,16 @@ typedef struct bmlhip_status {
     long long prof_touched[5];  /* distinct columns before thresholding */
 } bmlhip_status;
 
+/* Per-row records: kernels store (kept, distinct, products, nnz of A's row)
+ * per row as int4 into scratch slot BMLHIP_ROWSTAT_SLOT (kept = -1: row
+ * skipped) and bmlhip_rowstat_reduce folds them into the status block, with
+ * a few atomics per block instead of several per row; `what` is a mask of: */
+#define BMLHIP_ROWSTAT_SLOT 7
+#define BMLHIP_RS_OVF_KEPT 1 /* rows with kept > MC overflow */
+#define BMLHIP_RS_OVF_DIST 2 /* rows with distinct > MC overflow */
+#define BMLHIP_RS_STORED 4   /* max_nnz, total_nnz from min(kept, MC) */
+#define BMLHIP_RS_PROF 8     /* profiling sums of precision `prof` */
+
 /* ---- runtime (hip/runtime.hip) ----------------------------------------- */
 int bmlhip_init(int device);
 void bmlhip_shutdown(void);
@@ -107,6 +117,7 @@ int bmlhip_add(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
                double alpha, double beta, double threshold, int mode);
 /* as bmlhip_add with alpha = d_ab[0], beta = d_ab[1] read on the device and
  * nothing done when *d_skip != 0 */
+int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *d_skip);
 int bmlhip_add_dev(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
                    int MB, const void *d_Bv, const int *d_Bi, const int *d_Bn,
                    const double *d_ab, double threshold, int mode, const int *d_skip);

@@ -60,6 +60,7 @@ struct EbParams {
     int *soff;       // [NB*(maxs+1)] start of each slot's pair list
     int *pairs;      // [NB*maxp] jp << 16 | kp, grouped by slot
     double *rowtr;   // [2*NB] per block row tr X, tr X^2 (x2)
+    int4 *rowstat;   // [NB] (kept, touched, block pairs, nnzb of A's row)
     bmlhip_status *st;
     const int *skip;
 };
@@ -170,16 +171,8 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
             __syncthreads();
         }
     }
-    if (lane == 0) {
-        p.cnt[ib] = count;
-        // work counters: block pairs and rows
-        atomicAdd((unsigned long long *) &p.st->prof_products[p.prec], (unsigned long long) running);
-        atomicAdd((unsigned long long *) &p.st->prof_rows[p.prec], 1ull);
-        if (count > p.MC) {
-            atomicAdd(&p.st->m_overflow_rows, 1);
-            atomicMin(&p.st->first_m_overflow, ib);
-        }
-    }
+    // the numeric pass records the row's counts (touched > MC is an overflow)
+    if (lane == 0) p.cnt[ib] = count;
 }
 
 // ---------------------------------------------------------------------------
@@ -475,10 +468,8 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
     if (lane == 0) tr2_s[w] = E::re(tx2);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int stored = base < p.MC ? base : p.MC;
-        p.Cn[ib] = stored;
-        atomicMax(&p.st->max_nnz, stored);
-        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        p.Cn[ib] = base < p.MC ? base : p.MC;
+        p.rowstat[ib] = make_int4(base, count, off[count], p.An[ib]);
         if (p.kind == 0) {
             double t2 = 0.0;
             for (int v = 0; v < NUM_WAVES; v++) t2 += tr2_s[v];
@@ -506,6 +497,7 @@ struct EbAddParams {
     const double *ab;
     const int *skip;
     double *rownorm;
+    int4 *rowstat;  // [NB] (kept, 0, 0, 0)
     bmlhip_status *st;
 };
 
@@ -602,10 +594,8 @@ __global__ __launch_bounds__(64) void k_eb_add(EbAddParams p) {
     }
     for (int off = 32; off > 0; off >>= 1) nrm2 += __shfl_down(nrm2, off, 64);
     if (lane == 0) {
-        const int stored = ll < p.MA ? ll : p.MA;
-        p.An[ib] = stored;
-        atomicMax(&p.st->max_nnz, stored);
-        atomicAdd((unsigned long long *) &p.st->total_nnz, (unsigned long long) stored);
+        p.An[ib] = ll < p.MA ? ll : p.MA;
+        p.rowstat[ib] = make_int4(ll, 0, 0, 0);
         if (p.mode == 2) p.rownorm[ib] = nrm2;
     }
 }
@@ -831,7 +821,8 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     p.skb = (int *) bmlhip_scratch(4, sizeof(int) * (NB * maxs + 1));
     p.soff = (int *) bmlhip_scratch(5, sizeof(int) * (NB * (maxs + 1) + 1));
     p.pairs = (int *) bmlhip_scratch(6, sizeof(int) * (NB * maxp + 1));
-    if (!p.rowtr || !p.cnt || !p.skb || !p.soff || !p.pairs)
+    p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (NB + 1));
+    if (!p.rowtr || !p.cnt || !p.skb || !p.soff || !p.pairs || !p.rowstat)
         return fail_msg(4, "ellblock product: scratch allocation failed");
     int rc = bmlhip_status_reset();
     if (rc) return rc;
@@ -859,6 +850,10 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     }
     bmlhip_prof_mark_end();
     if (rc) return rc;
+    if ((rc = bmlhip_rowstat_reduce(a->NB, a->MC,
+                                    BMLHIP_RS_OVF_DIST | BMLHIP_RS_STORED | BMLHIP_RS_PROF,
+                                    a->prec, p.skip)))
+        return rc;
     if (a->kind == 0) {
         if ((rc = bmlhip_sum_to_status(p.rowtr, a->NB, 0))) return rc;
         if ((rc = bmlhip_sum_to_status(p.rowtr + a->NB, a->NB, 1))) return rc;
@@ -891,7 +886,8 @@ int eb_add_typed(int NB, int bt, const int *bsize, int MA, void *Av, int *Ai, in
     p.skip = skip;
     p.st = bmlhip_status_dev();
     p.rownorm = (double *) bmlhip_scratch(0, sizeof(double) * ((size_t) NB + 1));
-    if (!p.rownorm) return fail_msg(4, "ellblock add: scratch allocation failed");
+    p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * ((size_t) NB + 1));
+    if (!p.rownorm || !p.rowstat) return fail_msg(4, "ellblock add: scratch allocation failed");
     int rc = bmlhip_status_reset();
     if (rc) return rc;
     if (NB == 0) return 0;
@@ -906,6 +902,8 @@ int eb_add_typed(int NB, int bt, const int *bsize, int MA, void *Av, int *Ai, in
     default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
     }
     if ((rc = report("k_eb_add"))) return rc;
+    // overflowing rows were counted in the kernel (the first overflowing entry)
+    if ((rc = bmlhip_rowstat_reduce(NB, MA, BMLHIP_RS_STORED, 0, skip))) return rc;
     if (mode == 2) return bmlhip_sum_to_status(p.rownorm, NB, 2);
     return 0;
 }

@@ -709,13 +709,13 @@ __global__ __launch_bounds__(64) void k_spgemm_global(SpParams p, unsigned tsize
     }
 }
 
-// Reduces the per-row records (kept, distinct, products, nnz of A's row) of
-// a row product or an add into the status block: rows over MC and the
-// first of them (kind 1, AB: the distinct count is checked, as the reference
-// does), largest and total stored row, largest distinct count, and with
-// prof > 0 the profiling sums of that precision.  One set of atomics per
-// block.
-__global__ __launch_bounds__(256) void k_reduce_rowstat(const int4 *rs, int N, int MC, int kind,
+// Reduces per-row records (kept, distinct, products, nnz of A's row), kept
+// = -1 marking a skipped row, into the status block.  `what` selects:
+// BMLHIP_RS_OVF_KEPT / BMLHIP_RS_OVF_DIST count the rows whose kept /
+// distinct count exceeds MC (and the first of them), BMLHIP_RS_STORED the
+// largest and total stored row (kept capped at MC), BMLHIP_RS_PROF the
+// profiling sums of precision `prof`.  One set of atomics per block.
+__global__ __launch_bounds__(256) void k_reduce_rowstat(const int4 *rs, int N, int MC, int what,
                                                         int prof, const int *skip,
                                                         bmlhip_status *st) {
     if (skip && *skip) return;
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(256) void k_reduce_rowstat(const int4 *rs, int N, i
         const int4 r = rs[i];
         if (r.x < 0) continue;
         const int stored = r.x < MC ? r.x : MC;
-        if (kind == 1 ? r.y > MC : r.x > MC) {
+        if (((what & BMLHIP_RS_OVF_KEPT) && r.x > MC) || ((what & BMLHIP_RS_OVF_DIST) && r.y > MC)) {
             ovf++;
             first = i < first ? i : first;
         }
@@ -771,11 +771,11 @@ __global__ __launch_bounds__(256) void k_reduce_rowstat(const int4 *rs, int N, i
             atomicAdd(&st->m_overflow_rows, ovf);
             atomicMin(&st->first_m_overflow, first);
         }
-        if (rows) {
+        if ((what & BMLHIP_RS_STORED) && rows) {
             atomicMax(&st->max_nnz, mx);
             atomicAdd((unsigned long long *) &st->total_nnz, tot);
         }
-        if (prof > 0 && rows) {
+        if ((what & BMLHIP_RS_PROF) && rows) {
             atomicMax(&st->max_touched, mt);
             atomicAdd((unsigned long long *) &st->prof_products[prof], prods);
             atomicAdd((unsigned long long *) &st->prof_rows[prof], rows);
@@ -1341,7 +1341,7 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     p.rowtr = a->batch ? a->batch->d_rowtr
                        : (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
     p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
-    p.rowstat = (int4 *) bmlhip_scratch(7, sizeof(int4) * (size_t) (a->N + 1));
+    p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (a->N + 1));
     p.remap = a->remap;
     // global-table pass: one table per block, >= 2x the row bound
     long long bound = (long long) a->MA * a->MB;
@@ -1387,9 +1387,12 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     hipLaunchKernelGGL(k_spgemm_global<T>, dim3(gblocks), dim3(64), 0, s, p, (unsigned) gsize, gtab,
                        cursor);
     if ((rc = report("k_spgemm_global"))) return rc;
+    // kind 1 (AB) checks the distinct count against MC, as the reference does
     hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(a->N)), dim3(256), 0, s,
-                       (const int4 *) p.rowstat, a->N, a->MC, a->kind, (int) PrecOf<T>::value,
-                       p.skip, p.st);
+                       (const int4 *) p.rowstat, a->N, a->MC,
+                       (a->kind == 1 ? BMLHIP_RS_OVF_DIST : BMLHIP_RS_OVF_KEPT) |
+                           BMLHIP_RS_STORED | BMLHIP_RS_PROF,
+                       (int) PrecOf<T>::value, p.skip, p.st);
     if ((rc = report("k_reduce_rowstat"))) return rc;
     if (a->kind == 0 && a->batch) {
         const bmlhip_batch *b = a->batch;
@@ -1434,7 +1437,7 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
     p.ab_sys = batch ? batch->d_ab : nullptr;
     p.st = bmlhip_status_dev();
     p.rownorm = rownorm ? rownorm : (double *) bmlhip_scratch(0, sizeof(double) * (size_t) (N + 1));
-    p.rowstat = (int4 *) bmlhip_scratch(7, sizeof(int4) * (size_t) (N + 1));
+    p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (N + 1));
     if (!p.rownorm || !p.rowstat) return 4;
     int rc = bmlhip_status_reset();
     if (rc) return rc;
@@ -1453,7 +1456,8 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
             if ((rc = report("k_add_global"))) return rc;
         }
         hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(N)), dim3(256), 0, s,
-                           (const int4 *) p.rowstat, N, MA, 0, 0, skip, p.st);
+                           (const int4 *) p.rowstat, N, MA,
+                           BMLHIP_RS_OVF_KEPT | BMLHIP_RS_STORED, 0, skip, p.st);
         if ((rc = report("k_reduce_rowstat"))) return rc;
     }
     if (mode && batch) {
@@ -1592,6 +1596,15 @@ int bmlhip_add(int prec, int N, int MA, void *Av, int *Ai, int *An, int MB, cons
 #define CALL(T) add_typed<T>(N, MA, Av, Ai, An, MB, Bv, Bi, Bn, alpha, beta, thr, mode)
     DISPATCH(prec, CALL)
 #undef CALL
+}
+
+int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *skip) {
+    void *rs = bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (N + 1));
+    if (!rs) return fail_msg(4, "scratch allocation failed");
+    hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(N)), dim3(256), 0,
+                       (hipStream_t) bmlhip_stream(), (const int4 *) rs, N, MC, what, prof, skip,
+                       bmlhip_status_dev());
+    return report("k_reduce_rowstat");
 }
 
 int bmlhip_add_dev(int prec, int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
