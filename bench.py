@@ -13,6 +13,12 @@ The lattice uses hopping -0.1 (+-0.01 disorder) instead of -0.5: at -0.5 the
 density matrix keeps ~9000 entries per row above 1e-5, so M = 256 overflows
 (the reference exits with "Number of non-zeroes per row > M").
 
+The row products run on the unordered engine (bml_hip_set_row_engine:
+several wavefronts per row, LDS atomic adds; sums agree with the reference's
+to rounding, the FP64 tolerance being 1e-12 x max|X^2|); the same solves on
+the exact engine (the reference's summation order, bit-identical results)
+are reported beside them as "exact_engine".
+
 Prints one JSON line (rank 0) with the roofline of the dominant kernel
 (HIP-event durations inside the timed region) and the reference's CPU SP2
 iterations timed on the host (oracle/_ref, bml's own C code) as cpu_baseline.
@@ -330,7 +336,8 @@ def x2_config5(hip, G):
     hip.synchronize()
     dt = (time.perf_counter() - t0) / reps
     prof = hip.profile_end()
-    roof = roofline(prof, 4, 20, "k_spgemm_mw<cf64> (double complex X^2)")
+    eng = "unord" if hip.get_row_engine() == hip.ROW_ENGINE_UNORDERED else "mw"
+    roof = roofline(prof, 4, 20, f"k_spgemm_{eng}<cf64> (double complex X^2)")
     mx, tot = X.row_stats()
     m2, tot2 = X2.row_stats()
     return {"workload": "double complex Hermitian 64^3 lattice (Peierls phases), N=262144, "
@@ -363,6 +370,21 @@ def main():
     N = H.N
     dH = hip.from_ell(H)
 
+    # the exact engine first (reported beside the headline), then the
+    # unordered engine for the headline
+    hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+    time_solves(hip, dH, N, 1, True)
+    te_mix, pe_mix, _, Xe_mix = time_solves(hip, dH, N, args.steps, True)
+    te_dbl, pe_dbl, _, Xe_dbl = time_solves(hip, dH, N, args.steps, False)
+    exact = {"mixed_solves_per_s": round(args.steps / te_mix, 4),
+             "fp64_solves_per_s": round(args.steps / te_dbl, 4),
+             "roofline_fp32_x2": roofline(pe_mix, 1, 8, "k_spgemm_lds<float> (exact engine)"),
+             "roofline_fp64_x2": roofline(pe_dbl, 2, 12, "k_spgemm_lds<double> (exact engine)")}
+    Xe = Xe_dbl.get()
+    Xe_mix.free()
+    Xe_dbl.free()
+
+    hip.set_row_engine(hip.ROW_ENGINE_UNORDERED)
     for _ in range(args.warmup):
         time_solves(hip, dH, N, 1, True)
         time_solves(hip, dH, N, 1, False)
@@ -371,11 +393,14 @@ def main():
     t_dbl, p_dbl, r_dbl, X_dbl = time_solves(hip, dH, N, args.steps, False)
     rm, hm = r_mix[-1]
     rd, hd = r_dbl[-1]
-    diff = ell_max_diff(X_mix.get(), X_dbl.get())
+    Xd = X_dbl.get()
+    diff = ell_max_diff(X_mix.get(), Xd)
+    exact["max_abs_diff_fp64_vs_unordered_fp64"] = ell_max_diff(Xe, Xd)
+    del Xe, Xd
 
     # dominant kernel of the timed (mixed) region: the FP32 row product
-    roof = roofline(p_mix, 1, 8, "k_spgemm_lds<float> (FP32 SP2 iterations)")
-    roof64 = roofline(p_dbl, 2, 12, "k_spgemm_lds<double> (FP64 SP2 solves)")
+    roof = roofline(p_mix, 1, 8, "k_spgemm_unord<float, 4> (FP32 SP2 iterations)")
+    roof64 = roofline(p_dbl, 2, 12, "k_spgemm_unord<double, 4> (FP64 SP2 solves)")
     traffic = profiled_traffic()
     for r, p in ((roof, 1), (roof64, 2)):
         if r is not None and p in traffic:
@@ -414,6 +439,8 @@ def main():
         "max_abs_diff_mixed_vs_fp64": diff,
         "roofline": roof,
         "roofline_fp64_x2": roof64,
+        "row_engine": "unordered",
+        "exact_engine": exact,
         "copy_calibration": calib,
         "x2_config1": None if args.only_headline else x2_config1(hip, G, cpu=not args.no_cpu),
         "x2_config5": None if args.only_headline else x2_config5(hip, G),

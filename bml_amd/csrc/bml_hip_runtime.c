@@ -216,3 +216,21 @@ const char *bml_hip_version(void)
 {
     return "bml_hip 0.1 (gfx950)";
 }
+
+/* ---- row-product engine ------------------------------------------------- */
+
+static int s_row_engine = BML_HIP_ROW_ENGINE_EXACT;
+
+void bml_hip_set_row_engine(int engine)
+{
+    if (engine != BML_HIP_ROW_ENGINE_EXACT && engine != BML_HIP_ROW_ENGINE_UNORDERED) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "bml_hip_set_row_engine: unknown engine %d", engine);
+        return;
+    }
+    s_row_engine = engine;
+}
+
+int bml_hip_get_row_engine(void)
+{
+    return s_row_engine;
+}

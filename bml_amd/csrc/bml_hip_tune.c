@@ -101,7 +101,40 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
             tm -= 64;
         t = tm;
     }
+    int engine = bml_hip_get_row_engine() == BML_HIP_ROW_ENGINE_UNORDERED && g == 64
+                     ? BMLHIP_ROW_UNORDERED
+                     : BMLHIP_ROW_EXACT;
     const char *e;
+    if (engine == BMLHIP_ROW_UNORDERED) {
+        /* Four wavefronts share a row's table and no first-touch list is
+         * kept (key + double-component accumulator per slot).  Every
+         * wavefront's step in flight reserves its row of B, so the table
+         * needs est + 4 max_b; take the largest table that keeps the most
+         * blocks per CU.  On the 32^3 lattice X^2 (FP64, 1177 distinct
+         * columns per row) 4 wavefronts x 2560 slots (5 blocks per CU) ran
+         * 2.40 ms, 2 x 2048 3.03 ms, 8 x 3584 2.47 ms; the exact engine
+         * 4.75 ms. */
+        waves = (e = getenv("BML_HIP_WAVES")) ? atoi(e) : 4;
+        long long nu = est * 3 / 2;
+        if (nu < est + (long long) waves * max_b)
+            nu = est + (long long) waves * max_b;
+        nu = (nu + 64 + 63) & ~63ll;
+        /* double-component accumulators (8 or 16 bytes) + key */
+        const long long su = (precision == 3 || precision == 4 ? 16 : 8) + 4;
+        t = 0;
+        for (int k = 8; k >= 2 && !t; k--) {
+            const long long per = (160 * 1024 / k) / 1024 * 1024 - 1024;
+            const long long tk = (per / su - 72) / 64 * 64;
+            if (tk >= nu - 64)
+                t = tk;
+        }
+        if (!t) {
+            t = nu;
+            while (t > 64 && (t + 72) * su > 159 * 1024)
+                t -= 64;
+        }
+    }
+    out->engine = engine;
     out->waves = (e = getenv("BML_HIP_WAVES")) ? atoi(e) : waves;
     out->group = (e = getenv("BML_HIP_GROUP")) ? atoi(e) : g;
     out->table = (e = getenv("BML_HIP_TABLE")) ? atoi(e) : (int) t;

@@ -9,13 +9,14 @@ typedef struct bmlh_spgemm_shape {
     int table;  /* LDS hash slots per row */
     int bucket; /* slots per probe */
     int remap;  /* XCD-aware block order */
-    int waves;  /* wavefronts per row (long rows) */
+    int waves;  /* wavefronts per row (long rows; unordered engine) */
+    int engine; /* BMLHIP_ROW_EXACT or BMLHIP_ROW_UNORDERED */
 } bmlh_spgemm_shape;
 
 /* Launch shape for a product whose left rows hold up to max_a entries and
  * right rows up to max_b; touched_hint is the largest distinct-column count
  * seen the last time (0 = unknown), when the left rows held up to
- * hint_max_a entries. */
+ * hint_max_a entries.  The engine follows bml_hip_set_row_engine. */
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
                       int hint_max_a, bmlh_spgemm_shape *out);
 

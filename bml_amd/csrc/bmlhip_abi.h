@@ -104,9 +104,14 @@ typedef struct bmlhip_spgemm_args {
     int bucket;  /* slots per probe: 2, 4 or 8 */
     int remap;   /* 1: XCD-aware block-to-row order */
     int waves;   /* group 64: wavefronts sharing one row's table (1 = one) */
+    int engine;  /* BMLHIP_ROW_EXACT: sums in the reference's jp order, first-touch
+                    output order; BMLHIP_ROW_UNORDERED (group 64): waves
+                    wavefronts per row with LDS atomic adds, table order */
     const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
     const bmlhip_batch *batch; /* not NULL: per-system traces into batch->d_trx/d_trx2 */
 } bmlhip_spgemm_args;
+#define BMLHIP_ROW_EXACT 0
+#define BMLHIP_ROW_UNORDERED 1
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);
 
 /* A <- alpha*A + beta*B (mode 0), or that plus the norm over the union of
@@ -115,9 +120,9 @@ int bmlhip_spgemm(const bmlhip_spgemm_args *a);
 int bmlhip_add(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
                int MB, const void *d_Bv, const int *d_Bi, const int *d_Bn,
                double alpha, double beta, double threshold, int mode);
+int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *d_skip);
 /* as bmlhip_add with alpha = d_ab[0], beta = d_ab[1] read on the device and
  * nothing done when *d_skip != 0 */
-int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *d_skip);
 int bmlhip_add_dev(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
                    int MB, const void *d_Bv, const int *d_Bi, const int *d_Bn,
                    const double *d_ab, double threshold, int mode, const int *d_skip);

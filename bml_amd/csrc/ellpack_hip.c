@@ -428,6 +428,7 @@ static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
     a.bucket = sh.bucket;
     a.remap = sh.remap;
     a.waves = sh.waves;
+    a.engine = sh.engine;
     return bmlh_ok(bmlhip_spgemm(&a), what);
 }
 

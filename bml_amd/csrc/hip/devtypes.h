@@ -132,6 +132,23 @@ template <> struct Elem<cf64> {
     __device__ static double re_sq(cf64 y) { return y.x * y.x - y.y * y.y; }
 };
 
+// Atomic add into an LDS slot (ds_add_f32 / ds_add_f64; a complex value is
+// two component adds).  Concurrent adds to one slot land in arrival order.
+__device__ __forceinline__ void lds_add(float *p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(cf32 *p, cf32 v) {
+    lds_add(&p->x, v.x);
+    lds_add(&p->y, v.y);
+}
+__device__ __forceinline__ void lds_add(cf64 *p, cf64 v) {
+    lds_add(&p->x, v.x);
+    lds_add(&p->y, v.y);
+}
+
 // 64-lane wavefront helpers
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return (int) __lane_id(); }

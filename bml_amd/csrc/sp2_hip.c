@@ -81,6 +81,7 @@ static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *
     a.bucket = sh.bucket;
     a.remap = sh.remap;
     a.waves = sh.waves;
+    a.engine = sh.engine;
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
            bmlh_ok(bmlhip_add_dev(X->matrix_precision, X->N, X->M, X->value, X->index, X->nnz,
@@ -313,6 +314,7 @@ static int batch_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t
     a.bucket = sh->bucket;
     a.remap = sh->remap;
     a.waves = sh->waves;
+    a.engine = sh->engine;
     a.batch = &d->b;
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide_batch(&d->b, d->nocc, d->branch, d->stats), what) &&

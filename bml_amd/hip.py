@@ -120,6 +120,8 @@ _SIGS = {
     "bml_hip_profile_end": (None, [C.POINTER(Profile)]),
 }
 _SIGS.update({
+    "bml_hip_set_row_engine": (None, [_I]),
+    "bml_hip_get_row_engine": (_I, []),
     "bml_hip_set_block_engine": (None, [_I]),
     "bml_hip_set_block_sizes": (None, [_P, _I, _I]),
     "bml_hip_get_block_sizes": (_P, [_I, _I]),
@@ -149,6 +151,7 @@ _SIGS.update({
 _OPTIONAL: dict = {}
 
 BLOCK_ENGINE_AUTO, BLOCK_ENGINE_EXACT, BLOCK_ENGINE_MFMA, BLOCK_ENGINE_MFMA_BF16 = -1, 0, 1, 2
+ROW_ENGINE_EXACT, ROW_ENGINE_UNORDERED = 0, 1
 
 
 def header_symbols() -> list[str]:
@@ -520,6 +523,15 @@ class EllblockHip:
 
 def set_block_engine(engine: int) -> None:
     _call("bml_hip_set_block_engine", engine)
+
+
+def set_row_engine(engine: int) -> None:
+    """bml_hip_set_row_engine: ROW_ENGINE_EXACT or ROW_ENGINE_UNORDERED."""
+    _call("bml_hip_set_row_engine", engine)
+
+
+def get_row_engine() -> int:
+    return int(lib().bml_hip_get_row_engine())
 
 
 def block_matrix(precision: int, bsize, MB: int, M: int = 0) -> EllblockHip:

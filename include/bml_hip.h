@@ -129,6 +129,22 @@ void *bml_hip_stream(void);
 /* Library build string. */
 const char *bml_hip_version(void);
 
+/* Engine of the ELLPACK row products (x2, AB, multiply, and the SP2 drivers):
+ *   EXACT      one wavefront per row sums each column in the reference's
+ *              order (jp ascending, bml_multiply_ellpack_typed.c:280-293)
+ *              and writes the kept columns in first-touch order: results
+ *              bit-identical to the reference (the default);
+ *   UNORDERED  rows of 64+ lanes: two wavefronts share each row's table and
+ *              add products with LDS atomics, so a column's sum is formed in
+ *              arrival order (agrees with EXACT to rounding and may vary in
+ *              the last bits between runs) and the kept columns are written
+ *              in table order.  Rows of shorter groups use EXACT.
+ * No bml counterpart. */
+#define BML_HIP_ROW_ENGINE_EXACT 0
+#define BML_HIP_ROW_ENGINE_UNORDERED 1
+void bml_hip_set_row_engine(int engine);
+int bml_hip_get_row_engine(void);
+
 /* ---- ELLPACK-R: allocation and conversion ------------------------------- */
 
 /* bml_zero_matrix_ellpack (ellpack/bml_allocate_ellpack.c, typed

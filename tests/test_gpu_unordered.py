@@ -1,0 +1,127 @@
+"""The unordered ELLPACK row engine (bml_hip_set_row_engine(UNORDERED)).
+
+Two wavefronts share each row's table and add products with LDS atomics, so
+a column's sum is formed in arrival order and the row comes out in table
+order.  Each result is checked against the oracle (the reference's sums) as
+sets per row: values within 1e-12 x max|ref| in double precision (2e-5 in
+single), sparsity patterns equal except entries that close to the
+threshold, traces to the same relative tolerance.  Rows that overflow the
+LDS table are finished by the exact global-table pass.
+"""
+import numpy as np
+import pytest
+
+import orc
+from bml_amd import generators as G
+from helpers import DTYPES, assert_ell_close, rel_close
+
+pytestmark = pytest.mark.gpu
+
+
+def vtol(dtype):
+    return 1e-12 if np.dtype(dtype) in (np.float64, np.complex128) else 2e-5
+
+
+@pytest.fixture
+def unordered(hip):
+    hip.set_row_engine(hip.ROW_ENGINE_UNORDERED)
+    assert hip.get_row_engine() == hip.ROW_ENGINE_UNORDERED
+    yield hip
+    hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(600, 160, 120), (1500, 320, 300), (257, 128, 100)])
+def test_x2_unordered_matches_oracle(unordered, dtype, shape):
+    """Rows of B of 100-300 entries (64-lane rows; the longest need the
+    chunks beyond the first three)."""
+    hip = unordered
+    N, M, fill = shape
+    A = G.random_ell(N, M, fill, 300 + N, dtype)
+    thr = 0.05
+    ref, rtr = orc.x2(A, N, thr)
+    X2 = hip.zero_matrix(A.precision, N, N)
+    tr = hip.multiply_x2(hip.from_ell(A), X2, thr)
+    assert_ell_close(X2.get(), ref, vtol(dtype), thr, f"x2 {shape}")
+    assert rel_close(tr[0], rtr[0], vtol(dtype)) and rel_close(tr[1], rtr[1], vtol(dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("table", [192, 704])
+def test_unordered_table_overflow_goes_to_exact_pass(unordered, monkeypatch, dtype, table):
+    """A table too small for the rows: reservations fail, the rows go to the
+    global-table pass (exact engine) and the result is still the product."""
+    hip = unordered
+    A = G.random_ell(800, 160, 120, 5, dtype)
+    ref, rtr = orc.x2(A, 800, 0.05)
+    monkeypatch.setenv("BML_HIP_TABLE", str(table))
+    X2 = hip.zero_matrix(A.precision, A.N, 800)
+    tr = hip.multiply_x2(hip.from_ell(A), X2, 0.05)
+    assert_ell_close(X2.get(), ref, vtol(dtype), 0.05, f"table {table}")
+    assert rel_close(tr[1], rtr[1], vtol(dtype))
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_unordered_wavefronts_per_row(unordered, monkeypatch, waves):
+    hip = unordered
+    A = G.random_ell(1200, 256, 220, 9, np.float64)
+    ref, rtr = orc.x2(A, 1200, 0.1)
+    monkeypatch.setenv("BML_HIP_WAVES", str(waves))
+    X2 = hip.zero_matrix(A.precision, A.N, 1200)
+    tr = hip.multiply_x2(hip.from_ell(A), X2, 0.1)
+    assert_ell_close(X2.get(), ref, 1e-12, 0.1, f"waves {waves}")
+    assert rel_close(tr[1], rtr[1], 1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_multiply_ab_unordered(unordered, dtype):
+    """C = A B for two different matrices (bml_multiply with A != B), then
+    alpha, beta through the add."""
+    hip = unordered
+    A = G.random_ell(700, 160, 110, 21, dtype)
+    B = G.random_ell(700, 160, 120, 22, dtype)
+    ref = orc.ab(A, B, 700, 0.02)
+    C = hip.zero_matrix(A.precision, 700, 700)
+    hip.multiply_AB(hip.from_ell(A), hip.from_ell(B), C, 0.02)
+    assert_ell_close(C.get(), ref, 1e-12, 0.02, "AB")
+
+
+def test_sp2_unordered_matches_exact(hip):
+    """SP2 on the 12^3 lattice: the unordered engine's solve takes the same
+    branches and lands on the exact engine's density matrix."""
+    H = G.lattice_hamiltonian(12, 12, 12, M=16, hopping=-0.1, disorder=0.01)
+    dH = hip.from_ell(H)
+    out = {}
+    for eng in (hip.ROW_ENGINE_EXACT, hip.ROW_ENGINE_UNORDERED):
+        hip.set_row_engine(eng)
+        try:
+            X = hip.zero_matrix(H.precision, H.N, H.N)
+            r, _ = hip.sp2(dH, X, H.N / 2, threshold=1e-7, tol=1e-9, maxit=60)
+        finally:
+            hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+        out[eng] = (r, X.get().to_dense())
+    (re_, De), (ru, Du) = out[hip.ROW_ENGINE_EXACT], out[hip.ROW_ENGINE_UNORDERED]
+    assert ru.iterations == re_.iterations and ru.converged == re_.converged
+    assert abs(ru.trace_x - re_.trace_x) <= 1e-9
+    assert np.abs(Du - De).max() <= 1e-9
+
+
+def test_batch_unordered_matches_exact(hip):
+    shapes = [(4, 4, 4), (6, 6, 4), (8, 8, 8), (6, 6, 6), (10, 8, 6)]
+    Hs = [G.lattice_hamiltonian(*L, M=16, hopping=-0.1, disorder=0.01, seed=40 + k)
+          for k, L in enumerate(shapes)]
+    dHs = [hip.from_ell(H) for H in Hs]
+    nocc = [H.N / 2 for H in Hs]
+    res = {}
+    for eng in (hip.ROW_ENGINE_EXACT, hip.ROW_ENGINE_UNORDERED):
+        hip.set_row_engine(eng)
+        try:
+            Xs = [hip.zero_matrix(H.precision, H.N, 512) for H in Hs]
+            r = hip.sp2_batch(dHs, Xs, nocc, threshold=1e-6, tol=1e-9, maxit=50)
+        finally:
+            hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+        res[eng] = (r, [X.get().to_dense() for X in Xs])
+    (re_, De), (ru, Du) = res[hip.ROW_ENGINE_EXACT], res[hip.ROW_ENGINE_UNORDERED]
+    for a, b, da, db in zip(re_, ru, De, Du):
+        assert a.iterations == b.iterations and a.converged == b.converged
+        assert np.abs(da - db).max() <= 1e-9
