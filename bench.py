@@ -217,7 +217,8 @@ def batched_sp2(hip, G, plain_sample=32):
     (near-cubic lattices), batched into one block-diagonal matrix: FP64 and
     mixed (FP32, last 4 iterations FP64), against the plain FP64 path (one
     bml_hip_sp2_ellpack solve after another, timed on the first
-    plain_sample systems and scaled to 512).  Hopping -0.1 and M = 256 as in
+    plain_sample systems and scaled to 512) on the same row engine and on
+    the exact engine.  Hopping -0.1 and M = 256 as in
     config 2 (at -0.5 the rows overflow M); nocc = floor(N/2)."""
     from bml_amd import generators as Gm
     sizes = Gm.batch_sizes(512)
@@ -246,21 +247,30 @@ def batched_sp2(hip, G, plain_sample=32):
                      "max_trace_error": max(abs(r.trace_x - c) for r, c in zip(res, nocc))}
         for X in Xs:
             X.free()
-    # plain FP64 path: one solve after another
+    # plain FP64 path: one solve after another, on the same row engine and
+    # on the exact engine (the library's default)
     sub = list(range(plain_sample))
-    Xp = [hip.zero_matrix(Hs[k].precision, Hs[k].N, Mb) for k in sub]
-    hip.sp2(dHs[0], Xp[0], nocc[0], threshold=THR, tol=TOL, maxit=MAXIT)
-    hip.synchronize()
-    t0 = time.perf_counter()
-    for k in sub:
-        hip.sp2(dHs[k], Xp[k], nocc[k], threshold=THR, tol=TOL, maxit=MAXIT)
-    hip.synchronize()
-    dt = time.perf_counter() - t0
     share = sum(Hs[k].N for k in sub) / rows
-    out["plain_fp64"] = {"solves_per_s": round(len(sub) / dt, 2),
-                         "sample": f"first {len(sub)} systems ({100 * share:.1f}% of the rows)"}
-    out["mixed_batched_over_plain_fp64"] = round(out["mixed"]["solves_per_s"] /
-                                                 out["plain_fp64"]["solves_per_s"], 2)
+    engine = hip.get_row_engine()
+    for key, eng in (("plain_fp64", engine), ("plain_fp64_exact_engine", hip.ROW_ENGINE_EXACT)):
+        hip.set_row_engine(eng)
+        Xp = [hip.zero_matrix(Hs[k].precision, Hs[k].N, Mb) for k in sub]
+        hip.sp2(dHs[0], Xp[0], nocc[0], threshold=THR, tol=TOL, maxit=MAXIT)
+        hip.synchronize()
+        t0 = time.perf_counter()
+        for k in sub:
+            hip.sp2(dHs[k], Xp[k], nocc[k], threshold=THR, tol=TOL, maxit=MAXIT)
+        hip.synchronize()
+        dt = time.perf_counter() - t0
+        out[key] = {"solves_per_s": round(len(sub) / dt, 2),
+                    "sample": f"first {len(sub)} systems ({100 * share:.1f}% of the rows)"}
+        for X in Xp:
+            X.free()
+    hip.set_row_engine(engine)
+    for name in ("mixed", "fp64"):
+        for key in ("plain_fp64", "plain_fp64_exact_engine"):
+            out[f"{name}_batched_over_{key}"] = round(out[name]["solves_per_s"] /
+                                                      out[key]["solves_per_s"], 2)
     return out
 
 

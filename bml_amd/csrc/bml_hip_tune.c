@@ -121,8 +121,12 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
         nu = (nu + 64 + 63) & ~63ll;
         /* double-component accumulators (8 or 16 bytes) + key */
         const long long su = (precision == 3 || precision == 4 ? 16 : 8) + 4;
+        /* at most 6 blocks (24 wavefronts) per CU: the extra slack absorbs
+         * rows that grew past the estimate (a row that does not fit is
+         * redone by the global-table pass, far slower per row), and 24
+         * wavefronts already keep a CU busy */
         t = 0;
-        for (int k = 8; k >= 2 && !t; k--) {
+        for (int k = 6; k >= 2 && !t; k--) {
             const long long per = (160 * 1024 / k) / 1024 * 1024 - 1024;
             const long long tk = (per / su - 72) / 64 * 64;
             if (tk >= nu - 64)
