@@ -108,16 +108,16 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
     if (engine == BMLHIP_ROW_UNORDERED) {
         /* Four wavefronts share a row's table and no first-touch list is
          * kept (key + double-component accumulator per slot).  Every
-         * wavefront's step in flight reserves its row of B, so the table
-         * needs est + 4 max_b; take the largest table that keeps the most
-         * blocks per CU.  On the 32^3 lattice X^2 (FP64, 1177 distinct
-         * columns per row) 4 wavefronts x 2560 slots (5 blocks per CU) ran
-         * 2.40 ms, 2 x 2048 3.03 ms, 8 x 3584 2.47 ms; the exact engine
-         * 4.75 ms. */
+         * wavefront holds a reservation of up to max_b + 16 slots, so the
+         * table needs est + 4 (max_b + 16); take the largest table that
+         * keeps the most blocks per CU.  On the 32^3 lattice X^2 (FP64,
+         * 1177 distinct columns per row) 4 wavefronts x 2560 slots (5
+         * blocks per CU) ran 2.20 ms, 4 x 3200 (4 blocks) 2.67 ms, 8 x 3584
+         * 2.28 ms, 2 x 2048 3.03 ms; the exact engine 4.76 ms. */
         waves = (e = getenv("BML_HIP_WAVES")) ? atoi(e) : 4;
         long long nu = est * 3 / 2;
-        if (nu < est + (long long) waves * max_b)
-            nu = est + (long long) waves * max_b;
+        if (nu < est + (long long) waves * (max_b + 16))
+            nu = est + (long long) waves * (max_b + 16);
         nu = (nu + 64 + 63) & ~63ll;
         /* double-component accumulators (8 or 16 bytes) + key */
         const long long su = (precision == 3 || precision == 4 ? 16 : 8) + 4;

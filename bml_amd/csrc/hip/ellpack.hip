@@ -652,10 +652,11 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_mw(SpParams p, unsigned tsize
 // reference's jp order (it agrees with the exact engine to rounding, a few
 // ulps of the sum's magnitudes, and may differ between runs in the last
 // bits), and the output row lists its kept columns in table order.  Each
-// step reserves room for its row of B before probing: the row's columns
-// plus every wavefront's steps in flight stay within tsize - 8 slots, so a
-// probe always finds an empty slot.  A row that would pass that goes to the
-// global-table pass, which runs the exact engine.
+// wavefront holds a reservation of table slots on a block counter and tops
+// it up (to the step's row length of B + 16) before a step that could
+// outgrow it; its new columns use it up.  The reservations stay within
+// tsize - 8 slots, so a probe always finds an empty slot.  A row that would
+// pass that goes to the global-table pass, which runs the exact engine.
 //
 // Inserts follow the exact engine's bucket protocol (first empty slot of a
 // bucket, claimed with CAS; a lane that loses reads the bucket again), which
@@ -670,7 +671,9 @@ __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], con
     bool isnew[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
-        h[c] = act[c] ? home_bucket<false>(col[c], nbk) * BK : dummy;
+        // every lane hashes (idle lanes hold column of element 0): no branch
+        const unsigned hb = home_bucket<false>(col[c], nbk) * BK;
+        h[c] = act[c] ? hb : dummy;
         slot[c] = priv;
         isnew[c] = false;
     }
@@ -696,13 +699,18 @@ __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], con
             anyins = anyins | ins[c];
         }
         if (__any(anyins)) {
+            int old[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+                old[c] = atomicCAS(&keys[ins[c] ? at[c] : priv], EMPTY, col[c]);
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
-                const int old = atomicCAS(&keys[ins[c] ? at[c] : priv], EMPTY, col[c]);
-                const bool won = ins[c] & (old == EMPTY);
+                // lost to another wavefront inserting the same column: a hit
+                const bool won = ins[c] & (old[c] == EMPTY);
+                const bool met = ins[c] & (old[c] == col[c]);
                 isnew[c] = isnew[c] | won;
-                slot[c] = won ? at[c] : slot[c];
-                h[c] = won ? dummy : h[c];
+                slot[c] = (won | met) ? at[c] : slot[c];
+                h[c] = (won | met) ? dummy : h[c];
             }
         }
         bool anyp = false;
@@ -785,6 +793,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
     long long prods = 0;
     T trx = E::zero();
     bool stop = false;  // wave-uniform: this wavefront's reservation failed
+    int budget = 0;     // wave-uniform: reserved slots not yet filled
 
     auto load_b = [&](BRow<T> &r, int j, int nb, bool act) {
         const unsigned base = (unsigned) j * MB;
@@ -829,13 +838,20 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
             const int nb1 = bcast<64>(an, src);
             load_b(nxt, j1, nb1, t1 < steps);
             if (t < steps && !stop) {
-                int r = 0;
-                if (lane == 0) r = atomicAdd(&res_s, nb);
-                r = __builtin_amdgcn_readfirstlane(r);
-                if (r + nb > cap) {
-                    stop = true;
-                    if (lane == 0) ovf_s = 1;
-                } else {
+                // keep a reservation of at least nb slots for this step
+                if (budget < nb) {
+                    const int grab = nb - budget + 16;
+                    int r = 0;
+                    if (lane == 0) r = atomicAdd(&res_s, grab);
+                    r = __builtin_amdgcn_readfirstlane(r);
+                    if (r + grab > cap) {
+                        stop = true;
+                        if (lane == 0) ovf_s = 1;
+                    } else {
+                        budget += grab;
+                    }
+                }
+                if (!stop) {
                     prods += nb;
                     if (j == i) trx = E::add(trx, a);
                     int nnew = chunks(cur, 0, nb, a);
@@ -851,7 +867,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
                         }
                         nnew += chunks(extra, k0, nb, a);
                     }
-                    if (lane == 0) atomicSub(&res_s, nb - nnew);
+                    budget -= nnew;
                 }
             }
             t = t1;
