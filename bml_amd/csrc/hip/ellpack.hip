@@ -1586,7 +1586,7 @@ int launch_spgemm_mw(const SpParams &p, unsigned tsize, hipStream_t s) {
     return report("k_spgemm_mw");
 }
 
-template <typename T, int W>
+template <typename T, int W, int BK>
 int launch_spgemm_unord(const SpParams &p, unsigned tsize, hipStream_t s) {
     const size_t lds = (size_t) (tsize + table_pad<64>()) *
                        (sizeof(typename UAcc<T>::type) + sizeof(int));
@@ -1594,14 +1594,25 @@ int launch_spgemm_unord(const SpParams &p, unsigned tsize, hipStream_t s) {
         return fail_msg(2, "LDS table of %zu bytes exceeds 160 KiB", lds);
     }
     if (lds > 64 * 1024)
-        (void) hipFuncSetAttribute((const void *) k_spgemm_unord<T, W, 4>,
+        (void) hipFuncSetAttribute((const void *) k_spgemm_unord<T, W, BK>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
     if (p.N > 0) {
         bmlhip_prof_mark_begin(PrecOf<T>::value);
-        hipLaunchKernelGGL((k_spgemm_unord<T, W, 4>), dim3(p.N), dim3(64 * W), lds, s, p, tsize);
+        hipLaunchKernelGGL((k_spgemm_unord<T, W, BK>), dim3(p.N), dim3(64 * W), lds, s, p, tsize);
         bmlhip_prof_mark_end();
     }
     return report("k_spgemm_unord");
+}
+
+template <typename T, int BK>
+int launch_spgemm_unord_w(const SpParams &p, unsigned tsize, int waves, hipStream_t s) {
+    switch (waves) {
+    case 1: return launch_spgemm_unord<T, 1, BK>(p, tsize, s);
+    case 2: return launch_spgemm_unord<T, 2, BK>(p, tsize, s);
+    case 4: return launch_spgemm_unord<T, 4, BK>(p, tsize, s);
+    case 8: return launch_spgemm_unord<T, 8, BK>(p, tsize, s);
+    default: return fail_msg(2, "unordered engine: waves %d not one of 1/2/4/8", waves);
+    }
 }
 
 template <typename T, int GROUP>
@@ -1674,12 +1685,10 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     if (rc) return rc;
     const unsigned ts = (unsigned) tsize;
     if (a->engine == BMLHIP_ROW_UNORDERED && a->group == 64) {
-        switch (a->waves) {
-        case 1: rc = launch_spgemm_unord<T, 1>(p, ts, s); break;
-        case 2: rc = launch_spgemm_unord<T, 2>(p, ts, s); break;
-        case 4: rc = launch_spgemm_unord<T, 4>(p, ts, s); break;
-        case 8: rc = launch_spgemm_unord<T, 8>(p, ts, s); break;
-        default: return fail_msg(2, "unordered engine: waves %d not one of 1/2/4/8", a->waves);
+        switch (a->bucket) {
+        case 2: rc = launch_spgemm_unord_w<T, 2>(p, ts, a->waves, s); break;
+        case 4: rc = launch_spgemm_unord_w<T, 4>(p, ts, a->waves, s); break;
+        default: return fail_msg(2, "unordered engine: bucket %d not one of 2/4", a->bucket);
         }
     } else if (a->group == 64 && a->waves > 1) {
         switch (a->waves) {

@@ -1,6 +1,6 @@
 """The unordered ELLPACK row engine (bml_hip_set_row_engine(UNORDERED)).
 
-Two wavefronts share each row's table and add products with LDS atomics, so
+Several wavefronts share each row's table and add products with LDS atomics, so
 a column's sum is formed in arrival order and the row comes out in table
 order.  Each result is checked against the oracle (the reference's sums) as
 sets per row: values within 1e-12 x max|ref| in double precision (2e-5 in
@@ -61,15 +61,16 @@ def test_unordered_table_overflow_goes_to_exact_pass(unordered, monkeypatch, dty
     assert rel_close(tr[1], rtr[1], vtol(dtype))
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4])
-def test_unordered_wavefronts_per_row(unordered, monkeypatch, waves):
+@pytest.mark.parametrize("waves,bucket", [(1, 2), (2, 2), (4, 2), (8, 2), (4, 4), (8, 4)])
+def test_unordered_wavefronts_per_row(unordered, monkeypatch, waves, bucket):
     hip = unordered
     A = G.random_ell(1200, 256, 220, 9, np.float64)
     ref, rtr = orc.x2(A, 1200, 0.1)
     monkeypatch.setenv("BML_HIP_WAVES", str(waves))
+    monkeypatch.setenv("BML_HIP_BUCKET", str(bucket))
     X2 = hip.zero_matrix(A.precision, A.N, 1200)
     tr = hip.multiply_x2(hip.from_ell(A), X2, 0.1)
-    assert_ell_close(X2.get(), ref, 1e-12, 0.1, f"waves {waves}")
+    assert_ell_close(X2.get(), ref, 1e-12, 0.1, f"waves {waves} bucket {bucket}")
     assert rel_close(tr[1], rtr[1], 1e-12)
 
 
