@@ -106,36 +106,53 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
                      : BMLHIP_ROW_EXACT;
     const char *e;
     if (engine == BMLHIP_ROW_UNORDERED) {
-        /* Four wavefronts share a row's table and no first-touch list is
-         * kept (key + double-component accumulator per slot).  Every
-         * wavefront holds a reservation of up to max_b + 16 slots, so the
-         * table needs est + 4 (max_b + 16); take the largest table that
-         * keeps the most blocks per CU.  On the 32^3 lattice X^2 (FP64,
-         * 1177 distinct columns per row) 4 wavefronts x 2560 slots (5
-         * blocks per CU) ran 2.20 ms, 4 x 3200 (4 blocks) 2.67 ms, 8 x 3584
-         * 2.28 ms, 2 x 2048 3.03 ms; the exact engine 4.76 ms. */
-        waves = (e = getenv("BML_HIP_WAVES")) ? atoi(e) : 4;
+        /* W wavefronts share a row's table; no first-touch list is kept
+         * (key + double-component accumulator per slot).  Insertions are
+         * counted as they are claimed, so the table needs the row's
+         * distinct columns plus slack against growth: max(3/2 est, est +
+         * 256) + 64 slots.  For W = 4 and 8, take the largest table that
+         * keeps the most blocks per CU (at most 20 wavefronts per CU), and
+         * the W with the most wavefronts per CU (ties: the larger table).
+         * 32^3 lattice X^2 (FP64, 1177 distinct columns per row): 4 x 2560
+         * slots (5 blocks) 2.01 ms, the exact engine 4.76 ms; 64^3 complex
+         * X^2 (2139 distinct): 8 wavefronts 101-107 ms, 4 wavefronts 153 ms.
+         */
         long long nu = est * 3 / 2;
-        if (nu < est + (long long) waves * (max_b + 16))
-            nu = est + (long long) waves * (max_b + 16);
+        if (nu < est + 256)
+            nu = est + 256;
         nu = (nu + 64 + 63) & ~63ll;
         /* double-component accumulators (8 or 16 bytes) + key */
         const long long su = (precision == 3 || precision == 4 ? 16 : 8) + 4;
-        /* at most 6 blocks (24 wavefronts) per CU: the extra slack absorbs
-         * rows that grew past the estimate (a row that does not fit is
-         * redone by the global-table pass, far slower per row), and 24
-         * wavefronts already keep a CU busy */
-        t = 0;
-        for (int k = 6; k >= 2 && !t; k--) {
-            const long long per = (160 * 1024 / k) / 1024 * 1024 - 1024;
-            const long long tk = (per / su - 72) / 64 * 64;
-            if (tk >= nu - 64)
-                t = tk;
-        }
-        if (!t) {
-            t = nu;
-            while (t > 64 && (t + 72) * su > 159 * 1024)
-                t -= 64;
+        long long best_waves = -1;
+        for (int wv = 4; wv <= 8; wv *= 2) {
+            long long tw = 0, kw = 0;
+            for (int k = 32 / wv; k >= 1 && !tw; k--) {
+                const long long per = (160 * 1024 / k) / 1024 * 1024 - 1024;
+                const long long tk = (per / su - 72) / 64 * 64;
+                if (tk >= nu - 64) {
+                    tw = tk;
+                    kw = k;
+                }
+            }
+            if (!tw) {
+                tw = nu;
+                while (tw > 64 && (tw + 72) * su > 159 * 1024)
+                    tw -= 64;
+                kw = 1;
+            }
+            /* beyond ~20 wavefronts per CU the table's load and the
+             * epilogue's idle wavefronts cost more than the occupancy buys
+             * (lattice X^2: 8 x 3200 slots, 32 per CU, 2.33 ms) */
+            while (kw * wv > 20) {
+                kw--;
+                const long long per = (160 * 1024 / kw) / 1024 * 1024 - 1024;
+                tw = (per / su - 72) / 64 * 64;
+            }
+            if (kw * wv > best_waves || (kw * wv == best_waves && tw > t)) {
+                best_waves = kw * wv;
+                t = tw;
+                waves = wv;
+            }
         }
     }
     out->engine = engine;

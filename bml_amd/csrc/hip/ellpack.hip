@@ -651,31 +651,32 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_mw(SpParams p, unsigned tsize
 // A column's sum is therefore formed in arrival order rather than the
 // reference's jp order (it agrees with the exact engine to rounding, a few
 // ulps of the sum's magnitudes, and may differ between runs in the last
-// bits), and the output row lists its kept columns in table order.  Each
-// wavefront holds a reservation of table slots on a block counter and tops
-// it up (to the step's row length of B + 16) before a step that could
-// outgrow it; its new columns use it up.  The reservations stay within
-// tsize - 8 slots, so a probe always finds an empty slot.  A row that would
-// pass that goes to the global-table pass, which runs the exact engine.
+// bits), and the output row lists its kept columns in table order.  Every
+// insertion attempt is counted on a block counter before its CAS (a wave's
+// attempts of a round with one atomic, which also gives back its earlier
+// attempts that filled no slot); the count bounds the filled slots and is
+// kept within tsize - 8, so a probe always finds an empty slot.  A row that
+// would pass that goes to the global-table pass, which runs the exact
+// engine.
 //
 // Inserts follow the exact engine's bucket protocol (first empty slot of a
-// bucket, claimed with CAS; a lane that loses reads the bucket again), which
-// keeps every bucket's filled slots a prefix and every column in one slot
-// when several wavefronts insert at once.
+// bucket, claimed with CAS; a lane that loses reads the bucket again, unless
+// the winner inserted the same column), which keeps every bucket's filled
+// slots a prefix and every column in one slot when several wavefronts insert
+// at once.  probe_claim returns 0, or -1 when the claims would pass cap.
 template <int BK>
 __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], const bool (&act)[NCH],
                                            unsigned nbk, unsigned tsize, unsigned priv,
+                                           int *claims, int cap, int &lost,
                                            unsigned (&slot)[NCH]) {
     const unsigned dummy = tsize;
     unsigned h[NCH];
-    bool isnew[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
         // every lane hashes (idle lanes hold column of element 0): no branch
         const unsigned hb = home_bucket<false>(col[c], nbk) * BK;
         h[c] = act[c] ? hb : dummy;
         slot[c] = priv;
-        isnew[c] = false;
     }
     while (true) {
         Bucket<BK> kb[NCH];
@@ -699,6 +700,18 @@ __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], con
             anyins = anyins | ins[c];
         }
         if (__any(anyins)) {
+            // count the claims on the block's counter before making them: the
+            // count bounds the filled slots, and keeping it within tsize - 8
+            // leaves empty slots, so every probe ends
+            // (claims this wavefront lost earlier are given back here)
+            int n = 0;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) n += popc64(ballot64(ins[c]));
+            int base = 0;
+            if (lane_id() == 0) base = atomicAdd(claims, n - lost);
+            base = __builtin_amdgcn_readfirstlane(base);
+            lost = 0;
+            if (base + n > cap) return -1;
             int old[NCH];
 #pragma unroll
             for (int c = 0; c < NCH; c++)
@@ -708,9 +721,9 @@ __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], con
                 // lost to another wavefront inserting the same column: a hit
                 const bool won = ins[c] & (old[c] == EMPTY);
                 const bool met = ins[c] & (old[c] == col[c]);
-                isnew[c] = isnew[c] | won;
                 slot[c] = (won | met) ? at[c] : slot[c];
                 h[c] = (won | met) ? dummy : h[c];
+                lost += popc64(ballot64(ins[c] & !won));
             }
         }
         bool anyp = false;
@@ -718,10 +731,7 @@ __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], con
         for (int c = 0; c < NCH; c++) anyp = anyp | (h[c] != dummy);
         if (!__any(anyp)) break;
     }
-    int nnew = 0;
-#pragma unroll
-    for (int c = 0; c < NCH; c++) nnew += popc64(ballot64(isnew[c]));
-    return nnew;
+    return 0;
 }
 
 // The unordered engine accumulates in double components: on gfx950 an LDS
@@ -792,8 +802,8 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
 
     long long prods = 0;
     T trx = E::zero();
-    bool stop = false;  // wave-uniform: this wavefront's reservation failed
-    int budget = 0;     // wave-uniform: reserved slots not yet filled
+    bool stop = false;  // wave-uniform: the row's claims passed the table
+    int lost = 0;       // wave-uniform: claims that filled no slot, not yet given back
 
     auto load_b = [&](BRow<T> &r, int j, int nb, bool act) {
         const unsigned base = (unsigned) j * MB;
@@ -806,15 +816,17 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
         }
     };
     // add a times the chunks of r (entries k0.. of row j of B) into the table
+    // (-1: the table is full, the row goes to the global-table pass)
     auto chunks = [&](const BRow<T> &r, int k0, int nb, T a) {
         bool act[NCH];
 #pragma unroll
         for (int c = 0; c < NCH; c++) act[c] = k0 + c * 64 + lane < nb;
         unsigned slot[NCH];
-        const int nnew = probe_claim<BK>(keys, r.col, act, nbk, tsize, priv, slot);
+        if (probe_claim<BK>(keys, r.col, act, nbk, tsize, priv, &res_s, cap, lost, slot) < 0)
+            return -1;
 #pragma unroll
         for (int c = 0; c < NCH; c++) lds_add(&vals[slot[c]], UAcc<T>::to(E::mul(a, r.val[c])));
-        return nnew;
+        return 0;
     };
 
     for (int c0 = 0; c0 < nA && !stop; c0 += 64) {
@@ -838,36 +850,24 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
             const int nb1 = bcast<64>(an, src);
             load_b(nxt, j1, nb1, t1 < steps);
             if (t < steps && !stop) {
-                // keep a reservation of at least nb slots for this step
-                if (budget < nb) {
-                    const int grab = nb - budget + 16;
-                    int r = 0;
-                    if (lane == 0) r = atomicAdd(&res_s, grab);
-                    r = __builtin_amdgcn_readfirstlane(r);
-                    if (r + grab > cap) {
-                        stop = true;
-                        if (lane == 0) ovf_s = 1;
-                    } else {
-                        budget += grab;
-                    }
-                }
-                if (!stop) {
-                    prods += nb;
-                    if (j == i) trx = E::add(trx, a);
-                    int nnew = chunks(cur, 0, nb, a);
-                    for (int k0 = NCH * 64; k0 < nb; k0 += NCH * 64) {
-                        BRow<T> extra;
-                        const unsigned base = (unsigned) j * MB;
+                prods += nb;
+                if (j == i) trx = E::add(trx, a);
+                int rc = chunks(cur, 0, nb, a);
+                for (int k0 = NCH * 64; k0 < nb && rc == 0; k0 += NCH * 64) {
+                    BRow<T> extra;
+                    const unsigned base = (unsigned) j * MB;
 #pragma unroll
-                        for (int c = 0; c < NCH; c++) {
-                            const int kp = k0 + c * 64 + lane;
-                            const unsigned at2 = kp < nb ? base + (unsigned) kp : 0u;
-                            extra.col[c] = Bi[at2];
-                            extra.val[c] = Bv[at2];
-                        }
-                        nnew += chunks(extra, k0, nb, a);
+                    for (int c = 0; c < NCH; c++) {
+                        const int kp = k0 + c * 64 + lane;
+                        const unsigned at2 = kp < nb ? base + (unsigned) kp : 0u;
+                        extra.col[c] = Bi[at2];
+                        extra.val[c] = Bv[at2];
                     }
-                    budget -= nnew;
+                    rc = chunks(extra, k0, nb, a);
+                }
+                if (rc < 0) {
+                    stop = true;
+                    if (lane == 0) ovf_s = 1;
                 }
             }
             t = t1;
