@@ -134,11 +134,12 @@ const char *bml_hip_version(void);
  *              order (jp ascending, bml_multiply_ellpack_typed.c:280-293)
  *              and writes the kept columns in first-touch order: results
  *              bit-identical to the reference (the default);
- *   UNORDERED  rows of 64+ lanes: two wavefronts share each row's table and
- *              add products with LDS atomics, so a column's sum is formed in
- *              arrival order (agrees with EXACT to rounding and may vary in
- *              the last bits between runs) and the kept columns are written
- *              in table order.  Rows of shorter groups use EXACT.
+ *   UNORDERED  rows of 64+ lanes: four or eight wavefronts share each
+ *              row's table and add products with LDS atomics into double
+ *              accumulators, so a column's sum is formed in arrival order
+ *              (agrees with EXACT to rounding and may vary in the last bits
+ *              between runs) and the kept columns are written in table
+ *              order.  Rows of shorter groups use EXACT.
  * No bml counterpart. */
 #define BML_HIP_ROW_ENGINE_EXACT 0
 #define BML_HIP_ROW_ENGINE_UNORDERED 1
