@@ -126,3 +126,72 @@ def test_batch_unordered_matches_exact(hip):
     for a, b, da, db in zip(re_, ru, De, Du):
         assert a.iterations == b.iterations and a.converged == b.converged
         assert np.abs(da - db).max() <= 1e-9
+
+
+def test_unordered_empty_rows_no_diagonal_zero_threshold(unordered):
+    """Rows of ~110 entries (64-lane rows) without diagonals, every third row
+    empty, threshold 0."""
+    hip = unordered
+    A = G.random_ell(400, 128, 110, 23, np.float64, diag=False)
+    A.nnz[::3] = 0
+    ref, rtr = orc.x2(A, 400, 0.0)
+    X2 = hip.zero_matrix(A.precision, 400, 400)
+    tr = hip.multiply_x2(hip.from_ell(A), X2, 0.0)
+    got = X2.get()
+    np.testing.assert_array_equal(got.nnz, ref.nnz)
+    assert_ell_close(got, ref, 1e-12, 0.0, "empty rows")
+    # traces: per-row values summed in a fixed tree (the reference sums in
+    # row order)
+    assert rel_close(tr[0], rtr[0], 1e-13) and rel_close(tr[1], rtr[1], 1e-12)
+
+
+def test_unordered_m_overflow_raises(unordered):
+    hip = unordered
+    A = G.random_ell(300, 128, 110, 99, np.float64)
+    X2 = hip.zero_matrix(A.precision, 300, 64)
+    with pytest.raises(hip.BmlError) as e:
+        hip.multiply_x2(hip.from_ell(A), X2, 0.0)
+    assert e.value.code == hip.ERR_M_OVERFLOW
+    X2b = hip.zero_matrix(A.precision, 300, 300)
+    hip.multiply_x2(hip.from_ell(A), X2b, 0.0)
+
+
+def test_sp2_complex_unordered_matches_exact(hip):
+    """Hermitian (Peierls-phase) 8^3 lattice: the two engines' complex
+    double SP2 density matrices agree."""
+    H = G.peierls_hamiltonian(8, M=16)
+    dH = hip.from_ell(H)
+    out = {}
+    for eng in (hip.ROW_ENGINE_EXACT, hip.ROW_ENGINE_UNORDERED):
+        hip.set_row_engine(eng)
+        try:
+            X = hip.zero_matrix(H.precision, H.N, 512)
+            r, _ = hip.sp2(dH, X, H.N / 2, threshold=1e-8, tol=1e-9, maxit=60)
+        finally:
+            hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+        out[eng] = (r, X.get().to_dense())
+    (re_, De), (ru, Du) = out[hip.ROW_ENGINE_EXACT], out[hip.ROW_ENGINE_UNORDERED]
+    assert ru.iterations == re_.iterations
+    assert np.abs(Du - De).max() <= 1e-9
+
+
+def test_batch_unordered_graph_matches_eager(hip):
+    """Graph replays of the unordered engine's batched iteration give the
+    eager launches' density matrices (to rounding: the atomics' order)."""
+    shapes = [(6, 6, 4), (8, 8, 8), (10, 8, 6)]
+    Hs = [G.lattice_hamiltonian(*L, M=16, hopping=-0.1, disorder=0.01, seed=60 + k)
+          for k, L in enumerate(shapes)]
+    dHs = [hip.from_ell(H) for H in Hs]
+    nocc = [H.N / 2 for H in Hs]
+    outs = {}
+    hip.set_row_engine(hip.ROW_ENGINE_UNORDERED)
+    try:
+        for graph in (False, True):
+            Xs = [hip.zero_matrix(H.precision, H.N, 512) for H in Hs]
+            res = hip.sp2_batch(dHs, Xs, nocc, threshold=1e-6, tol=1e-9, maxit=50, graph=graph)
+            outs[graph] = (res, [X.get().to_dense() for X in Xs])
+    finally:
+        hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+    for a, b, da, db in zip(outs[False][0], outs[True][0], outs[False][1], outs[True][1]):
+        assert a.iterations == b.iterations
+        assert np.abs(da - db).max() <= 1e-9
