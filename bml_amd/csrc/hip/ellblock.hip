@@ -183,6 +183,7 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
 // accumulator: the order of the reference's block kernels.
 template <typename T, int BT>
 struct ExactAcc {
+    static constexpr bool kPipelined = false;
     static constexpr int Q = (BT * BT + 63) / 64;
     T a[Q];
     __device__ void zero() {
@@ -239,29 +240,39 @@ struct ExactAcc {
 // contiguous row of R per step.  Output: lane l, register v holds row
 // 8*(v/4) + 4*(l/32) + v%4, column l%32.
 struct MfmaAccF32 {
+    static constexpr bool kPipelined = true;
     floatx16 a;
+    // one pair's operands in registers
+    struct Frag {
+        float lv[16], rv[16];
+    };
     __device__ void zero() {
 #pragma unroll
         for (int v = 0; v < 16; v++) a[v] = 0.0f;
     }
-    __device__ void mac(const float *L, const float *R, int, int, int) {
+    __device__ static void load(const float *L, const float *R, Frag &f) {
         const int lane = lane_id();
         const int i = lane & 31, h = lane >> 5;
         const float4 *l4 = reinterpret_cast<const float4 *>(L + i * 32 + h * 16);
-        float lv[16];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const float4 t = l4[q];
-            lv[4 * q] = t.x;
-            lv[4 * q + 1] = t.y;
-            lv[4 * q + 2] = t.z;
-            lv[4 * q + 3] = t.w;
+            f.lv[4 * q] = t.x;
+            f.lv[4 * q + 1] = t.y;
+            f.lv[4 * q + 2] = t.z;
+            f.lv[4 * q + 3] = t.w;
         }
-        float rv[16];
 #pragma unroll
-        for (int s = 0; s < 16; s++) rv[s] = R[(h * 16 + s) * 32 + i];
+        for (int s = 0; s < 16; s++) f.rv[s] = R[(h * 16 + s) * 32 + i];
+    }
+    __device__ void mma(const Frag &f) {
 #pragma unroll
-        for (int s = 0; s < 16; s++) a = __builtin_amdgcn_mfma_f32_32x32x2f32(lv[s], rv[s], a, 0, 0, 0);
+        for (int s = 0; s < 16; s++) a = __builtin_amdgcn_mfma_f32_32x32x2f32(f.lv[s], f.rv[s], a, 0, 0, 0);
+    }
+    __device__ void mac(const float *L, const float *R, int, int, int) {
+        Frag f;
+        load(L, R, f);
+        mma(f);
     }
     __device__ double maxmag() const {
         float m = 0.0f;
@@ -288,7 +299,10 @@ struct MfmaAccF32 {
 // group of 8, lanes 32-63 the next); instruction q covers k = 16q .. 16q+15.
 // Same output layout as the FP32 engine.
 struct MfmaAccBF16 : MfmaAccF32 {
-    __device__ void mac(const float *L, const float *R, int, int, int) {
+    struct Frag {
+        bf16x8 av[2], bv[2];
+    };
+    __device__ static void load(const float *L, const float *R, Frag &f) {
         const int lane = lane_id();
         const int i = lane & 31, h = lane >> 5;
 #pragma unroll
@@ -296,19 +310,26 @@ struct MfmaAccBF16 : MfmaAccF32 {
             const int k0 = 16 * q + 8 * h;
             const float4 *l4 = reinterpret_cast<const float4 *>(L + i * 32 + k0);
             const float4 x0 = l4[0], x1 = l4[1];
-            bf16x8 av, bv;
-            av[0] = (__bf16) x0.x;
-            av[1] = (__bf16) x0.y;
-            av[2] = (__bf16) x0.z;
-            av[3] = (__bf16) x0.w;
-            av[4] = (__bf16) x1.x;
-            av[5] = (__bf16) x1.y;
-            av[6] = (__bf16) x1.z;
-            av[7] = (__bf16) x1.w;
+            f.av[q][0] = (__bf16) x0.x;
+            f.av[q][1] = (__bf16) x0.y;
+            f.av[q][2] = (__bf16) x0.z;
+            f.av[q][3] = (__bf16) x0.w;
+            f.av[q][4] = (__bf16) x1.x;
+            f.av[q][5] = (__bf16) x1.y;
+            f.av[q][6] = (__bf16) x1.z;
+            f.av[q][7] = (__bf16) x1.w;
 #pragma unroll
-            for (int e = 0; e < 8; e++) bv[e] = (__bf16) R[(k0 + e) * 32 + i];
-            a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, a, 0, 0, 0);
+            for (int e = 0; e < 8; e++) f.bv[q][e] = (__bf16) R[(k0 + e) * 32 + i];
         }
+    }
+    __device__ void mma(const Frag &f) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.av[q], f.bv[q], a, 0, 0, 0);
+    }
+    __device__ void mac(const float *L, const float *R, int, int, int) {
+        Frag f;
+        load(L, R, f);
+        mma(f);
     }
 };
 
@@ -318,7 +339,11 @@ struct MfmaAccBF16 : MfmaAccF32 {
 // groups g, so a lane reads 8 contiguous elements of its row of L.  Output:
 // lane l, register v holds row l/16 + 4v, column l%16 of its 16x16 tile.
 struct MfmaAccF64 {
+    static constexpr bool kPipelined = true;
     doublex4 a[2][2];
+    struct Frag {
+        double lv[2][8], rv[2][8];
+    };
     __device__ void zero() {
 #pragma unroll
         for (int ti = 0; ti < 2; ti++)
@@ -327,32 +352,38 @@ struct MfmaAccF64 {
 #pragma unroll
                 for (int v = 0; v < 4; v++) a[ti][tj][v] = 0.0;
     }
-    __device__ void mac(const double *L, const double *R, int, int, int) {
+    __device__ static void load(const double *L, const double *R, Frag &f) {
         const int lane = lane_id();
         const int c = lane & 15, g = lane >> 4;
-        double lv[2][8], rv[2][8];
 #pragma unroll
         for (int ti = 0; ti < 2; ti++) {
             const double2 *l2 = reinterpret_cast<const double2 *>(L + (ti * 16 + c) * 32 + g * 8);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const double2 t = l2[q];
-                lv[ti][2 * q] = t.x;
-                lv[ti][2 * q + 1] = t.y;
+                f.lv[ti][2 * q] = t.x;
+                f.lv[ti][2 * q + 1] = t.y;
             }
         }
 #pragma unroll
         for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-            for (int s = 0; s < 8; s++) rv[tj][s] = R[(g * 8 + s) * 32 + tj * 16 + c];
+            for (int s = 0; s < 8; s++) f.rv[tj][s] = R[(g * 8 + s) * 32 + tj * 16 + c];
+    }
+    __device__ void mma(const Frag &f) {
 #pragma unroll
         for (int s = 0; s < 8; s++)
 #pragma unroll
             for (int ti = 0; ti < 2; ti++)
 #pragma unroll
                 for (int tj = 0; tj < 2; tj++)
-                    a[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(lv[ti][s], rv[tj][s], a[ti][tj],
-                                                                     0, 0, 0);
+                    a[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.lv[ti][s], f.rv[tj][s],
+                                                                     a[ti][tj], 0, 0, 0);
+    }
+    __device__ void mac(const double *L, const double *R, int, int, int) {
+        Frag f;
+        load(L, R, f);
+        mma(f);
     }
     __device__ double maxmag() const {
         double m = 0.0;
@@ -389,6 +420,13 @@ template <typename T, int ENGINE, int BT> struct AccOf { using type = ExactAcc<T
 template <> struct AccOf<float, 1, 32> { using type = MfmaAccF32; };
 template <> struct AccOf<float, 2, 32> { using type = MfmaAccBF16; };
 template <> struct AccOf<double, 1, 32> { using type = MfmaAccF64; };
+
+// lane src's value (src wave-uniform): two scalar reads of the halves
+__device__ __forceinline__ size_t lane_get(size_t v, int src) {
+    const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) v, src);
+    const unsigned hi = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) (v >> 32), src);
+    return ((size_t) hi << 32) | lo;
+}
 
 __device__ __forceinline__ double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -436,11 +474,40 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
         if (s < count) {
             kb = p.skb[(size_t) ib * p.maxs + s];
             const int bj = p.bsize[kb];
-            for (int t = off[s]; t < off[s + 1]; t++) {
-                const int jp = (int) ((unsigned) pr[t] >> 16), kp = pr[t] & 0xffff;
-                const size_t sa = (size_t) ib * p.MA + jp;
-                const int jb = p.Ai[sa];
-                acc.mac(Av + sa * B2, Bv + ((size_t) jb * p.MB + kp) * B2, bi, p.bsize[jb], bj);
+            if constexpr (Acc::kPipelined) {
+                // up to 64 pairs at a time: lane q decodes pair t0 + q (the
+                // tile offsets), then the wavefront runs the pairs with the
+                // next pair's tiles loading while the current pair's MFMAs run
+                using Frag = typename Acc::Frag;
+                for (int t0 = off[s]; t0 < off[s + 1]; t0 += 64) {
+                    const int np = off[s + 1] - t0 < 64 ? off[s + 1] - t0 : 64;
+                    size_t la = 0, lb = 0;
+                    if (lane < np) {
+                        const unsigned pv = (unsigned) pr[t0 + lane];
+                        const size_t sa = (size_t) ib * p.MA + (pv >> 16);
+                        la = sa * B2;
+                        lb = ((size_t) p.Ai[sa] * p.MB + (pv & 0xffffu)) * B2;
+                    }
+                    Frag f0, f1;
+                    Acc::load(Av + lane_get(la, 0), Bv + lane_get(lb, 0), f0);
+                    for (int q = 0; q < np; q += 2) {
+                        if (q + 1 < np)
+                            Acc::load(Av + lane_get(la, q + 1), Bv + lane_get(lb, q + 1), f1);
+                        acc.mma(f0);
+                        if (q + 1 < np) {
+                            if (q + 2 < np)
+                                Acc::load(Av + lane_get(la, q + 2), Bv + lane_get(lb, q + 2), f0);
+                            acc.mma(f1);
+                        }
+                    }
+                }
+            } else {
+                for (int t = off[s]; t < off[s + 1]; t++) {
+                    const int jp = (int) ((unsigned) pr[t] >> 16), kp = pr[t] & 0xffff;
+                    const size_t sa = (size_t) ib * p.MA + jp;
+                    const int jb = p.Ai[sa];
+                    acc.mac(Av + sa * B2, Bv + ((size_t) jb * p.MB + kp) * B2, bi, p.bsize[jb], bj);
+                }
             }
             const double nrm = wave_max(acc.maxmag());
             keep = kb == ib || nrm > p.thr;
