@@ -568,12 +568,25 @@ struct EbAddParams {
     bmlhip_status *st;
 };
 
+// One block row per block: lane 0 lists the union (A's blocks, then B's
+// new ones, as bml_add_ellblock_typed.c:35-197), then EB_ADD_WAVES
+// wavefronts take consecutive union entries, one each per round: every
+// wavefront reads its two tiles and forms alpha*A + beta*B with the block's
+// max norm, the round's keep flags are ranked in union order, and the kept
+// tiles are written in place into A.  A tile is read before any write of
+// its round, and writes only reach slots whose A tiles were read in earlier
+// rounds or this one.
+constexpr int EB_ADD_WAVES = 4;
+
 template <typename T, int BT>
-__global__ __launch_bounds__(64) void k_eb_add(EbAddParams p) {
+__global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
     using E = Elem<T>;
     extern __shared__ int usm[];
+    __shared__ int keep_s[EB_ADD_WAVES];
+    __shared__ double nrm_s[EB_ADD_WAVES];
     if (p.skip && *p.skip) return;
-    const int ib = blockIdx.x, lane = threadIdx.x;
+    const int ib = blockIdx.x;
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int cap = p.MA + p.MB;
     int *ucol = usm, *ua = usm + cap, *ub = usm + 2 * cap;
     const double alpha = p.ab ? p.ab[0] : p.alpha;
@@ -584,8 +597,8 @@ __global__ __launch_bounds__(64) void k_eb_add(EbAddParams p) {
     T *Av = (T *) p.Av;
     const T *Bv = (const T *) p.Bv;
     constexpr size_t B2 = (size_t) BT * BT;
-    // union, lane 0 (rows hold at most MA + MB block columns)
-    if (lane == 0) {
+    // union, thread 0 (rows hold at most MA + MB block columns)
+    if (threadIdx.x == 0) {
         int n = 0;
         for (int jp = 0; jp < nA; jp++) {
             ucol[n] = p.Ai[(size_t) ib * p.MA + jp];
@@ -614,8 +627,10 @@ __global__ __launch_bounds__(64) void k_eb_add(EbAddParams p) {
     constexpr int Q = (BT * BT + 63) / 64;
     int ll = 0;
     double nrm2 = 0.0;
-    for (int u = 0; u < n; u++) {
-        const int jb = ucol[u], sa = ua[u], sb = ub[u];
+    for (int u0 = 0; u0 < n; u0 += EB_ADD_WAVES) {
+        const int u = u0 + w;
+        const bool live = u < n;
+        const int jb = live ? ucol[u] : 0, sa = live ? ua[u] : -1, sb = live ? ub[u] : -1;
         const int bj = p.bsize[jb];
         T x[Q];
         double m = 0.0;
@@ -641,29 +656,40 @@ __global__ __launch_bounds__(64) void k_eb_add(EbAddParams p) {
             if (p.mode == 2) nrm2 += E::abs2(y);
         }
         m = wave_max(m);
-        const bool keep = E::above(E::from_real(m), p.thr);
+        const bool keep = live && E::above(E::from_real(m), p.thr);
+        if (lane == 0) keep_s[w] = keep ? 1 : 0;
+        __syncthreads();  // every tile of the round read; keep flags in
+        int pos = ll, tot = 0;
+        for (int v = 0; v < EB_ADD_WAVES; v++) {
+            pos += v < w ? keep_s[v] : 0;
+            tot += keep_s[v];
+        }
         if (keep) {
-            if (ll < p.MA) {
-                const size_t d = ((size_t) ib * p.MA + ll) * B2;
+            if (pos < p.MA) {
+                const size_t d = ((size_t) ib * p.MA + pos) * B2;
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
                     const int e = lane + 64 * q;
                     if (e < BT * BT) Av[d + e] = x[q];
                 }
-                if (lane == 0) p.Ai[(size_t) ib * p.MA + ll] = jb;
-            } else if (lane == 0 && ll == p.MA) {
+                if (lane == 0) p.Ai[(size_t) ib * p.MA + pos] = jb;
+            } else if (lane == 0 && pos == p.MA) {
                 atomicAdd(&p.st->m_overflow_rows, 1);
                 atomicMin(&p.st->first_m_overflow, ib);
             }
-            ll++;
         }
-        __syncthreads();  // slot ll consumed before a later entry overwrites it
+        ll += tot;
+        __syncthreads();  // writes of the round done before the next reads
     }
     for (int off = 32; off > 0; off >>= 1) nrm2 += __shfl_down(nrm2, off, 64);
-    if (lane == 0) {
+    if (lane == 0) nrm_s[w] = nrm2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int v = 0; v < EB_ADD_WAVES; v++) t += nrm_s[v];
         p.An[ib] = ll < p.MA ? ll : p.MA;
         p.rowstat[ib] = make_int4(ll, 0, 0, 0);
-        if (p.mode == 2) p.rownorm[ib] = nrm2;
+        if (p.mode == 2) p.rownorm[ib] = t;
     }
 }
 
@@ -963,9 +989,9 @@ int eb_add_typed(int NB, int bt, const int *bsize, int MA, void *Av, int *Ai, in
         return fail_msg(5, "ellblock add: %d + %d block slots per row exceed the LDS union list",
                         MA, MB);
     switch (bt) {
-    case 8: hipLaunchKernelGGL((k_eb_add<T, 8>), dim3(NB), dim3(64), lds, s, p); break;
-    case 16: hipLaunchKernelGGL((k_eb_add<T, 16>), dim3(NB), dim3(64), lds, s, p); break;
-    case 32: hipLaunchKernelGGL((k_eb_add<T, 32>), dim3(NB), dim3(64), lds, s, p); break;
+    case 8: hipLaunchKernelGGL((k_eb_add<T, 8>), dim3(NB), dim3(64 * EB_ADD_WAVES), lds, s, p); break;
+    case 16: hipLaunchKernelGGL((k_eb_add<T, 16>), dim3(NB), dim3(64 * EB_ADD_WAVES), lds, s, p); break;
+    case 32: hipLaunchKernelGGL((k_eb_add<T, 32>), dim3(NB), dim3(64 * EB_ADD_WAVES), lds, s, p); break;
     default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
     }
     if ((rc = report("k_eb_add"))) return rc;
