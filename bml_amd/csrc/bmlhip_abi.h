@@ -39,6 +39,7 @@ typedef struct bmlhip_status {
  * skipped) and bmlhip_rowstat_reduce folds them into the status block, with
  * a few atomics per block instead of several per row; `what` is a mask of: */
 #define BMLHIP_ROWSTAT_SLOT 7
+#define BMLHIP_SYMFILL_SLOT 8 /* symmetric X^2: per-row mirror counters, kept zero */
 #define BMLHIP_RS_OVF_KEPT 1 /* rows with kept > MC overflow */
 #define BMLHIP_RS_OVF_DIST 2 /* rows with distinct > MC overflow */
 #define BMLHIP_RS_STORED 4   /* max_nnz, total_nnz from min(kept, MC) */
@@ -60,7 +61,10 @@ void *bmlhip_stream(void);
 /* Library-owned scratch, grown on demand; slot selects one of a few
  * independent buffers.  The pointer stays valid until the next call with
  * the same slot and a larger size. */
+#define BMLHIP_SCRATCH_SLOTS 10
 void *bmlhip_scratch(int slot, size_t bytes);
+/* as bmlhip_scratch, zero-filled whenever the buffer is (re)allocated */
+void *bmlhip_scratch_zeroed(int slot, size_t bytes);
 bmlhip_status *bmlhip_status_dev(void);
 /* reset the per-operation fields (not the overflow fields) */
 int bmlhip_status_reset(void);
@@ -107,6 +111,8 @@ typedef struct bmlhip_spgemm_args {
     int engine;  /* BMLHIP_ROW_EXACT: sums in the reference's jp order, first-touch
                     output order; BMLHIP_ROW_UNORDERED (group 64): waves
                     wavefronts per row with LDS atomic adds, table order */
+    int sym;     /* x2 of a symmetric / Hermitian X on the unordered engine:
+                    rows form columns k >= i, the rest is mirrored */
     const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
     const bmlhip_batch *batch; /* not NULL: per-system traces into batch->d_trx/d_trx2 */
 } bmlhip_spgemm_args;
@@ -121,6 +127,10 @@ int bmlhip_add(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
                int MB, const void *d_Bv, const int *d_Bi, const int *d_Bn,
                double alpha, double beta, double threshold, int mode);
 int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *d_skip);
+/* *symmetric = 1 when every stored (i, j, v) has (j, i, conj(v)) stored,
+ * bit for bit (synchronous; not for use under graph capture) */
+int bmlhip_is_symmetric(int prec, int N, int M, const void *d_Av, const int *d_Ai,
+                        const int *d_An, int *symmetric);
 /* as bmlhip_add with alpha = d_ab[0], beta = d_ab[1] read on the device and
  * nothing done when *d_skip != 0 */
 int bmlhip_add_dev(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,

@@ -429,6 +429,11 @@ static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
     a.remap = sh.remap;
     a.waves = sh.waves;
     a.engine = sh.engine;
+    /* BML_HIP_ASSUME_SYMMETRIC=1: the caller asserts X is symmetric
+     * (Hermitian) and x2 forms the upper triangle and mirrors it, as the SP2
+     * drivers do for a symmetric H (a test hook for that path) */
+    const char *e = getenv("BML_HIP_ASSUME_SYMMETRIC");
+    a.sym = kind == 0 && e && atoi(e) != 0;
     return bmlh_ok(bmlhip_spgemm(&a), what);
 }
 

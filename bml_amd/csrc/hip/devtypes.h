@@ -24,6 +24,7 @@ template <> struct Elem<float> {
     using real = float;
     static constexpr bool complex = false;
     __device__ static float zero() { return 0.0f; }
+    __device__ static float conj(float v) { return v; }
     __device__ static float add(float a, float b) { return a + b; }
     __device__ static float sub(float a, float b) { return a - b; }
     __device__ static float mul(float a, float b) { return a * b; }
@@ -56,6 +57,7 @@ template <> struct Elem<double> {
     using real = double;
     static constexpr bool complex = false;
     __device__ static double zero() { return 0.0; }
+    __device__ static double conj(double v) { return v; }
     __device__ static double add(double a, double b) { return a + b; }
     __device__ static double sub(double a, double b) { return a - b; }
     __device__ static double mul(double a, double b) { return a * b; }
@@ -76,6 +78,7 @@ template <> struct Elem<cf32> {
     using real = float;
     static constexpr bool complex = true;
     __device__ static cf32 zero() { return {0.0f, 0.0f}; }
+    __device__ static cf32 conj(cf32 v) { return {v.x, -v.y}; }
     __device__ static cf32 add(cf32 a, cf32 b) { return {a.x + b.x, a.y + b.y}; }
     __device__ static cf32 sub(cf32 a, cf32 b) { return {a.x - b.x, a.y - b.y}; }
     __device__ static cf32 mul(cf32 a, cf32 b) {
@@ -109,6 +112,7 @@ template <> struct Elem<cf64> {
     using real = double;
     static constexpr bool complex = true;
     __device__ static cf64 zero() { return {0.0, 0.0}; }
+    __device__ static cf64 conj(cf64 v) { return {v.x, -v.y}; }
     __device__ static cf64 add(cf64 a, cf64 b) { return {a.x + b.x, a.y + b.y}; }
     __device__ static cf64 sub(cf64 a, cf64 b) { return {a.x - b.x, a.y - b.y}; }
     __device__ static cf64 mul(cf64 a, cf64 b) {

@@ -81,6 +81,7 @@ struct SpParams {
     int4 *rowstat;    // [N]: (kept, distinct columns, products, nnz of A's row),
                       // kept = -1 for a skipped row; reduced by k_row_stats
     int remap;        // XCD-aware block-to-row order
+    int sym;          // symmetric x2: rows form only the columns k >= i
     // batched systems: row i belongs to system row_sys[i]; rows of systems
     // with sys_done set are skipped (both NULL: one system)
     const int *row_sys, *sys_done;
@@ -237,7 +238,9 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
         bool isnew[NCH];
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
-            const bool act = gact & (kbase + c * GROUP + gl < nb);
+            bool act = gact & (kbase + c * GROUP + gl < nb);
+            // the global-table pass of a symmetric x2 forms columns k >= i only
+            if constexpr (BIG) act = act & (!p.sym | (r.col[c] >= i));
             h[c] = act ? home_bucket<BIG>(r.col[c], nbk) * BK : dummy;
             slot[c] = priv;
             isnew[c] = false;
@@ -664,28 +667,28 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_mw(SpParams p, unsigned tsize
 // the winner inserted the same column), which keeps every bucket's filled
 // slots a prefix and every column in one slot when several wavefronts insert
 // at once.  probe_claim returns 0, or -1 when the claims would pass cap.
-template <int BK>
-__device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], const bool (&act)[NCH],
+template <int BK, int NC = NCH>
+__device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NC], const bool (&act)[NC],
                                            unsigned nbk, unsigned tsize, unsigned priv,
                                            int *claims, int cap, int &lost,
-                                           unsigned (&slot)[NCH]) {
+                                           unsigned (&slot)[NC]) {
     const unsigned dummy = tsize;
-    unsigned h[NCH];
+    unsigned h[NC];
 #pragma unroll
-    for (int c = 0; c < NCH; c++) {
+    for (int c = 0; c < NC; c++) {
         // every lane hashes (idle lanes hold column of element 0): no branch
         const unsigned hb = home_bucket<false>(col[c], nbk) * BK;
         h[c] = act[c] ? hb : dummy;
         slot[c] = priv;
     }
     while (true) {
-        Bucket<BK> kb[NCH];
+        Bucket<BK> kb[NC];
 #pragma unroll
-        for (int c = 0; c < NCH; c++) kb[c] = load_bucket<BK>(keys, h[c]);
-        unsigned at[NCH];
-        bool ins[NCH], anyins = false;
+        for (int c = 0; c < NC; c++) kb[c] = load_bucket<BK>(keys, h[c]);
+        unsigned at[NC];
+        bool ins[NC], anyins = false;
 #pragma unroll
-        for (int c = 0; c < NCH; c++) {
+        for (int c = 0; c < NC; c++) {
             const bool pend = h[c] != dummy;
             unsigned eh, nf;
             bucket_scan<BK>(kb[c], col[c], eh, nf);
@@ -706,18 +709,18 @@ __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], con
             // (claims this wavefront lost earlier are given back here)
             int n = 0;
 #pragma unroll
-            for (int c = 0; c < NCH; c++) n += popc64(ballot64(ins[c]));
+            for (int c = 0; c < NC; c++) n += popc64(ballot64(ins[c]));
             int base = 0;
             if (lane_id() == 0) base = atomicAdd(claims, n - lost);
             base = __builtin_amdgcn_readfirstlane(base);
             lost = 0;
             if (base + n > cap) return -1;
-            int old[NCH];
+            int old[NC];
 #pragma unroll
-            for (int c = 0; c < NCH; c++)
+            for (int c = 0; c < NC; c++)
                 old[c] = atomicCAS(&keys[ins[c] ? at[c] : priv], EMPTY, col[c]);
 #pragma unroll
-            for (int c = 0; c < NCH; c++) {
+            for (int c = 0; c < NC; c++) {
                 // lost to another wavefront inserting the same column: a hit
                 const bool won = ins[c] & (old[c] == EMPTY);
                 const bool met = ins[c] & (old[c] == col[c]);
@@ -728,7 +731,7 @@ __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NCH], con
         }
         bool anyp = false;
 #pragma unroll
-        for (int c = 0; c < NCH; c++) anyp = anyp | (h[c] != dummy);
+        for (int c = 0; c < NC; c++) anyp = anyp | (h[c] != dummy);
         if (!__any(anyp)) break;
     }
     return 0;
@@ -761,7 +764,11 @@ template <> struct UAcc<cf64> {
     __device__ static cf64 from(cf64 v) { return v; }
 };
 
-template <typename T, int W, int BK>
+// SYM (X symmetric or Hermitian, SP2 on such an H): row i forms only the
+// columns k >= i of X^2 (the other lanes of a chunk sit idle: no probe, no
+// add), the table holds about half the row's columns, and k_sym_mirror
+// copies each kept (i, k), k > i, to (k, i) conjugated.
+template <typename T, int W, int BK, bool SYM>
 __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned tsize) {
     using E = Elem<T>;
     using A = typename UAcc<T>::type;
@@ -820,12 +827,16 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
     auto chunks = [&](const BRow<T> &r, int k0, int nb, T a) {
         bool act[NCH];
 #pragma unroll
-        for (int c = 0; c < NCH; c++) act[c] = k0 + c * 64 + lane < nb;
+        for (int c = 0; c < NCH; c++) {
+            act[c] = k0 + c * 64 + lane < nb;
+            if constexpr (SYM) act[c] = act[c] & (r.col[c] >= i);
+        }
         unsigned slot[NCH];
         if (probe_claim<BK>(keys, r.col, act, nbk, tsize, priv, &res_s, cap, lost, slot) < 0)
             return -1;
 #pragma unroll
-        for (int c = 0; c < NCH; c++) lds_add(&vals[slot[c]], UAcc<T>::to(E::mul(a, r.val[c])));
+        for (int c = 0; c < NCH; c++)
+            lds_add(&vals[slot[c]], act[c] ? UAcc<T>::to(E::mul(a, r.val[c])) : Elem<A>::zero());
         return 0;
     };
 
@@ -932,6 +943,47 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
             p.rowtr[p.N + i] = E::re(d2);
         }
     }
+}
+
+// Mirror of the upper rows: entry (i, k), k > i, of row i's upper part goes
+// to row k at position Cn[k] + fill[k]++ (conjugated).  One wavefront per
+// row; rows of finished systems are skipped.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sym_mirror(SpParams p, int *fill) {
+    if (p.skip && *p.skip) return;
+    const int i = blockIdx.x * 4 + threadIdx.x / 64;
+    if (i >= p.N) return;
+    if (p.row_sys && p.sys_done[p.row_sys[i]]) return;
+    const int lane = threadIdx.x & 63;
+    const int MC = p.MC;
+    T *Cv = (T *) p.Cv;
+    const int u = p.Cn[i];
+    for (int e = lane; e < u; e += 64) {
+        const int k = p.Ci[(size_t) i * MC + e];
+        if (k > i) {
+            const int pos = p.Cn[k] + atomicAdd(&fill[k], 1);
+            if (pos < MC) {
+                p.Ci[(size_t) k * MC + pos] = i;
+                Cv[(size_t) k * MC + pos] = Elem<T>::conj(Cv[(size_t) i * MC + e]);
+            }
+        }
+    }
+}
+
+// Final row lengths after the mirror: upper + mirrored entries (capped at
+// MC; the row record keeps the full count for the overflow check), and the
+// fill counters cleared for the next product.
+__global__ __launch_bounds__(256) void k_sym_finish(SpParams p, int *fill) {
+    if (p.skip && *p.skip) return;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= p.N) return;
+    if (p.row_sys && p.sys_done[p.row_sys[k]]) return;
+    const int f = fill[k];
+    if (f == 0) return;
+    fill[k] = 0;
+    const int tot = p.Cn[k] + f;
+    p.Cn[k] = tot < p.MC ? tot : p.MC;
+    p.rowstat[k].x += f;
 }
 
 // First pass: table in LDS, 64/GROUP rows per 64-thread block.
@@ -1586,7 +1638,9 @@ int launch_spgemm_mw(const SpParams &p, unsigned tsize, hipStream_t s) {
     return report("k_spgemm_mw");
 }
 
-template <typename T, int W, int BK>
+// (the symmetric x2's marks are set by the caller around the row kernel
+// and the mirror together)
+template <typename T, int W, int BK, bool SYM>
 int launch_spgemm_unord(const SpParams &p, unsigned tsize, hipStream_t s) {
     const size_t lds = (size_t) (tsize + table_pad<64>()) *
                        (sizeof(typename UAcc<T>::type) + sizeof(int));
@@ -1594,23 +1648,23 @@ int launch_spgemm_unord(const SpParams &p, unsigned tsize, hipStream_t s) {
         return fail_msg(2, "LDS table of %zu bytes exceeds 160 KiB", lds);
     }
     if (lds > 64 * 1024)
-        (void) hipFuncSetAttribute((const void *) k_spgemm_unord<T, W, BK>,
+        (void) hipFuncSetAttribute((const void *) k_spgemm_unord<T, W, BK, SYM>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
     if (p.N > 0) {
-        bmlhip_prof_mark_begin(PrecOf<T>::value);
-        hipLaunchKernelGGL((k_spgemm_unord<T, W, BK>), dim3(p.N), dim3(64 * W), lds, s, p, tsize);
-        bmlhip_prof_mark_end();
+        if (!SYM) bmlhip_prof_mark_begin(PrecOf<T>::value);
+        hipLaunchKernelGGL((k_spgemm_unord<T, W, BK, SYM>), dim3(p.N), dim3(64 * W), lds, s, p, tsize);
+        if (!SYM) bmlhip_prof_mark_end();
     }
     return report("k_spgemm_unord");
 }
 
-template <typename T, int BK>
+template <typename T, int BK, bool SYM>
 int launch_spgemm_unord_w(const SpParams &p, unsigned tsize, int waves, hipStream_t s) {
     switch (waves) {
-    case 1: return launch_spgemm_unord<T, 1, BK>(p, tsize, s);
-    case 2: return launch_spgemm_unord<T, 2, BK>(p, tsize, s);
-    case 4: return launch_spgemm_unord<T, 4, BK>(p, tsize, s);
-    case 8: return launch_spgemm_unord<T, 8, BK>(p, tsize, s);
+    case 1: return launch_spgemm_unord<T, 1, BK, SYM>(p, tsize, s);
+    case 2: return launch_spgemm_unord<T, 2, BK, SYM>(p, tsize, s);
+    case 4: return launch_spgemm_unord<T, 4, BK, SYM>(p, tsize, s);
+    case 8: return launch_spgemm_unord<T, 8, BK, SYM>(p, tsize, s);
     default: return fail_msg(2, "unordered engine: waves %d not one of 1/2/4/8", waves);
     }
 }
@@ -1662,6 +1716,7 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
     p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (a->N + 1));
     p.remap = a->remap;
+    p.sym = 0;
     // global-table pass: one table per block, >= 2x the row bound
     long long bound = (long long) a->MA * a->MB;
     if (bound > a->N) bound = a->N;
@@ -1684,10 +1739,22 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     int rc = bmlhip_status_reset();
     if (rc) return rc;
     const unsigned ts = (unsigned) tsize;
-    if (a->engine == BMLHIP_ROW_UNORDERED && a->group == 64) {
+    const bool sym = a->sym && a->kind == 0 && a->engine == BMLHIP_ROW_UNORDERED && a->group == 64;
+    int *fill = nullptr;
+    if (sym) {
+        fill = (int *) bmlhip_scratch_zeroed(BMLHIP_SYMFILL_SLOT, sizeof(int) * (size_t) (a->N + 1));
+        if (!fill) return fail_msg(4, "scratch allocation failed");
+        p.sym = 1;
+        if (a->N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);  // ended after the mirror
         switch (a->bucket) {
-        case 2: rc = launch_spgemm_unord_w<T, 2>(p, ts, a->waves, s); break;
-        case 4: rc = launch_spgemm_unord_w<T, 4>(p, ts, a->waves, s); break;
+        case 2: rc = launch_spgemm_unord_w<T, 2, true>(p, ts, a->waves, s); break;
+        case 4: rc = launch_spgemm_unord_w<T, 4, true>(p, ts, a->waves, s); break;
+        default: return fail_msg(2, "symmetric x2: bucket %d not one of 2/4", a->bucket);
+        }
+    } else if (a->engine == BMLHIP_ROW_UNORDERED && a->group == 64) {
+        switch (a->bucket) {
+        case 2: rc = launch_spgemm_unord_w<T, 2, false>(p, ts, a->waves, s); break;
+        case 4: rc = launch_spgemm_unord_w<T, 4, false>(p, ts, a->waves, s); break;
         default: return fail_msg(2, "unordered engine: bucket %d not one of 2/4", a->bucket);
         }
     } else if (a->group == 64 && a->waves > 1) {
@@ -1712,6 +1779,12 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     hipLaunchKernelGGL(k_spgemm_global<T>, dim3(gblocks), dim3(64), 0, s, p, (unsigned) gsize, gtab,
                        cursor);
     if ((rc = report("k_spgemm_global"))) return rc;
+    if (sym && a->N > 0) {
+        hipLaunchKernelGGL(k_sym_mirror<T>, dim3((a->N + 3) / 4), dim3(256), 0, s, p, fill);
+        hipLaunchKernelGGL(k_sym_finish, dim3((a->N + 255) / 256), dim3(256), 0, s, p, fill);
+        bmlhip_prof_mark_end();
+        if ((rc = report("k_sym_mirror"))) return rc;
+    }
     // kind 1 (AB) checks the distinct count against MC, as the reference does
     hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(a->N)), dim3(256), 0, s,
                        (const int4 *) p.rowstat, a->N, a->MC,
@@ -1897,6 +1970,58 @@ int identity_typed(int N, int M, void *Av, int *Ai, int *An) {
     return report("k_identity");
 }
 
+// Symmetry test: entry (i, j, v) needs (j, i, conj(v)) in row j, bit for bit.
+// One thread per stored entry; mismatches are counted.
+template <typename T>
+__global__ __launch_bounds__(256) void k_check_sym(int N, int M, const T *Av, const int *Ai,
+                                                   const int *An, int *bad) {
+    const size_t t = (size_t) blockIdx.x * 256 + threadIdx.x;
+    if (t >= (size_t) N * M) return;
+    const int i = (int) (t / M), e = (int) (t % M);
+    if (e >= An[i]) return;
+    const int j = Ai[t];
+    if (j < 0 || j >= N) {
+        atomicAdd(bad, 1);
+        return;
+    }
+    const T v = Elem<T>::conj(Av[t]);
+    const int n = An[j];
+    bool ok = false;
+    for (int k = 0; k < n && !ok; k++)
+        if (Ai[(size_t) j * M + k] == i) {
+            union U {
+                T t;
+                int w[sizeof(T) / 4];
+            } x, y;
+            x.t = Av[(size_t) j * M + k];
+            y.t = v;
+            ok = true;
+#pragma unroll
+            for (int q = 0; q < (int) (sizeof(T) / 4); q++) ok = ok & (x.w[q] == y.w[q]);
+        }
+    if (!ok) atomicAdd(bad, 1);
+}
+
+template <typename T>
+int check_sym_typed(int N, int M, const void *Av, const int *Ai, const int *An, int *symmetric) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int *bad = (int *) bmlhip_scratch(BMLHIP_SYMFILL_SLOT + 1, sizeof(int) * 64);
+    if (!bad) return fail_msg(4, "scratch allocation failed");
+    if (hipMemsetAsync(bad, 0, sizeof(int), s) != hipSuccess) return fail_msg(3, "memset failed");
+    const size_t n = (size_t) N * M;
+    if (n > 0)
+        hipLaunchKernelGGL(k_check_sym<T>, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, N, M,
+                           (const T *) Av, Ai, An, bad);
+    int rc = report("k_check_sym");
+    if (rc) return rc;
+    int h = 0;
+    if (hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail_msg(3, "symmetry test: copy failed");
+    *symmetric = h == 0;
+    return 0;
+}
+
 }  // namespace
 
 #define DISPATCH(prec, call_tmpl)                                                  \
@@ -1930,6 +2055,13 @@ int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *skip) {
                        (hipStream_t) bmlhip_stream(), (const int4 *) rs, N, MC, what, prof, skip,
                        bmlhip_status_dev());
     return report("k_reduce_rowstat");
+}
+
+int bmlhip_is_symmetric(int prec, int N, int M, const void *Av, const int *Ai, const int *An,
+                        int *symmetric) {
+#define CALL(T) check_sym_typed<T>(N, M, Av, Ai, An, symmetric)
+    DISPATCH(prec, CALL)
+#undef CALL
 }
 
 int bmlhip_add_dev(int prec, int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,

@@ -18,8 +18,8 @@ struct Runtime {
     int device = -1;
     hipStream_t stream = nullptr;
     bmlhip_status *status = nullptr;
-    void *scratch[8] = {};
-    size_t scratch_bytes[8] = {};
+    void *scratch[BMLHIP_SCRATCH_SLOTS] = {};
+    size_t scratch_bytes[BMLHIP_SCRATCH_SLOTS] = {};
     char msg[512] = {0};
 };
 
@@ -84,7 +84,7 @@ int bmlhip_init(int device) {
 void bmlhip_shutdown(void) {
     if (g.device < 0) return;
     (void) hipStreamSynchronize(g.stream);
-    for (int s = 0; s < 8; s++) {
+    for (int s = 0; s < BMLHIP_SCRATCH_SLOTS; s++) {
         if (g.scratch[s]) (void) hipFree(g.scratch[s]);
         g.scratch[s] = nullptr;
         g.scratch_bytes[s] = 0;
@@ -150,9 +150,15 @@ int bmlhip_sync(void) {
 }
 
 void *bmlhip_scratch(int slot, size_t bytes) {
-    if (slot < 0 || slot >= 8) return nullptr;
+    if (slot < 0 || slot >= BMLHIP_SCRATCH_SLOTS) return nullptr;
     if (bytes == 0) bytes = 256;
     if (g.scratch_bytes[slot] >= bytes) return g.scratch[slot];
+    // a graph capture cannot record an allocation: the eager pass before it
+    // sizes every buffer
+    if (g.capturing) {
+        snprintf(g.msg, sizeof g.msg, "scratch slot %d would grow during graph capture", slot);
+        return nullptr;
+    }
     if (g.scratch[slot]) {
         (void) hipStreamSynchronize(g.stream);
         (void) hipFree(g.scratch[slot]);
@@ -167,6 +173,13 @@ void *bmlhip_scratch(int slot, size_t bytes) {
     }
     g.scratch_bytes[slot] = want;
     return g.scratch[slot];
+}
+
+void *bmlhip_scratch_zeroed(int slot, size_t bytes) {
+    const bool grow = slot >= 0 && slot < BMLHIP_SCRATCH_SLOTS && g.scratch_bytes[slot] < bytes;
+    void *p = bmlhip_scratch(slot, bytes);
+    if (p && grow && hipMemsetAsync(p, 0, g.scratch_bytes[slot], g.stream) != hipSuccess) return nullptr;
+    return p;
 }
 
 bmlhip_status *bmlhip_status_dev(void) { return g.status; }

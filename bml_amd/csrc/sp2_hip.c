@@ -56,7 +56,7 @@ static int stalled(const bml_hip_sp2_options *o, stall_track *t, double norm)
 /* one SP2 iteration on (X, X2), all queued on the library stream */
 static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *X2,
                          bmlhip_sp2_state *d_state, double *d_hist,
-                         const bml_hip_sp2_options *o, double nocc, const char *what)
+                         const bml_hip_sp2_options *o, double nocc, int sym, const char *what)
 {
     bmlhip_spgemm_args a;
     memset(&a, 0, sizeof a);
@@ -82,6 +82,7 @@ static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *
     a.remap = sh.remap;
     a.waves = sh.waves;
     a.engine = sh.engine;
+    a.sym = sym;
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
            bmlh_ok(bmlhip_add_dev(X->matrix_precision, X->N, X->M, X->value, X->index, X->nnz,
@@ -128,6 +129,13 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     r.emin = g[0];
     r.emax = g[1];
     free(g);
+    /* A symmetric (Hermitian) H gives symmetric iterates: on the unordered
+     * engine, X^2 then forms its upper triangle and mirrors it */
+    int sym = 0;
+    if (bml_hip_get_row_engine() == BML_HIP_ROW_ENGINE_UNORDERED &&
+        !bmlh_ok(bmlhip_is_symmetric(H->matrix_precision, H->N, H->M, H->value, H->index,
+                                     H->nnz, &sym), what))
+        return;
     bml_copy_ellpack_hip(H, X);
     if (bmlh_failed())
         return;
@@ -166,7 +174,7 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     stall_track stall = { 0, 0.0, 0.0 };
     for (int it = 0; it < o.maxit; it++) {
         bml_matrix_ellpack_hip_t *cur = single ? Xs : X, *cur2 = single ? X2s : X2;
-        if (!sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, what))
+        if (!sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, sym, what))
             goto out;
         int last_single = single && it + 1 >= o.maxit - o.refine_iters;
         if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {
@@ -233,6 +241,7 @@ typedef struct {
     bmlhip_batch b;
     int *row_sys, *row_off, *done, *branch, *it, *alldone, *stats;
     double *ab, *trx, *trx2, *norm2, *norm, *rowtr, *rownorm, *nocc, *emin, *emax;
+    int sym; /* every H symmetric (Hermitian): symmetric x2 on the unordered engine */
 } batch_dev;
 
 static void batch_free(batch_dev *d)
@@ -315,6 +324,7 @@ static int batch_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t
     a.remap = sh->remap;
     a.waves = sh->waves;
     a.engine = sh->engine;
+    a.sym = d->sym;
     a.batch = &d->b;
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide_batch(&d->b, d->nocc, d->branch, d->stats), what) &&
@@ -422,6 +432,10 @@ void bml_hip_sp2_ellpack_batch(int nsys, bml_matrix_ellpack_hip_t *const *H,
         Xc->aux->stats_valid = 1;
         Xc->aux->max_nnz = hmax + 1;
     }
+    d.sym = 0;
+    if (bml_hip_get_row_engine() == BML_HIP_ROW_ENGINE_UNORDERED &&
+        !bmlh_ok(bmlhip_is_symmetric(prec, N, M, Xc->value, Xc->index, Xc->nnz, &d.sym), what))
+        goto out;
     if (!bmlh_ok(bmlhip_gershgorin_batch(prec, N, M, Xc->value, Xc->index, Xc->nnz, &d.b,
                                          d.emin, d.emax), what) ||
         !bmlh_ok(bmlhip_normalize_batch(prec, N, M, Xc->value, Xc->index, Xc->nnz, &d.b,

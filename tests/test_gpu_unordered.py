@@ -195,3 +195,75 @@ def test_batch_unordered_graph_matches_eager(hip):
     for a, b, da, db in zip(outs[False][0], outs[True][0], outs[False][1], outs[True][1]):
         assert a.iterations == b.iterations
         assert np.abs(da - db).max() <= 1e-9
+
+
+def _symmetric_ell(N, fill, seed, dtype):
+    """Random symmetric (Hermitian) ELLPACK matrix with ~2 fill entries per
+    row and the diagonal."""
+    rng = np.random.default_rng(seed)
+    D = np.zeros((N, N), dtype=dtype)
+    for i in range(N):
+        cols = rng.choice(N, size=fill, replace=False)
+        v = rng.uniform(-1, 1, fill)
+        if np.iscomplexobj(D):
+            v = v + 1j * rng.uniform(-1, 1, fill)
+        D[i, cols] = v
+    D = D + D.conj().T
+    D[np.arange(N), np.arange(N)] = rng.uniform(0.5, 1.0, N)
+    from bml_amd.ell import Ell
+    M = int((np.abs(D) > 0).sum(axis=1).max())
+    return Ell.from_dense(D, M)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("table", [None, 320])
+def test_x2_symmetric_path_matches_oracle(unordered, monkeypatch, dtype, table):
+    """The symmetric X^2 (upper triangle formed, lower mirrored, products
+    packed into full chunks) against the oracle; the small table sends rows
+    through the global-table pass, which then forms the upper part too."""
+    hip = unordered
+    A = _symmetric_ell(700, 60, 5, dtype)
+    ref, rtr = orc.x2(A, 700, 0.05)
+    monkeypatch.setenv("BML_HIP_ASSUME_SYMMETRIC", "1")
+    if table:
+        monkeypatch.setenv("BML_HIP_TABLE", str(table))
+    X2 = hip.zero_matrix(A.precision, A.N, 700)
+    tr = hip.multiply_x2(hip.from_ell(A), X2, 0.05)
+    got = X2.get()
+    assert_ell_close(got, ref, vtol(dtype), 0.05, f"symmetric {np.dtype(dtype)} table {table}")
+    assert rel_close(tr[0], rtr[0], vtol(dtype)) and rel_close(tr[1], rtr[1], vtol(dtype))
+    # the result is exactly symmetric (Hermitian)
+    D = got.to_dense()
+    assert np.array_equal(D, D.conj().T)
+
+
+def test_x2_symmetric_path_m_overflow(unordered, monkeypatch):
+    hip = unordered
+    A = _symmetric_ell(400, 60, 7, np.float64)
+    monkeypatch.setenv("BML_HIP_ASSUME_SYMMETRIC", "1")
+    X2 = hip.zero_matrix(A.precision, A.N, 64)
+    with pytest.raises(hip.BmlError) as e:
+        hip.multiply_x2(hip.from_ell(A), X2, 0.0)
+    assert e.value.code == hip.ERR_M_OVERFLOW
+
+
+def test_sp2_nonsymmetric_h_uses_full_product(hip):
+    """An H that is not exactly symmetric keeps the full X^2 on the unordered
+    engine: the iterate is not mirrored into an exactly symmetric matrix, and
+    the solve stays near the exact engine's (the non-normal iteration
+    amplifies rounding, so only to 1e-5)."""
+    H = G.lattice_hamiltonian(8, 8, 8, M=16, hopping=-0.1, disorder=0.01)
+    e = int(np.nonzero(H.index[3, :H.nnz[3]] != 3)[0][0])
+    H.value[3, e] += 1e-9  # break the symmetry of one off-diagonal entry
+    dH = hip.from_ell(H)
+    out = {}
+    for eng in (hip.ROW_ENGINE_EXACT, hip.ROW_ENGINE_UNORDERED):
+        hip.set_row_engine(eng)
+        try:
+            X = hip.zero_matrix(H.precision, H.N, H.N)
+            r, _ = hip.sp2(dH, X, H.N / 2, threshold=1e-7, tol=1e-9, maxit=40)
+        finally:
+            hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+        out[eng] = X.get().to_dense()
+    assert np.abs(out[0] - out[1]).max() <= 1e-5
+    assert not np.array_equal(out[1], out[1].T)
