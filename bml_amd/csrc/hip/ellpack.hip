@@ -1244,12 +1244,14 @@ struct AddParams {
     int4 *rowstat;  // [N]: (kept, 0, 0, 0), kept = -1 for a skipped row
 };
 
-template <typename T>
-__device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int *ord, int tbits) {
+// tsize: any table size; a column's home slot is its multiplicative hash
+// scaled to the table, probing is linear with wrap-around
+template <typename T, typename ORD>
+__device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, ORD *ord,
+                        unsigned tsize) {
     using E = Elem<T>;
     const int lane = lane_id();
     const uint64_t ltmask = (1ull << lane) - 1ull;
-    const unsigned tmask = (1u << tbits) - 1u;
     T *Av = (T *) p.Av;
     const T *Bv = (const T *) p.Bv;
     const size_t rA = (size_t) i * p.MA, rB = (size_t) i * p.MB;
@@ -1270,7 +1272,7 @@ __device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int 
             if (act) {
                 col = part ? p.Bi[rB + kp] : p.Ai[rA + kp];
                 v = part ? Bv[rB + kp] : Av[rA + kp];
-                unsigned h = hash_slot(col, tbits);
+                unsigned h = __umulhi((unsigned) col * 0x9E3779B1u, tsize);
                 while (true) {
                     const int kk = keys[h];
                     if (kk == col) break;
@@ -1282,12 +1284,12 @@ __device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int 
                         }
                         if (old == col) break;
                     }
-                    h = (h + 1u) & tmask;
+                    h = h + 1u == tsize ? 0u : h + 1u;
                 }
                 slot = (int) h;
             }
             const uint64_t nm = ballot64(isnew);
-            if (isnew) ord[count + popc64(nm & ltmask)] = slot;
+            if (isnew) ord[count + popc64(nm & ltmask)] = (ORD) slot;
             count += popc64(nm);
             if (act) {
                 xs[slot] = E::axpy(xs[slot], s, v);
@@ -1333,15 +1335,14 @@ __device__ void add_row(const AddParams &p, int i, int *keys, T *xs, T *ys, int 
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_add_lds(AddParams p, int tbits) {
+__global__ __launch_bounds__(64) void k_add_lds(AddParams p, unsigned tsize) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (p.skip && *p.skip) return;
-    const int tsize = 1 << tbits;
     T *xs = (T *) smem;
     T *ys = xs + tsize;
     int *keys = (int *) (ys + tsize);
-    int *ord = keys + tsize;
-    for (int s = threadIdx.x; s < tsize; s += 64) {
+    uint16_t *ord = (uint16_t *) (keys + tsize);
+    for (unsigned s = threadIdx.x; s < tsize; s += 64) {
         keys[s] = EMPTY;
         xs[s] = Elem<T>::zero();
         ys[s] = Elem<T>::zero();
@@ -1352,7 +1353,7 @@ __global__ __launch_bounds__(64) void k_add_lds(AddParams p, int tbits) {
         if (threadIdx.x == 0) p.rowstat[i] = make_int4(-1, 0, 0, 0);
         return;
     }
-    add_row<T>(p, i, keys, xs, ys, ord, tbits);
+    add_row<T, uint16_t>(p, i, keys, xs, ys, ord, tsize);
 }
 
 template <typename T>
@@ -1375,7 +1376,7 @@ __global__ __launch_bounds__(64) void k_add_global(AddParams p, int tbits, unsig
             if (threadIdx.x == 0) p.rowstat[i] = make_int4(-1, 0, 0, 0);
             continue;
         }
-        add_row<T>(p, i, keys, xs, ys, ord, tbits);
+        add_row<T, int>(p, i, keys, xs, ys, ord, (unsigned) tsize);
         __syncthreads();
     }
 }
@@ -1839,12 +1840,15 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
     if (!p.rownorm || !p.rowstat) return 4;
     int rc = bmlhip_status_reset();
     if (rc) return rc;
-    // a row's union has at most MA + MB columns; table at most 3/4 full
+    // a row's union has at most MA + MB columns: the LDS table keeps it at
+    // most 4/5 full (multiple of 64 slots; first-touch list in 16 bits)
     const int tbits = ceil_log2((long long) (MA + MB) * 4 / 3 + 64);
     const size_t lds = ((size_t) 1 << tbits) * (2 * sizeof(T) + 2 * sizeof(int));
+    const unsigned tl = (unsigned) (((long long) (MA + MB) * 5 / 4 + 64 + 63) & ~63ll);
+    const size_t lds_l = (size_t) tl * (2 * sizeof(T) + sizeof(int) + sizeof(uint16_t));
     if (N > 0) {
-        if (lds <= 64 * 1024) {
-            hipLaunchKernelGGL(k_add_lds<T>, dim3(N), dim3(64), lds, s, p, tbits);
+        if (lds_l <= 64 * 1024 && tl < 65536) {
+            hipLaunchKernelGGL(k_add_lds<T>, dim3(N), dim3(64), lds_l, s, p, tl);
             if ((rc = report("k_add_lds"))) return rc;
         } else {
             const int blocks = 256;
