@@ -153,6 +153,19 @@ __device__ __forceinline__ void lds_add(cf64 *p, cf64 v) {
     lds_add(&p->y, v.y);
 }
 
+// XCD-aware block order.  The dispatcher deals the blocks of a grid to the 8
+// XCDs round-robin (block b runs on XCD b % 8) and each XCD runs its share
+// in order.  xcd_block(b, n) renumbers the n blocks so that XCD x takes the
+// x-th contiguous range of about n/8 blocks: blocks resident on one XCD at a
+// time are neighbours, and the operand rows they share stay in its L2.  A
+// bijection on [0, n).
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int per = (n + 7) / 8;
+    const int xcd = b % 8, k = b / 8;
+    const int full = n % 8 == 0 ? 8 : n % 8;  // XCDs that take `per` blocks
+    return xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+}
+
 // 64-lane wavefront helpers
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return (int) __lane_id(); }

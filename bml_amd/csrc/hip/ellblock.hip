@@ -27,6 +27,7 @@
 #include "devtypes.h"
 
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -63,6 +64,7 @@ struct EbParams {
     int4 *rowstat;   // [NB] (kept, touched, block pairs, nnzb of A's row)
     bmlhip_status *st;
     const int *skip;
+    int remap;  // numeric pass: XCD-aware block-row order (xcd_block)
 };
 
 __device__ __forceinline__ unsigned eb_hash(int kb, unsigned mask) {
@@ -438,14 +440,27 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // ---------------------------------------------------------------------------
 // numeric pass
+//
+// Phase 1: the wavefronts of a block row claim slots one at a time from an
+// LDS counter (no barrier between slots: a slot's pair count ranges from 1 to
+// the row's nnzb, and a round of equal shares left a third of the waves idle
+// on a lattice).  Each forms its slot's tile, takes the max |element| and
+// stores a kept tile at row position s, which is free as long as the row
+// does not overflow (touched blocks <= MC).  Phase 2 compacts the kept tiles
+// to their ranks in slot order: position(s) <= s, so moving the tiles in
+// increasing s, a group of NUM_WAVES at a time with a barrier between the
+// group's reads and its writes, never overwrites a tile not yet moved.
+// Every tile is formed by one wavefront in a fixed order, so the result does
+// not depend on which wavefront claims which slot.
 template <typename T, int ENGINE, int BT>
 __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
     using E = Elem<T>;
     using Acc = typename AccOf<T, ENGINE, BT>::type;
-    __shared__ int keep_s[NUM_WAVES];
+    extern __shared__ unsigned kbits[];  // kept flag of each slot s < min(maxs, MC)
+    __shared__ int next_s, kept_s;
     __shared__ T diag_s[NUM_WAVES][BT];
     if (p.skip && *p.skip) return;
-    const int ib = blockIdx.x;
+    const int ib = p.remap ? xcd_block(blockIdx.x, gridDim.x) : (int) blockIdx.x;
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int count = p.cnt[ib];
     const int bi = p.bsize[ib];
@@ -455,6 +470,13 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
     constexpr size_t B2 = (size_t) BT * BT;
     const int *off = p.soff + (size_t) ib * (p.maxs + 1);
     const int *pr = p.pairs + (size_t) ib * p.maxp;
+    const int *skb = p.skb + (size_t) ib * p.maxs;
+    const int nbits = count < p.MC ? count : p.MC;  // slots with a row position
+    for (int q = threadIdx.x; q < (nbits + 31) / 32; q += 64 * NUM_WAVES) kbits[q] = 0u;
+    if (threadIdx.x == 0) {
+        next_s = 0;
+        kept_s = 0;
+    }
     T tx = E::zero(), tx2 = E::zero();
     if (p.kind == 0 && threadIdx.x == 0) {
         // tr X: the diagonal of every diagonal block met in the expand phase
@@ -464,79 +486,124 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
                 for (int kk = 0; kk < bi; kk++) tx = E::add(tx, Av[sa * B2 + kk * BT + kk]);
         }
     }
-    int base = 0;
-    for (int r0 = 0; r0 < count; r0 += NUM_WAVES) {
-        const int s = r0 + w;
-        bool keep = false;
-        int kb = -1;
+    __syncthreads();
+    while (true) {
+        int s = 0;
+        if (lane == 0) s = atomicAdd(&next_s, 1);
+        s = __shfl(s, 0, 64);
+        if (s >= count) break;
+        const int kb = skb[s];
+        const int bj = p.bsize[kb];
         Acc acc;
         acc.zero();
-        if (s < count) {
-            kb = p.skb[(size_t) ib * p.maxs + s];
-            const int bj = p.bsize[kb];
-            if constexpr (Acc::kPipelined) {
-                // up to 64 pairs at a time: lane q decodes pair t0 + q (the
-                // tile offsets), then the wavefront runs the pairs with the
-                // next pair's tiles loading while the current pair's MFMAs run
-                using Frag = typename Acc::Frag;
-                for (int t0 = off[s]; t0 < off[s + 1]; t0 += 64) {
-                    const int np = off[s + 1] - t0 < 64 ? off[s + 1] - t0 : 64;
-                    size_t la = 0, lb = 0;
-                    if (lane < np) {
-                        const unsigned pv = (unsigned) pr[t0 + lane];
-                        const size_t sa = (size_t) ib * p.MA + (pv >> 16);
-                        la = sa * B2;
-                        lb = ((size_t) p.Ai[sa] * p.MB + (pv & 0xffffu)) * B2;
-                    }
-                    Frag f0, f1;
-                    Acc::load(Av + lane_get(la, 0), Bv + lane_get(lb, 0), f0);
-                    for (int q = 0; q < np; q += 2) {
-                        if (q + 1 < np)
-                            Acc::load(Av + lane_get(la, q + 1), Bv + lane_get(lb, q + 1), f1);
-                        acc.mma(f0);
-                        if (q + 1 < np) {
-                            if (q + 2 < np)
-                                Acc::load(Av + lane_get(la, q + 2), Bv + lane_get(lb, q + 2), f0);
-                            acc.mma(f1);
-                        }
-                    }
+        if constexpr (Acc::kPipelined) {
+            // up to 64 pairs at a time: lane q decodes pair t0 + q (the tile
+            // offsets), then the wavefront runs the pairs with the next
+            // pair's tiles loading while the current pair's MFMAs run
+            using Frag = typename Acc::Frag;
+            for (int t0 = off[s]; t0 < off[s + 1]; t0 += 64) {
+                const int np = off[s + 1] - t0 < 64 ? off[s + 1] - t0 : 64;
+                size_t la = 0, lb = 0;
+                if (lane < np) {
+                    const unsigned pv = (unsigned) pr[t0 + lane];
+                    const size_t sa = (size_t) ib * p.MA + (pv >> 16);
+                    la = sa * B2;
+                    lb = ((size_t) p.Ai[sa] * p.MB + (pv & 0xffffu)) * B2;
                 }
-            } else {
-                for (int t = off[s]; t < off[s + 1]; t++) {
-                    const int jp = (int) ((unsigned) pr[t] >> 16), kp = pr[t] & 0xffff;
-                    const size_t sa = (size_t) ib * p.MA + jp;
-                    const int jb = p.Ai[sa];
-                    acc.mac(Av + sa * B2, Bv + ((size_t) jb * p.MB + kp) * B2, bi, p.bsize[jb], bj);
+                Frag f0, f1;
+                Acc::load(Av + lane_get(la, 0), Bv + lane_get(lb, 0), f0);
+                for (int q = 0; q < np; q += 2) {
+                    if (q + 1 < np)
+                        Acc::load(Av + lane_get(la, q + 1), Bv + lane_get(lb, q + 1), f1);
+                    acc.mma(f0);
+                    if (q + 1 < np) {
+                        if (q + 2 < np)
+                            Acc::load(Av + lane_get(la, q + 2), Bv + lane_get(lb, q + 2), f0);
+                        acc.mma(f1);
+                    }
                 }
             }
-            const double nrm = wave_max(acc.maxmag());
-            keep = kb == ib || nrm > p.thr;
-            if (p.kind == 0 && kb == ib) acc.diag(diag_s[w]);
+        } else {
+            for (int t = off[s]; t < off[s + 1]; t++) {
+                const int jp = (int) ((unsigned) pr[t] >> 16), kp = pr[t] & 0xffff;
+                const size_t sa = (size_t) ib * p.MA + jp;
+                const int jb = p.Ai[sa];
+                acc.mac(Av + sa * B2, Bv + ((size_t) jb * p.MB + kp) * B2, bi, p.bsize[jb], bj);
+            }
         }
-        if (lane == 0) keep_s[w] = keep ? 1 : 0;
-        __syncthreads();
-        int pos = base, tot = 0;
-        for (int v = 0; v < NUM_WAVES; v++) {
-            pos += v < w ? keep_s[v] : 0;
-            tot += keep_s[v];
+        const double nrm = wave_max(acc.maxmag());
+        const bool keep = kb == ib || nrm > p.thr;
+        if (p.kind == 0 && kb == ib) {
+            // tr X^2: the diagonal block's diagonal, summed in order by lane 0
+            acc.diag(diag_s[w]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == 0)
+                for (int kk = 0; kk < bi; kk++) tx2 = E::add(tx2, diag_s[w][kk]);
         }
-        if (p.kind == 0 && kb == ib && s < count && lane == 0)
-            for (int kk = 0; kk < bi; kk++) tx2 = E::add(tx2, diag_s[w][kk]);
-        if (keep && pos < p.MC) {
-            const size_t sc = (size_t) ib * p.MC + pos;
-            acc.store(Cv + sc * B2);
-            if (lane == 0) p.Ci[sc] = kb;
+        if (keep) {
+            if (s < p.MC) {
+                const size_t sc = (size_t) ib * p.MC + s;
+                acc.store(Cv + sc * B2);
+                if (lane == 0) {
+                    p.Ci[sc] = kb;
+                    atomicOr(&kbits[s >> 5], 1u << (s & 31));
+                }
+            }
+            if (lane == 0) atomicAdd(&kept_s, 1);
         }
-        base += tot;
-        __syncthreads();
     }
-    // tx2 lives in the lane that owned the diagonal block: combine
+    __syncthreads();
+    // phase 2: move each kept tile to its rank among the kept slots
+    constexpr int Q = (int) ((B2 + 63) / 64);
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    int running = 0;
+    for (int c0 = 0; c0 < nbits; c0 += 64) {
+        const int sl = c0 + lane;
+        const bool kp = sl < nbits && ((kbits[sl >> 5] >> (sl & 31)) & 1u);
+        const uint64_t km = ballot64(kp);
+        const int pos = running + popc64(km & ltmask);
+        running += popc64(km);
+        const bool mv = kp && pos != sl;
+        const uint64_t mm = ballot64(mv);
+        const int rank = popc64(mm & ltmask);
+        const int nmv = popc64(mm);
+        for (int g0 = 0; g0 < nmv; g0 += NUM_WAVES) {
+            const uint64_t mine = ballot64(mv && rank == g0 + w);
+            T buf[Q];
+            int src = -1, dst = -1;
+            if (mine) {
+                const int l = __ffsll((unsigned long long) mine) - 1;
+                src = c0 + l;
+                dst = __shfl(pos, l, 64);
+                const T *S = Cv + ((size_t) ib * p.MC + src) * B2;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const size_t e = (size_t) lane + 64 * q;
+                    if (e < B2) buf[q] = S[e];
+                }
+            }
+            __syncthreads();
+            if (mine) {
+                T *D = Cv + ((size_t) ib * p.MC + dst) * B2;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const size_t e = (size_t) lane + 64 * q;
+                    if (e < B2) D[e] = buf[q];
+                }
+                if (lane == 0) p.Ci[(size_t) ib * p.MC + dst] = skb[src];
+            }
+        }
+    }
+    // tx2 lives in lane 0 of the wavefront that formed the diagonal block
     __shared__ double tr2_s[NUM_WAVES];
     if (lane == 0) tr2_s[w] = E::re(tx2);
     __syncthreads();
     if (threadIdx.x == 0) {
-        p.Cn[ib] = base < p.MC ? base : p.MC;
-        p.rowstat[ib] = make_int4(base, count, off[count], p.An[ib]);
+        const int kept = kept_s;
+        p.Cn[ib] = kept < p.MC ? kept : p.MC;
+        p.rowstat[ib] = make_int4(kept, count, off[count], p.An[ib]);
         if (p.kind == 0) {
             double t2 = 0.0;
             for (int v = 0; v < NUM_WAVES; v++) t2 += tr2_s[v];
@@ -865,7 +932,9 @@ int launch_numeric(const EbParams &p, hipStream_t s) {
                                                 (ENGINE == 1 && std::is_same<T, double>::value)))) {
         return fail_msg(2, "no MFMA engine for this element type and tile");
     } else {
-        hipLaunchKernelGGL((k_eb_numeric<T, ENGINE, BT>), dim3(p.NB), dim3(64 * NUM_WAVES), 0, s,
+        const int nbits = p.maxs < p.MC ? p.maxs : p.MC;
+        const size_t lds = sizeof(unsigned) * (size_t) ((nbits + 31) / 32 + 1);
+        hipLaunchKernelGGL((k_eb_numeric<T, ENGINE, BT>), dim3(p.NB), dim3(64 * NUM_WAVES), lds, s,
                            p);
         return report("k_eb_numeric");
     }
@@ -898,6 +967,12 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     p.thr = a->threshold;
     p.skip = a->d_skip;
     p.st = bmlhip_status_dev();
+    {
+        /* contiguous block rows per XCD: 1.68 -> 1.60 ms for the FP64 MFMA
+         * X^2 of the 16x16x8-site, 32-orbital lattice */
+        const char *e = getenv("BML_HIP_EB_REMAP");
+        p.remap = e ? atoi(e) : 1;
+    }
     const long long maxp = (long long) a->MA * a->MB;
     const long long maxs = maxp < a->NB ? maxp : a->NB;
     int tbits = 6;

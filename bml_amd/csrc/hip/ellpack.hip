@@ -789,7 +789,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
         ovf_s = 0;
     }
     __syncthreads();
-    const int row = blockIdx.x;
+    const int row = p.remap ? xcd_block(blockIdx.x, gridDim.x) : (int) blockIdx.x;
     if (row >= p.N) return;
     if (p.row_sys && p.sys_done[p.row_sys[row]]) {  // uniform
         if (threadIdx.x == 0) p.rowstat[row] = make_int4(-1, 0, 0, 0);
@@ -1001,18 +1001,8 @@ __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
         vals[s] = Elem<T>::zero();
     }
     __syncthreads();
-    // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8);
-    // with p.remap each XCD gets a contiguous range of rows, so the rows
-    // resident on one XCD at a time are neighbours and share the rows of B
-    // in its L2.
-    int vb = blockIdx.x;
-    if (p.remap) {
-        const int nblk = gridDim.x;
-        const int per_xcd = (nblk + 7) / 8;
-        const int xcd = blockIdx.x % 8, k = blockIdx.x / 8;
-        const int full = nblk % 8 == 0 ? 8 : nblk % 8;  // XCDs that get per_xcd blocks
-        vb = xcd < full ? xcd * per_xcd + k : full * per_xcd + (xcd - full) * (per_xcd - 1) + k;
-    }
+    // with p.remap each XCD takes a contiguous range of rows
+    const int vb = p.remap ? xcd_block(blockIdx.x, gridDim.x) : (int) blockIdx.x;
     const int g = threadIdx.x / GROUP;
     const int row = vb * R + g;
     const int i = (row < p.N && !(p.row_sys && p.sys_done[p.row_sys[row]])) ? row : -1;
