@@ -236,8 +236,14 @@ typedef struct bmlhip_eb_args {
     double threshold;
     int engine;         /* 0 exact, 1 MFMA, 2 MFMA with bf16 inputs */
     const int *d_skip;  /* if not NULL and *d_skip != 0 the kernels do nothing */
+    int sym;            /* x2 on an MFMA engine of an exactly symmetric (Hermitian)
+                         * X: form the blocks kb >= ib, mirror the others */
 } bmlhip_eb_args;
 int bmlhip_eb_product(const bmlhip_eb_args *a);
+/* *symmetric = 1 iff every stored block (ib, jb) has a stored (jb, ib) whose
+ * tile is its conjugate transpose, bit for bit */
+int bmlhip_eb_is_symmetric(int prec, int NB, int bt, int MB, const void *d_Av, const int *d_Ai,
+                           const int *d_An, int *symmetric);
 /* A <- alpha*A + beta*B blockwise; mode 2 adds |A - B|^2 over the union
  * into scalars[2] (alpha, beta from d_ab when not NULL) */
 int bmlhip_eb_add(int prec, int NB, int bt, const int *d_bsize, int MA, void *d_Av,

@@ -476,6 +476,12 @@ static int run_eb_product(bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_
     a.d_Cn = C->nnzb;
     a.threshold = threshold;
     a.engine = bmlh_block_engine_for(A);
+    /* BML_HIP_ASSUME_SYMMETRIC=1: the caller asserts X is symmetric
+     * (Hermitian) and x2 on an MFMA engine forms the blocks kb >= ib and
+     * mirrors them, as the SP2 driver does for a symmetric H (a test hook for
+     * that path) */
+    const char *e = getenv("BML_HIP_ASSUME_SYMMETRIC");
+    a.sym = kind == 0 && e && atoi(e) != 0;
     return bmlh_ok(bmlhip_eb_product(&a), what);
 }
 

@@ -65,6 +65,8 @@ struct EbParams {
     bmlhip_status *st;
     const int *skip;
     int remap;  // numeric pass: XCD-aware block-row order (xcd_block)
+    int sym;    // x2 of an exactly symmetric X: form the slots kb >= ib only
+    int *kflag; // [NB*maxs] sym: kept flag of every slot, set by its row or its mirror
 };
 
 __device__ __forceinline__ unsigned eb_hash(int kb, unsigned mask) {
@@ -132,7 +134,9 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
             }
             count += popc64(nm);
             __syncthreads();  // new slot numbers visible to every lane
-            if (act) fill[sval[h]] += 1;  // the chunk's slots are distinct
+            // the chunk's slots are distinct; with p.sym the slots kb < ib
+            // get no pairs (their tiles are the mirrors of row kb's)
+            if (act && (!p.sym || kb >= ib)) fill[sval[h]] += 1;
             __syncthreads();
         }
     }
@@ -162,8 +166,8 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
         const int nb = p.Bn[jb];
         for (int k0 = 0; k0 < nb; k0 += 64) {
             const int kp = k0 + lane;
-            if (kp < nb) {
-                const int kb = p.Bi[(size_t) jb * p.MB + kp];
+            const int kb = kp < nb ? p.Bi[(size_t) jb * p.MB + kp] : -1;
+            if (kp < nb && (!p.sym || kb >= ib)) {
                 unsigned h = eb_hash(kb, mask);
                 while (keys[h] != kb) h = (h + 1u) & mask;
                 const int s = sval[h];
@@ -173,6 +177,8 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
             __syncthreads();
         }
     }
+    if (p.sym)
+        for (int q = lane; q < count; q += 64) p.kflag[(size_t) ib * p.maxs + q] = 0;
     // the numeric pass records the row's counts (touched > MC is an overflow)
     if (lane == 0) p.cnt[ib] = count;
 }
@@ -215,12 +221,13 @@ struct ExactAcc {
         }
         return m;
     }
-    __device__ void store(T *C) const {
+    // the tile into C, rows ld elements apart
+    __device__ void store(T *C, int ld = BT) const {
         const int lane = lane_id();
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             const int e = lane + 64 * q;
-            if (e < BT * BT) C[e] = a[q];
+            if (e < BT * BT) C[(e / BT) * ld + e % BT] = a[q];
         }
     }
     // diagonal element kk held by this lane (in diag[kk]) for kk < BT
@@ -283,10 +290,10 @@ struct MfmaAccF32 {
         return (double) m;
     }
     __device__ static int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
-    __device__ void store(float *C) const {
+    __device__ void store(float *C, int ld = 32) const {
         const int lane = lane_id();
 #pragma unroll
-        for (int v = 0; v < 16; v++) C[row(v, lane) * 32 + (lane & 31)] = a[v];
+        for (int v = 0; v < 16; v++) C[row(v, lane) * ld + (lane & 31)] = a[v];
     }
     __device__ void diag(float *dg) const {
         const int lane = lane_id();
@@ -397,7 +404,7 @@ struct MfmaAccF64 {
                 for (int v = 0; v < 4; v++) m = fmax(m, fabs(a[ti][tj][v]));
         return m;
     }
-    __device__ void store(double *C) const {
+    __device__ void store(double *C, int ld = 32) const {
         const int lane = lane_id();
         const int c = lane & 15, g = lane >> 4;
 #pragma unroll
@@ -405,7 +412,7 @@ struct MfmaAccF64 {
 #pragma unroll
             for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) C[(ti * 16 + g + 4 * v) * 32 + tj * 16 + c] = a[ti][tj][v];
+                for (int v = 0; v < 4; v++) C[(ti * 16 + g + 4 * v) * ld + tj * 16 + c] = a[ti][tj][v];
     }
     __device__ void diag(double *dg) const {
         const int lane = lane_id();
@@ -446,17 +453,90 @@ __device__ __forceinline__ double wave_max(double v) {
 // the row's nnzb, and a round of equal shares left a third of the waves idle
 // on a lattice).  Each forms its slot's tile, takes the max |element| and
 // stores a kept tile at row position s, which is free as long as the row
-// does not overflow (touched blocks <= MC).  Phase 2 compacts the kept tiles
-// to their ranks in slot order: position(s) <= s, so moving the tiles in
-// increasing s, a group of NUM_WAVES at a time with a barrier between the
-// group's reads and its writes, never overwrites a tile not yet moved.
-// Every tile is formed by one wavefront in a fixed order, so the result does
-// not depend on which wavefront claims which slot.
+// does not overflow (touched blocks <= MC).  Phase 2 (compact_kept) moves
+// the kept tiles to their ranks in slot order.  Every tile is formed by one
+// wavefront in a fixed order, so the result does not depend on which
+// wavefront claims which slot.
+//
+// With p.sym (x2 of an exactly symmetric X, so X^2 is symmetric): row ib
+// forms only its slots kb >= ib.  A tile is staged in LDS and stored twice:
+// at (ib, s) and, transposed, at (kb, s') where s' is the slot of ib in row
+// kb; the kept flag goes to both slots.  The diagonal tile takes its upper
+// triangle for both halves, so X^2 is exactly symmetric.  The compaction
+// then runs in k_eb_compact, once every row's mirrors have landed.
+
+// Move each kept slot's tile (flags in kbits, slots [0, nbits)) of row ib to
+// its rank among the kept slots.  rank(s) <= s, so moving the tiles in
+// increasing s, NUM_WAVES at a time with a barrier between a group's reads
+// and its writes, never overwrites a tile not yet moved.  Called by all
+// NUM_WAVES wavefronts of the block.
+template <typename T, int BT>
+__device__ void compact_kept(const EbParams &p, int ib, int nbits, const unsigned *kbits) {
+    constexpr size_t B2 = (size_t) BT * BT;
+    constexpr int Q = (int) ((B2 + 63) / 64);
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    T *Cv = (T *) p.Cv;
+    const int *skb = p.skb + (size_t) ib * p.maxs;
+    int running = 0;
+    for (int c0 = 0; c0 < nbits; c0 += 64) {
+        const int sl = c0 + lane;
+        const bool kp = sl < nbits && ((kbits[sl >> 5] >> (sl & 31)) & 1u);
+        const uint64_t km = ballot64(kp);
+        const int pos = running + popc64(km & ltmask);
+        running += popc64(km);
+        const bool mv = kp && pos != sl;
+        const uint64_t mm = ballot64(mv);
+        const int rank = popc64(mm & ltmask);
+        const int nmv = popc64(mm);
+        for (int g0 = 0; g0 < nmv; g0 += NUM_WAVES) {
+            const uint64_t mine = ballot64(mv && rank == g0 + w);
+            T buf[Q];
+            int src = -1, dst = -1;
+            if (mine) {
+                const int l = __ffsll((unsigned long long) mine) - 1;
+                src = c0 + l;
+                dst = __shfl(pos, l, 64);
+                const T *S = Cv + ((size_t) ib * p.MC + src) * B2;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const size_t e = (size_t) lane + 64 * q;
+                    if (e < B2) buf[q] = S[e];
+                }
+            }
+            __syncthreads();
+            if (mine) {
+                T *D = Cv + ((size_t) ib * p.MC + dst) * B2;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const size_t e = (size_t) lane + 64 * q;
+                    if (e < B2) D[e] = buf[q];
+                }
+                if (lane == 0) p.Ci[(size_t) ib * p.MC + dst] = skb[src];
+            }
+        }
+    }
+}
+
+// slot of block column col among the count slots of block row r (-1: none)
+__device__ __forceinline__ int find_slot(const EbParams &p, int r, int col) {
+    const int n = p.cnt[r];
+    const int *skb = p.skb + (size_t) r * p.maxs;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const int q = c0 + lane_id();
+        const uint64_t m = ballot64(q < n && skb[q] == col);
+        if (m) return c0 + __ffsll((unsigned long long) m) - 1;
+    }
+    return -1;
+}
+
 template <typename T, int ENGINE, int BT>
 __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
     using E = Elem<T>;
     using Acc = typename AccOf<T, ENGINE, BT>::type;
-    extern __shared__ unsigned kbits[];  // kept flag of each slot s < min(maxs, MC)
+    // dynamic LDS: the kept flag of each slot s < min(maxs, MC), then (sym)
+    // one staging tile per wavefront, rows BT + 1 elements apart
+    extern __shared__ __align__(16) unsigned kbits[];
     __shared__ int next_s, kept_s;
     __shared__ T diag_s[NUM_WAVES][BT];
     if (p.skip && *p.skip) return;
@@ -468,10 +548,13 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
     const T *Bv = (const T *) p.Bv;
     T *Cv = (T *) p.Cv;
     constexpr size_t B2 = (size_t) BT * BT;
+    constexpr int LD = BT + 1;
     const int *off = p.soff + (size_t) ib * (p.maxs + 1);
     const int *pr = p.pairs + (size_t) ib * p.maxp;
     const int *skb = p.skb + (size_t) ib * p.maxs;
     const int nbits = count < p.MC ? count : p.MC;  // slots with a row position
+    const int mbits = p.maxs < p.MC ? p.maxs : p.MC;
+    T *stg = (T *) (kbits + ((mbits + 31) / 32 + 4) / 4 * 4) + (size_t) w * BT * LD;
     for (int q = threadIdx.x; q < (nbits + 31) / 32; q += 64 * NUM_WAVES) kbits[q] = 0u;
     if (threadIdx.x == 0) {
         next_s = 0;
@@ -493,6 +576,7 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
         s = __shfl(s, 0, 64);
         if (s >= count) break;
         const int kb = skb[s];
+        if (p.sym && kb < ib) continue;  // the mirror of row kb's slot
         const int bj = p.bsize[kb];
         Acc acc;
         acc.zero();
@@ -542,74 +626,111 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
             if (lane == 0)
                 for (int kk = 0; kk < bi; kk++) tx2 = E::add(tx2, diag_s[w][kk]);
         }
-        if (keep) {
+        if (!p.sym) {
+            if (keep) {
+                if (s < p.MC) {
+                    const size_t sc = (size_t) ib * p.MC + s;
+                    acc.store(Cv + sc * B2);
+                    if (lane == 0) {
+                        p.Ci[sc] = kb;
+                        atomicOr(&kbits[s >> 5], 1u << (s & 31));
+                    }
+                }
+                if (lane == 0) atomicAdd(&kept_s, 1);
+            }
+            continue;
+        }
+        // sym: the tile at (ib, s) and its transpose at (kb, s')
+        const int sp = kb > ib ? find_slot(p, kb, ib) : -1;
+        if (lane == 0) {
+            p.kflag[(size_t) ib * p.maxs + s] = keep;
+            if (sp >= 0) p.kflag[(size_t) kb * p.maxs + sp] = keep;
+        }
+        if (keep && (s < p.MC || (sp >= 0 && sp < p.MC))) {
+            acc.store(stg, LD);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (s < p.MC) {
                 const size_t sc = (size_t) ib * p.MC + s;
-                acc.store(Cv + sc * B2);
-                if (lane == 0) {
-                    p.Ci[sc] = kb;
-                    atomicOr(&kbits[s >> 5], 1u << (s & 31));
+                T *D = Cv + sc * B2;
+                for (int e = lane; e < (int) B2; e += 64) {
+                    const int r = e / BT, c = e % BT;
+                    D[e] = (kb == ib && r > c) ? E::conj(stg[c * LD + r]) : stg[r * LD + c];
                 }
+                if (lane == 0) p.Ci[sc] = kb;
             }
-            if (lane == 0) atomicAdd(&kept_s, 1);
+            if (sp >= 0 && sp < p.MC) {
+                const size_t sc = (size_t) kb * p.MC + sp;
+                T *D = Cv + sc * B2;
+                for (int e = lane; e < (int) B2; e += 64) {
+                    const int r = e / BT, c = e % BT;
+                    D[e] = E::conj(stg[c * LD + r]);
+                }
+                if (lane == 0) p.Ci[sc] = ib;
+            }
+            // the staging tile is read before the next slot overwrites it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
     __syncthreads();
-    // phase 2: move each kept tile to its rank among the kept slots
-    constexpr int Q = (int) ((B2 + 63) / 64);
-    const uint64_t ltmask = (1ull << lane) - 1ull;
-    int running = 0;
-    for (int c0 = 0; c0 < nbits; c0 += 64) {
-        const int sl = c0 + lane;
-        const bool kp = sl < nbits && ((kbits[sl >> 5] >> (sl & 31)) & 1u);
-        const uint64_t km = ballot64(kp);
-        const int pos = running + popc64(km & ltmask);
-        running += popc64(km);
-        const bool mv = kp && pos != sl;
-        const uint64_t mm = ballot64(mv);
-        const int rank = popc64(mm & ltmask);
-        const int nmv = popc64(mm);
-        for (int g0 = 0; g0 < nmv; g0 += NUM_WAVES) {
-            const uint64_t mine = ballot64(mv && rank == g0 + w);
-            T buf[Q];
-            int src = -1, dst = -1;
-            if (mine) {
-                const int l = __ffsll((unsigned long long) mine) - 1;
-                src = c0 + l;
-                dst = __shfl(pos, l, 64);
-                const T *S = Cv + ((size_t) ib * p.MC + src) * B2;
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const size_t e = (size_t) lane + 64 * q;
-                    if (e < B2) buf[q] = S[e];
-                }
-            }
-            __syncthreads();
-            if (mine) {
-                T *D = Cv + ((size_t) ib * p.MC + dst) * B2;
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const size_t e = (size_t) lane + 64 * q;
-                    if (e < B2) D[e] = buf[q];
-                }
-                if (lane == 0) p.Ci[(size_t) ib * p.MC + dst] = skb[src];
-            }
-        }
-    }
+    if (!p.sym) compact_kept<T, BT>(p, ib, nbits, kbits);
     // tx2 lives in lane 0 of the wavefront that formed the diagonal block
     __shared__ double tr2_s[NUM_WAVES];
     if (lane == 0) tr2_s[w] = E::re(tx2);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int kept = kept_s;
-        p.Cn[ib] = kept < p.MC ? kept : p.MC;
-        p.rowstat[ib] = make_int4(kept, count, off[count], p.An[ib]);
+        if (!p.sym) {
+            const int kept = kept_s;
+            p.Cn[ib] = kept < p.MC ? kept : p.MC;
+            p.rowstat[ib] = make_int4(kept, count, off[count], p.An[ib]);
+        }
         if (p.kind == 0) {
             double t2 = 0.0;
             for (int v = 0; v < NUM_WAVES; v++) t2 += tr2_s[v];
             p.rowtr[ib] = E::re(tx);
             p.rowtr[p.NB + ib] = t2;
         }
+    }
+}
+
+// sym: after every row's tiles and mirrors are stored, each block row
+// gathers its kept flags (its own slots and the mirrored ones), compacts the
+// kept tiles into slot order and records its counts.
+template <typename T, int BT>
+__global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_compact(EbParams p) {
+    extern __shared__ __align__(16) unsigned kbits[];
+    __shared__ int kept_s;
+    if (p.skip && *p.skip) return;
+    const int ib = blockIdx.x;
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int count = p.cnt[ib];
+    const int nbits = count < p.MC ? count : p.MC;
+    const int *fl = p.kflag + (size_t) ib * p.maxs;
+    if (threadIdx.x == 0) kept_s = 0;
+    __syncthreads();
+    for (int c0 = 64 * w; c0 < count; c0 += 64 * NUM_WAVES) {
+        const int sl = c0 + lane;
+        const uint64_t m = ballot64(sl < count && fl[sl] != 0);
+        if (lane == 0) {
+            atomicAdd(&kept_s, popc64(m));
+            const uint64_t in = c0 + 64 <= nbits ? m
+                                : c0 < nbits ? m & ((1ull << (nbits - c0)) - 1ull) : 0ull;
+            if (c0 < nbits) {
+                kbits[c0 >> 5] = (unsigned) in;
+                if (c0 + 32 < nbits) kbits[(c0 >> 5) + 1] = (unsigned) (in >> 32);
+            }
+        }
+    }
+    __syncthreads();
+    compact_kept<T, BT>(p, ib, nbits, kbits);
+    if (threadIdx.x == 0) {
+        const int kept = kept_s;
+        const int *off = p.soff + (size_t) ib * (p.maxs + 1);
+        p.Cn[ib] = kept < p.MC ? kept : p.MC;
+        p.rowstat[ib] = make_int4(kept, count, off[count], p.An[ib]);
     }
 }
 
@@ -932,11 +1053,24 @@ int launch_numeric(const EbParams &p, hipStream_t s) {
                                                 (ENGINE == 1 && std::is_same<T, double>::value)))) {
         return fail_msg(2, "no MFMA engine for this element type and tile");
     } else {
-        const int nbits = p.maxs < p.MC ? p.maxs : p.MC;
-        const size_t lds = sizeof(unsigned) * (size_t) ((nbits + 31) / 32 + 1);
+        // kept flags, then (sym) a staging tile per wavefront, as the kernel
+        // lays them out
+        const int mbits = p.maxs < p.MC ? p.maxs : p.MC;
+        size_t lds = sizeof(unsigned) * (size_t) ((mbits + 31) / 32 + 4) / 4 * 4;
+        if (p.sym) lds += sizeof(T) * (size_t) NUM_WAVES * BT * (BT + 1);
+        if (lds > 64 * 1024) {
+            if (lds > 160 * 1024) return fail_msg(5, "ellblock product: %d block slots per row exceed LDS", p.MC);
+            (void) hipFuncSetAttribute((const void *) k_eb_numeric<T, ENGINE, BT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+        }
         hipLaunchKernelGGL((k_eb_numeric<T, ENGINE, BT>), dim3(p.NB), dim3(64 * NUM_WAVES), lds, s,
                            p);
-        return report("k_eb_numeric");
+        int rc = report("k_eb_numeric");
+        if (rc || !p.sym) return rc;
+        const int nbits = p.maxs < p.MC ? p.maxs : p.MC;
+        const size_t lc = sizeof(unsigned) * (size_t) ((nbits + 31) / 32 + 1);
+        hipLaunchKernelGGL((k_eb_compact<T, BT>), dim3(p.NB), dim3(64 * NUM_WAVES), lc, s, p);
+        return report("k_eb_compact");
     }
 }
 
@@ -989,6 +1123,9 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     p.skb = (int *) bmlhip_scratch(4, sizeof(int) * (NB * maxs + 1));
     p.soff = (int *) bmlhip_scratch(5, sizeof(int) * (NB * (maxs + 1) + 1));
     p.pairs = (int *) bmlhip_scratch(6, sizeof(int) * (NB * maxp + 1));
+    p.sym = a->kind == 0 && a->sym && a->engine != 0;
+    if (p.sym && !(p.kflag = (int *) bmlhip_scratch(1, sizeof(int) * (NB * maxs + 1))))
+        return fail_msg(4, "ellblock product: scratch allocation failed");
     p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (NB + 1));
     if (!p.rowtr || !p.cnt || !p.skb || !p.soff || !p.pairs || !p.rowstat)
         return fail_msg(4, "ellblock product: scratch allocation failed");
@@ -1176,6 +1313,83 @@ int eb_identity_typed(int NB, int bt, const int *bsize, int MB, void *Av, int *A
     return report("k_eb_identity");
 }
 
+// ---------------------------------------------------------------------------
+// symmetry test: one wavefront per block row; every stored block (ib, jb)
+// needs a stored (jb, ib) holding its conjugate transpose (values compared
+// with ==, so +0 and -0 match and a NaN never does)
+__device__ __forceinline__ bool same_value(float a, float b) { return a == b; }
+__device__ __forceinline__ bool same_value(double a, double b) { return a == b; }
+__device__ __forceinline__ bool same_value(cf32 a, cf32 b) { return a.x == b.x && a.y == b.y; }
+__device__ __forceinline__ bool same_value(cf64 a, cf64 b) { return a.x == b.x && a.y == b.y; }
+
+template <typename T, int BT>
+__global__ __launch_bounds__(64) void k_eb_check_sym(int NB, int MB, const T *Av, const int *Ai,
+                                                     const int *An, int *bad) {
+    const int ib = blockIdx.x, lane = threadIdx.x;
+    constexpr int B2 = BT * BT;
+    bool ok = true;
+    const int n = An[ib];
+    for (int jp = 0; jp < n && ok; jp++) {
+        const size_t sa = (size_t) ib * MB + jp;
+        const int jb = Ai[sa];
+        if (jb < 0 || jb >= NB) {
+            ok = false;
+            break;
+        }
+        const int nj = An[jb];
+        int sp = -1;
+        for (int c0 = 0; c0 < nj && sp < 0; c0 += 64) {
+            const int q = c0 + lane;
+            const uint64_t m = ballot64(q < nj && Ai[(size_t) jb * MB + q] == ib);
+            if (m) sp = c0 + __ffsll((unsigned long long) m) - 1;
+        }
+        if (sp < 0) {
+            ok = false;
+            break;
+        }
+        const T *X = Av + sa * B2, *Y = Av + ((size_t) jb * MB + sp) * B2;
+        bool eq = true;
+        for (int e = lane; e < B2; e += 64)
+            eq = eq && same_value(X[e], Elem<T>::conj(Y[(e % BT) * BT + e / BT]));
+        ok = ballot64(!eq) == 0;
+    }
+    if (!ok && lane == 0) atomicAdd(bad, 1);
+}
+
+template <typename T>
+int eb_check_sym_typed(int NB, int bt, int MB, const void *Av, const int *Ai, const int *An,
+                       int *symmetric) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int *bad = (int *) bmlhip_scratch(BMLHIP_SYMFILL_SLOT + 1, sizeof(int) * 64);
+    if (!bad) return fail_msg(4, "scratch allocation failed");
+    if (hipMemsetAsync(bad, 0, sizeof(int), s) != hipSuccess) return fail_msg(3, "memset failed");
+    if (NB > 0) {
+        switch (bt) {
+        case 8:
+            hipLaunchKernelGGL((k_eb_check_sym<T, 8>), dim3(NB), dim3(64), 0, s, NB, MB,
+                               (const T *) Av, Ai, An, bad);
+            break;
+        case 16:
+            hipLaunchKernelGGL((k_eb_check_sym<T, 16>), dim3(NB), dim3(64), 0, s, NB, MB,
+                               (const T *) Av, Ai, An, bad);
+            break;
+        case 32:
+            hipLaunchKernelGGL((k_eb_check_sym<T, 32>), dim3(NB), dim3(64), 0, s, NB, MB,
+                               (const T *) Av, Ai, An, bad);
+            break;
+        default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+        }
+    }
+    int rc = report("k_eb_check_sym");
+    if (rc) return rc;
+    int h = 0;
+    if (hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail_msg(3, "symmetry test: copy failed");
+    *symmetric = h == 0;
+    return 0;
+}
+
 }  // namespace
 
 #define DISPATCH(prec, call_tmpl)                                                  \
@@ -1192,6 +1406,13 @@ extern "C" {
 int bmlhip_eb_product(const bmlhip_eb_args *a) {
 #define CALL(T) eb_product_typed<T>(a)
     DISPATCH(a->prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_is_symmetric(int prec, int NB, int bt, int MB, const void *Av, const int *Ai,
+                           const int *An, int *symmetric) {
+#define CALL(T) eb_check_sym_typed<T>(NB, bt, MB, Av, Ai, An, symmetric)
+    DISPATCH(prec, CALL)
 #undef CALL
 }
 

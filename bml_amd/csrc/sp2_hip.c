@@ -576,7 +576,8 @@ out:
 
 static int eb_sp2_iteration(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hip_t *X2,
                             bmlhip_sp2_state *d_state, double *d_hist,
-                            const bml_hip_sp2_options *o, double nocc, const char *what)
+                            const bml_hip_sp2_options *o, double nocc, int sym,
+                            const char *what)
 {
     bmlhip_eb_args a;
     memset(&a, 0, sizeof a);
@@ -596,6 +597,7 @@ static int eb_sp2_iteration(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hi
     a.threshold = o->threshold;
     a.engine = bmlh_block_engine_for(X);
     a.d_skip = &d_state->done;
+    a.sym = sym;
     return bmlh_ok(bmlhip_eb_product(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
            bmlh_ok(bmlhip_eb_add(X->matrix_precision, X->NB, X->bt, X->bsize, X->MB, X->value,
@@ -654,6 +656,13 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
     r.emin = g[0];
     r.emax = g[1];
     free(g);
+    /* A symmetric H gives symmetric iterates: on the MFMA engines X^2 then
+     * forms the blocks kb >= ib and mirrors them */
+    int sym = 0;
+    if (bmlh_block_engine_for(H) != BML_HIP_BLOCK_ENGINE_EXACT &&
+        !bmlh_ok(bmlhip_eb_is_symmetric(H->matrix_precision, H->NB, H->bt, H->MB, H->value,
+                                        H->indexb, H->nnzb, &sym), what))
+        return;
     bml_copy_ellblock_hip(H, X);
     if (bmlh_failed())
         return;
@@ -689,7 +698,7 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
     stall_track stall = { 0, 0.0, 0.0 };
     for (int it = 0; it < o.maxit; it++) {
         bml_matrix_ellblock_hip_t *cur = single ? Xs : X, *cur2 = single ? X2s : X2;
-        if (!eb_sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, what))
+        if (!eb_sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, sym, what))
             goto out;
         int last_single = single && it + 1 >= o.maxit - o.refine_iters;
         if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {

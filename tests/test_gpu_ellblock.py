@@ -250,3 +250,105 @@ def test_sp2_mixed_precision(hip):
         assert 5 <= rm.single_iterations < rm.iterations
         assert abs(rm.trace_x - N / 2) < 1e-7
         assert np.abs(hip.eb_export_to_dense(Xm) - ref).max() < err
+
+
+def eb_is_symmetric(hip, dA) -> bool:
+    """bmlhip_eb_is_symmetric through the C ABI."""
+    import ctypes as C
+    s = dA._s
+    out = C.c_int(-1)
+    rc = hip.lib().bmlhip_eb_is_symmetric(s.matrix_precision, s.NB, s.bt, s.MB,
+                                          C.c_void_p(s.value), C.c_void_p(s.indexb),
+                                          C.c_void_p(s.nnzb), C.byref(out))
+    assert rc == 0 and out.value in (0, 1)
+    return out.value == 1
+
+
+@pytest.mark.parametrize("dtype,engine,rtol", [(np.float64, 1, 1e-12), (np.float32, 1, 1e-5),
+                                               (np.float32, 2, 2.0 ** -7)])
+def test_mfma_symmetric_x2_matches_oracle(hip, monkeypatch, dtype, engine, rtol):
+    """x2 of an exactly symmetric X forming only the blocks kb >= ib and
+    mirroring them: the same blocks in the same order as the oracle, values
+    to the engine's tolerance, and an exactly symmetric result."""
+    H = G.block_lattice_hamiltonian(4, 4, 2, b=32, MB=32, dtype=dtype)
+    ref, otr = orc.eb_x2(H, 32, 0.05)
+    hip.set_block_engine(engine)
+    monkeypatch.setenv("BML_HIP_ASSUME_SYMMETRIC", "1")
+    try:
+        dH = hip.from_blk(H)
+        assert eb_is_symmetric(hip, dH)
+        X2 = hip.block_matrix(H.precision, H.bsize, 32)
+        tr = hip.eb_multiply_x2(dH, X2, 0.05)
+        got = X2.get()
+    finally:
+        hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
+    D = got.to_dense()
+    assert np.array_equal(D, D.T)
+    if engine == 2:
+        absH = Blk(np.abs(H.value), H.indexb, H.nnzb, H.bsize)
+        bound, _ = orc.eb_x2(absH, 32, 0.0)
+        assert (np.abs(D - ref.to_dense()) <= rtol * bound.to_dense() + 1e-30).all()
+        np.testing.assert_array_equal(got.nnzb, ref.nnzb)
+        assert rel_close(tr[1], otr[1], 1e-2)
+    else:
+        blocks_close(got, ref, rtol, f"symmetric, engine {engine}")
+        assert rel_close(tr[0], otr[0], 1e-12)
+        assert rel_close(tr[1], otr[1], 1e-12 if dtype == np.float64 else 1e-5)
+
+
+def test_mfma_symmetric_x2_dense_and_overflow(hip, monkeypatch):
+    """The symmetric path at threshold 0 is the dense product; with fewer
+    slots than touched block columns it raises the overflow, after which the
+    library stays usable."""
+    H = G.block_lattice_hamiltonian(3, 3, 2, b=32, MB=32)
+    Dh = H.to_dense()
+    monkeypatch.setenv("BML_HIP_ASSUME_SYMMETRIC", "1")
+    dH = hip.from_blk(H)
+    X2 = hip.block_matrix(H.precision, H.bsize, 18)
+    tr = hip.eb_multiply_x2(dH, X2, 0.0)
+    ref = Dh @ Dh
+    got = hip.eb_export_to_dense(X2)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(got, got.T)
+    assert rel_close(tr[1], np.trace(ref), 1e-12)
+    small = hip.block_matrix(H.precision, H.bsize, 4)
+    with pytest.raises(hip.BmlError) as e:
+        hip.eb_multiply_x2(dH, small, 0.0)
+    assert e.value.code == hip.ERR_M_OVERFLOW
+    hip.eb_multiply_x2(dH, X2, 0.0)
+    assert np.array_equal(hip.eb_export_to_dense(X2), got)
+
+
+def test_eb_symmetry_test(hip):
+    """The symmetry test that puts SP2 on the mirrored X^2: a symmetric H
+    passes; one changed element, or a block without its transpose, fails."""
+    H = G.block_lattice_hamiltonian(3, 2, 2, b=32, MB=8)
+    assert eb_is_symmetric(hip, hip.from_blk(H))
+    P = Blk(H.value.copy(), H.indexb.copy(), H.nnzb.copy(), H.bsize)
+    e = int(np.nonzero(P.indexb[1, :P.nnzb[1]] != 1)[0][0])
+    P.value[1, e, 3, 5] += 1e-12
+    assert not eb_is_symmetric(hip, hip.from_blk(P))
+    Q = Blk(H.value.copy(), H.indexb.copy(), H.nnzb.copy(), H.bsize)
+    Q.nnzb[0] -= 1  # drop row 0's last block, keep its transpose
+    assert not eb_is_symmetric(hip, hip.from_blk(Q))
+
+
+def test_sp2_nonsymmetric_h_uses_full_product(hip):
+    """An H that is not exactly symmetric keeps the full X^2 on the MFMA
+    engine: the result stays close to the exact engine's solve."""
+    H = G.block_lattice_hamiltonian(2, 2, 2, b=32, MB=8, onsite_shift=6.0, coupling=0.05)
+    e = int(np.nonzero(H.indexb[2, :H.nnzb[2]] != 2)[0][0])
+    H.value[2, e, 0, 1] += 1e-9
+    out = {}
+    for engine in (hip.BLOCK_ENGINE_EXACT, hip.BLOCK_ENGINE_MFMA):
+        hip.set_block_engine(engine)
+        try:
+            dH = hip.from_blk(H)
+            assert not eb_is_symmetric(hip, dH)
+            X = hip.block_matrix(H.precision, H.bsize, 8)
+            r, _ = hip.eb_sp2(dH, X, H.N / 2, threshold=1e-9, tol=1e-9, maxit=60)
+            assert r.converged
+        finally:
+            hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
+        out[engine] = hip.eb_export_to_dense(X)
+    assert np.abs(out[hip.BLOCK_ENGINE_EXACT] - out[hip.BLOCK_ENGINE_MFMA]).max() <= 1e-8
