@@ -61,7 +61,7 @@ void *bmlhip_stream(void);
 /* Library-owned scratch, grown on demand; slot selects one of a few
  * independent buffers.  The pointer stays valid until the next call with
  * the same slot and a larger size. */
-#define BMLHIP_SCRATCH_SLOTS 10
+#define BMLHIP_SCRATCH_SLOTS 12
 void *bmlhip_scratch(int slot, size_t bytes);
 /* as bmlhip_scratch, zero-filled whenever the buffer is (re)allocated */
 void *bmlhip_scratch_zeroed(int slot, size_t bytes);

@@ -39,6 +39,9 @@ namespace {
 
 constexpr int EMPTY = -1;
 constexpr int NUM_WAVES = 4;  // wavefronts per block row in k_eb_numeric
+// pairs per batch, the unit of work a wavefront of k_eb_numeric claims: batch
+// k holds the slots whose first pair lies in [EB_BATCH k, EB_BATCH (k+1))
+constexpr int EB_BATCH = 32;
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
@@ -60,14 +63,28 @@ struct EbParams {
     int *skb;        // [NB*maxs] block column of each slot
     int *soff;       // [NB*(maxs+1)] start of each slot's pair list
     int *pairs;      // [NB*maxp] jp << 16 | kp, grouped by slot
+    int *pslot;      // [NB*maxp] slot of each pair
+    int *bfirst;     // [NB*(maxp/EB_BATCH+2)] batch k: slots [bfirst[k], bfirst[k+1])
     double *rowtr;   // [2*NB] per block row tr X, tr X^2 (x2)
     int4 *rowstat;   // [NB] (kept, touched, block pairs, nnzb of A's row)
     bmlhip_status *st;
     const int *skip;
     int remap;  // numeric pass: XCD-aware block-row order (xcd_block)
-    int sym;    // x2 of an exactly symmetric X: form the slots kb >= ib only
+    int sym;    // x2 of an exactly symmetric X: each row forms the slots it owns (sym_owner)
     int *kflag; // [NB*maxs] sym: kept flag of every slot, set by its row or its mirror
 };
+
+// The symmetric X^2 forms each off-diagonal pair of blocks (i, k), (k, i)
+// once, in the block row this picks: a hash of the unordered pair, so every
+// block row forms about half of its blocks whatever the ordering (on a
+// periodic lattice "k >= i" would leave the first planes' rows with most of
+// the work and the last planes' with little, one XCD's share against
+// another's).  Symmetric in (i, k); the diagonal block is its row's.
+__device__ __forceinline__ bool sym_owner(int ib, int kb) {
+    const unsigned lo = (unsigned) (ib < kb ? ib : kb), hi = (unsigned) (ib < kb ? kb : ib);
+    const unsigned h = lo * 0x9E3779B1u ^ hi * 0x85EBCA77u;
+    return ib == kb || ((h >> 31) != 0) == (ib == (int) lo);
+}
 
 __device__ __forceinline__ unsigned eb_hash(int kb, unsigned mask) {
     return ((unsigned) kb * 0x9E3779B1u >> 7) & mask;
@@ -134,9 +151,9 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
             }
             count += popc64(nm);
             __syncthreads();  // new slot numbers visible to every lane
-            // the chunk's slots are distinct; with p.sym the slots kb < ib
-            // get no pairs (their tiles are the mirrors of row kb's)
-            if (act && (!p.sym || kb >= ib)) fill[sval[h]] += 1;
+            // the chunk's slots are distinct; with p.sym the slots row ib
+            // does not own get no pairs (their tiles are mirrors)
+            if (act && (!p.sym || sym_owner(ib, kb))) fill[sval[h]] += 1;
             __syncthreads();
         }
     }
@@ -158,6 +175,18 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
         running += __shfl(inc, 63, 64);
     }
     if (lane == 0) off[count] = running;
+    // batch starts: bfirst[k] = the first slot s with off[s] >= EB_BATCH k
+    // (count for the end), k <= running / EB_BATCH + 1
+    int *bf = p.bfirst + (size_t) ib * (p.maxp / EB_BATCH + 2);
+    for (int s0 = 0; s0 <= count; s0 += 64) {
+        const int s = s0 + lane;
+        if (s <= count) {
+            const int cur = s < count ? fill[s] : running;
+            const int klo = s == 0 ? 0 : fill[s - 1] / EB_BATCH + 1;
+            for (int k = klo; k <= cur / EB_BATCH; k++) bf[k] = s;
+        }
+    }
+    if (lane == 0) bf[running / EB_BATCH + 1] = count;
     __syncthreads();
     // pass 2: pairs grouped by slot, jp order kept within a slot
     int *pr = p.pairs + (size_t) ib * p.maxp;
@@ -167,11 +196,12 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
         for (int k0 = 0; k0 < nb; k0 += 64) {
             const int kp = k0 + lane;
             const int kb = kp < nb ? p.Bi[(size_t) jb * p.MB + kp] : -1;
-            if (kp < nb && (!p.sym || kb >= ib)) {
+            if (kp < nb && (!p.sym || sym_owner(ib, kb))) {
                 unsigned h = eb_hash(kb, mask);
                 while (keys[h] != kb) h = (h + 1u) & mask;
                 const int s = sval[h];
                 pr[fill[s]] = (int) (((unsigned) jp << 16) | (unsigned) kp);
+                p.pslot[(size_t) ib * p.maxp + fill[s]] = s;
                 fill[s] += 1;
             }
             __syncthreads();
@@ -459,7 +489,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // wavefront claims which slot.
 //
 // With p.sym (x2 of an exactly symmetric X, so X^2 is symmetric): row ib
-// forms only its slots kb >= ib.  A tile is staged in LDS and stored twice:
+// forms only the slots it owns (sym_owner).  A tile is staged in LDS and stored twice:
 // at (ib, s) and, transposed, at (kb, s') where s' is the slot of ib in row
 // kb; the kept flag goes to both slots.  The diagonal tile takes its upper
 // triangle for both halves, so X^2 is exactly symmetric.  The compaction
@@ -518,20 +548,42 @@ __device__ void compact_kept(const EbParams &p, int ib, int nbits, const unsigne
     }
 }
 
-// slot of block column col among the count slots of block row r (-1: none)
-__device__ __forceinline__ int find_slot(const EbParams &p, int r, int col) {
-    const int n = p.cnt[r];
+// Slot of block column col among the slots of block row r (-1: none).  The
+// first 3 chunks of r's slot list are loaded by mirror_probe when a slot
+// starts, so their latency hides behind the slot's products; mirror_slot
+// then matches them, and scans any further chunks itself.
+struct MirrorProbe {
+    int n;
+    int v[3];
+};
+__device__ __forceinline__ MirrorProbe mirror_probe(const EbParams &p, int r) {
+    MirrorProbe m;
     const int *skb = p.skb + (size_t) r * p.maxs;
-    for (int c0 = 0; c0 < n; c0 += 64) {
+    m.n = p.cnt[r];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int q = 64 * c + lane_id();
+        m.v[c] = q < p.maxs ? skb[q] : -1;
+    }
+    return m;
+}
+__device__ __forceinline__ int mirror_slot(const EbParams &p, const MirrorProbe &m, int r, int col) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const uint64_t b = ballot64(64 * c + lane_id() < m.n && m.v[c] == col);
+        if (b) return 64 * c + __ffsll((unsigned long long) b) - 1;
+    }
+    const int *skb = p.skb + (size_t) r * p.maxs;
+    for (int c0 = 192; c0 < m.n; c0 += 64) {
         const int q = c0 + lane_id();
-        const uint64_t m = ballot64(q < n && skb[q] == col);
-        if (m) return c0 + __ffsll((unsigned long long) m) - 1;
+        const uint64_t b = ballot64(q < m.n && skb[q] == col);
+        if (b) return c0 + __ffsll((unsigned long long) b) - 1;
     }
     return -1;
 }
 
 template <typename T, int ENGINE, int BT>
-__global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
+__global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
     using E = Elem<T>;
     using Acc = typename AccOf<T, ENGINE, BT>::type;
     // dynamic LDS: the kept flag of each slot s < min(maxs, MC), then (sym)
@@ -570,51 +622,20 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
         }
     }
     __syncthreads();
-    while (true) {
-        int s = 0;
-        if (lane == 0) s = atomicAdd(&next_s, 1);
-        s = __shfl(s, 0, 64);
-        if (s >= count) break;
-        const int kb = skb[s];
-        if (p.sym && kb < ib) continue;  // the mirror of row kb's slot
-        const int bj = p.bsize[kb];
-        Acc acc;
-        acc.zero();
-        if constexpr (Acc::kPipelined) {
-            // up to 64 pairs at a time: lane q decodes pair t0 + q (the tile
-            // offsets), then the wavefront runs the pairs with the next
-            // pair's tiles loading while the current pair's MFMAs run
-            using Frag = typename Acc::Frag;
-            for (int t0 = off[s]; t0 < off[s + 1]; t0 += 64) {
-                const int np = off[s + 1] - t0 < 64 ? off[s + 1] - t0 : 64;
-                size_t la = 0, lb = 0;
-                if (lane < np) {
-                    const unsigned pv = (unsigned) pr[t0 + lane];
-                    const size_t sa = (size_t) ib * p.MA + (pv >> 16);
-                    la = sa * B2;
-                    lb = ((size_t) p.Ai[sa] * p.MB + (pv & 0xffffu)) * B2;
-                }
-                Frag f0, f1;
-                Acc::load(Av + lane_get(la, 0), Bv + lane_get(lb, 0), f0);
-                for (int q = 0; q < np; q += 2) {
-                    if (q + 1 < np)
-                        Acc::load(Av + lane_get(la, q + 1), Bv + lane_get(lb, q + 1), f1);
-                    acc.mma(f0);
-                    if (q + 1 < np) {
-                        if (q + 2 < np)
-                            Acc::load(Av + lane_get(la, q + 2), Bv + lane_get(lb, q + 2), f0);
-                        acc.mma(f1);
-                    }
-                }
-            }
-        } else {
-            for (int t = off[s]; t < off[s + 1]; t++) {
-                const int jp = (int) ((unsigned) pr[t] >> 16), kp = pr[t] & 0xffff;
-                const size_t sa = (size_t) ib * p.MA + jp;
-                const int jb = p.Ai[sa];
-                acc.mac(Av + sa * B2, Bv + ((size_t) jb * p.MB + kp) * B2, bi, p.bsize[jb], bj);
-            }
+    // the next slot this wavefront forms (-1: none left); with p.sym the
+    // slots row ib does not own are skipped (mirrors).  Every formed slot has
+    // at least one pair: a slot exists because a pair touched it.
+    auto claim = [&]() -> int {
+        while (true) {
+            int c = 0;
+            if (lane == 0) c = atomicAdd(&next_s, 1);
+            c = __shfl(c, 0, 64);
+            if (c >= count) return -1;
+            if (!(p.sym && !sym_owner(ib, skb[c]))) return c;
         }
+    };
+    // keep rule, tr X^2 and the stores of slot s's finished tile
+    auto finish_slot = [&](const Acc &acc, int s, int kb, const MirrorProbe &mp) {
         const double nrm = wave_max(acc.maxmag());
         const bool keep = kb == ib || nrm > p.thr;
         if (p.kind == 0 && kb == ib) {
@@ -638,10 +659,10 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
                 }
                 if (lane == 0) atomicAdd(&kept_s, 1);
             }
-            continue;
+            return;
         }
         // sym: the tile at (ib, s) and its transpose at (kb, s')
-        const int sp = kb > ib ? find_slot(p, kb, ib) : -1;
+        const int sp = kb != ib ? mirror_slot(p, mp, kb, ib) : -1;
         if (lane == 0) {
             p.kflag[(size_t) ib * p.maxs + s] = keep;
             if (sp >= 0) p.kflag[(size_t) kb * p.maxs + sp] = keep;
@@ -673,6 +694,120 @@ __global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_numeric(EbParams p) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    };
+    if constexpr (Acc::kPipelined) {
+        // One stream of pairs per wavefront.  It claims batches (about
+        // EB_BATCH pairs of whole slots) from an LDS counter; a unit is up to
+        // 64 pairs of a batch, and lane q holds pair t0 + q's A tile, B tile,
+        // slot and block column.  The next unit is decoded as soon as a unit
+        // starts, and the next pair's tiles load while the current pair's
+        // MFMAs run, across slots and units, so a slot's keep test and stores
+        // and the index loads behind the next tiles overlap the MFMAs.
+        using Frag = typename Acc::Frag;
+        const int *ps = p.pslot + (size_t) ib * p.maxp;
+        const int *bf = p.bfirst + (size_t) ib * (p.maxp / EB_BATCH + 2);
+        const int nbatch = off[count] / EB_BATCH + 1;
+        struct Unit {
+            int t0, te, np;
+            int sa, bt, sid, kb;  // per lane
+        };
+        auto decode = [&](Unit &u) {
+            u.np = u.te - u.t0 < 64 ? u.te - u.t0 : 64;
+            u.sa = 0;
+            u.bt = 0;
+            u.sid = -1;
+            u.kb = -1;
+            if (lane < u.np) {
+                const int t = u.t0 + lane;
+                const unsigned pv = (unsigned) pr[t];
+                u.sid = ps[t];
+                u.sa = ib * p.MA + (int) (pv >> 16);
+                u.bt = p.Ai[u.sa] * p.MB + (int) (pv & 0xffffu);
+                u.kb = skb[u.sid];
+            }
+        };
+        // the unit after u (u.np == 0: the first); false when no batch is left
+        auto next_unit = [&](Unit &n, const Unit &u) -> bool {
+            if (u.np > 0 && u.t0 + u.np < u.te) {
+                n.t0 = u.t0 + u.np;
+                n.te = u.te;
+            } else {
+                while (true) {
+                    int k = 0;
+                    if (lane == 0) k = atomicAdd(&next_s, 1);
+                    k = __shfl(k, 0, 64);
+                    if (k >= nbatch) return false;
+                    n.t0 = off[bf[k]];
+                    n.te = off[bf[k + 1]];
+                    if (n.te > n.t0) break;
+                }
+            }
+            decode(n);
+            return true;
+        };
+        auto tile_a = [&](const Unit &u, int q) { return Av + (size_t) __builtin_amdgcn_readlane(u.sa, q) * B2; };
+        auto tile_b = [&](const Unit &u, int q) { return Bv + (size_t) __builtin_amdgcn_readlane(u.bt, q) * B2; };
+        Unit cu, nu;
+        cu.np = 0;
+        if (next_unit(cu, cu)) {
+            bool have_nu = next_unit(nu, cu);
+            Acc acc;
+            acc.zero();
+            Frag f0, f1;
+            int q = 0, slot = -1, kb = -1;
+            MirrorProbe mp;
+            Acc::load(tile_a(cu, 0), tile_b(cu, 0), f0);
+            // run pair q of cu (in cur) while the pair after it loads into nxt
+            auto step = [&](Frag &cur, Frag &nxt) -> bool {
+                if (q + 1 < cu.np)
+                    Acc::load(tile_a(cu, q + 1), tile_b(cu, q + 1), nxt);
+                else if (have_nu)
+                    Acc::load(tile_a(nu, 0), tile_b(nu, 0), nxt);
+                const int sq = __builtin_amdgcn_readlane(cu.sid, q);
+                if (sq != slot) {
+                    if (slot >= 0) {
+                        finish_slot(acc, slot, kb, mp);
+                        acc.zero();
+                    }
+                    slot = sq;
+                    kb = __builtin_amdgcn_readlane(cu.kb, q);
+                    if (p.sym && kb != ib) mp = mirror_probe(p, kb);
+                }
+                acc.mma(cur);
+                if (q + 1 < cu.np) {
+                    q++;
+                    return true;
+                }
+                if (!have_nu) {
+                    finish_slot(acc, slot, kb, mp);
+                    return false;
+                }
+                cu = nu;
+                q = 0;
+                have_nu = next_unit(nu, cu);
+                return true;
+            };
+            while (step(f0, f1) && step(f1, f0)) {
+            }
+        }
+    } else {
+        while (true) {
+            const int s = claim();
+            if (s < 0) break;
+            const int kb = skb[s];
+            const int bj = p.bsize[kb];
+            Acc acc;
+            acc.zero();
+            for (int t = off[s]; t < off[s + 1]; t++) {
+                const int jp = (int) ((unsigned) pr[t] >> 16), kp = pr[t] & 0xffff;
+                const size_t sa = (size_t) ib * p.MA + jp;
+                const int jb = p.Ai[sa];
+                acc.mac(Av + sa * B2, Bv + ((size_t) jb * p.MB + kp) * B2, bi, p.bsize[jb], bj);
+            }
+            MirrorProbe mp;
+            if (p.sym && kb != ib) mp = mirror_probe(p, kb);
+            finish_slot(acc, s, kb, mp);
         }
     }
     __syncthreads();
@@ -1123,11 +1258,15 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     p.skb = (int *) bmlhip_scratch(4, sizeof(int) * (NB * maxs + 1));
     p.soff = (int *) bmlhip_scratch(5, sizeof(int) * (NB * (maxs + 1) + 1));
     p.pairs = (int *) bmlhip_scratch(6, sizeof(int) * (NB * maxp + 1));
+    p.pslot = (int *) bmlhip_scratch(10, sizeof(int) * (NB * maxp + 1));
+    p.bfirst = (int *) bmlhip_scratch(11, sizeof(int) * (NB * (maxp / EB_BATCH + 2) + 1));
+    if ((long long) a->NB * a->MA >= (1ll << 31) || (long long) a->NB * a->MB >= (1ll << 31))
+        return fail_msg(5, "ellblock product: more than 2^31 block slots");
     p.sym = a->kind == 0 && a->sym && a->engine != 0;
     if (p.sym && !(p.kflag = (int *) bmlhip_scratch(1, sizeof(int) * (NB * maxs + 1))))
         return fail_msg(4, "ellblock product: scratch allocation failed");
     p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (NB + 1));
-    if (!p.rowtr || !p.cnt || !p.skb || !p.soff || !p.pairs || !p.rowstat)
+    if (!p.rowtr || !p.cnt || !p.skb || !p.soff || !p.pairs || !p.pslot || !p.bfirst || !p.rowstat)
         return fail_msg(4, "ellblock product: scratch allocation failed");
     int rc = bmlhip_status_reset();
     if (rc) return rc;
