@@ -718,7 +718,7 @@ __device__ __forceinline__ int probe_claim(int *keys, const int (&col)[NC], cons
             int old[NC];
 #pragma unroll
             for (int c = 0; c < NC; c++)
-                old[c] = atomicCAS(&keys[ins[c] ? at[c] : priv], EMPTY, col[c]);
+                old[c] = ins[c] ? atomicCAS(&keys[at[c]], EMPTY, col[c]) : EMPTY;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 // lost to another wavefront inserting the same column: a hit
@@ -836,7 +836,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
             return -1;
 #pragma unroll
         for (int c = 0; c < NCH; c++)
-            lds_add(&vals[slot[c]], act[c] ? UAcc<T>::to(E::mul(a, r.val[c])) : Elem<A>::zero());
+            if (act[c]) lds_add(&vals[slot[c]], UAcc<T>::to(E::mul(a, r.val[c])));
         return 0;
     };
 

@@ -352,3 +352,27 @@ def test_sp2_nonsymmetric_h_uses_full_product(hip):
             hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
         out[engine] = hip.eb_export_to_dense(X)
     assert np.abs(out[hip.BLOCK_ENGINE_EXACT] - out[hip.BLOCK_ENGINE_MFMA]).max() <= 1e-8
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_mfma_x2_slots_longer_than_a_unit(hip, monkeypatch, sym):
+    """Every block present (70 block rows of 32): each slot has 70 block
+    pairs, more than the 64 a wavefront decodes at a time, and a batch spans
+    units.  Against the dense product, with and without the symmetric path."""
+    NB, b = 70, 32
+    rng = np.random.default_rng(11)
+    D = rng.uniform(-1.0, 1.0, (NB * b, NB * b)) / (NB * b)
+    D = D + D.T
+    H = Blk.from_dense(D, [b] * NB, NB)
+    assert int(H.nnzb.min()) == NB
+    if sym:
+        monkeypatch.setenv("BML_HIP_ASSUME_SYMMETRIC", "1")
+    dH = hip.from_blk(H)
+    X2 = hip.block_matrix(H.precision, H.bsize, NB)
+    tr = hip.eb_multiply_x2(dH, X2, 0.0)
+    ref = D @ D
+    got = hip.eb_export_to_dense(X2)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert rel_close(tr[1], np.trace(ref), 1e-12)
+    if sym:
+        assert np.array_equal(got, got.T)
