@@ -368,7 +368,7 @@ def main():
                     help="skip the ELLBLOCK (config 3) and batched (config 4) measurements")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
