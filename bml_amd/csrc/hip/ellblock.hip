@@ -908,7 +908,10 @@ struct EbAddParams {
 // tiles are written in place into A.  A tile is read before any write of
 // its round, and writes only reach slots whose A tiles were read in earlier
 // rounds or this one.
-constexpr int EB_ADD_WAVES = 4;
+#ifndef BMLHIP_EB_ADD_WAVES
+#define BMLHIP_EB_ADD_WAVES 4
+#endif
+constexpr int EB_ADD_WAVES = BMLHIP_EB_ADD_WAVES;
 
 template <typename T, int BT>
 __global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
