@@ -695,3 +695,166 @@ void bml_normalize_ellpack_hip(bml_matrix_ellpack_hip_t *A, double mineval, doub
     finish_into(what, A);
     A->aux->stats_valid = 0;
 }
+
+/* ---- Matrix Market files -------------------------------------------------- */
+
+/* bml_write_bml_matrix_ellpack (ellpack/bml_utilities_ellpack_typed.c:128-190):
+ * a coordinate file, "%%%MatrixMarket matrix coordinate real general" (three
+ * '%', as the reference prints it; "complex" for complex precisions), then
+ * "N N nnz", then one line per stored entry in row order and each row's
+ * storage order: 1-based row and column and the value as %20.15g (complex:
+ * real and imaginary parts). */
+void bml_write_bml_matrix_ellpack_hip(const bml_matrix_ellpack_hip_t *A, const char *filename)
+{
+    const char *what = "bml_write_bml_matrix_ellpack_hip";
+    if (!valid_ell(A, what))
+        return;
+    const int N = A->N, M = A->M, prec = A->matrix_precision;
+    const size_t es = bmlh_elem_size(prec);
+    char *v = malloc((size_t) N * M * es + 1);
+    int *ix = malloc(((size_t) N * M + 1) * sizeof(int));
+    int *nn = malloc(((size_t) N + 1) * sizeof(int));
+    FILE *f = NULL;
+    if (!v || !ix || !nn) {
+        bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+        goto out;
+    }
+    bml_ellpack_hip_get_arrays(A, v, ix, nn);
+    if (bmlh_failed())
+        goto out;
+    if (!(f = fopen(filename, "w"))) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: cannot open %s", what, filename);
+        goto out;
+    }
+    const int cplx = prec == BML_HIP_SINGLE_COMPLEX || prec == BML_HIP_DOUBLE_COMPLEX;
+    fprintf(f, "%%%%%%MatrixMarket matrix coordinate %s general\n", cplx ? "complex" : "real");
+    long long tot = 0;
+    for (int i = 0; i < N; i++)
+        tot += nn[i];
+    fprintf(f, "%d %d %lld\n", N, N, tot);
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < nn[i]; k++) {
+            const size_t at = (size_t) i * M + k;
+            const int c = ix[at] + 1;
+            switch (prec) {
+            case BML_HIP_SINGLE_REAL:
+                fprintf(f, "%d %d %20.15g\n", i + 1, c, (double) ((const float *) v)[at]);
+                break;
+            case BML_HIP_DOUBLE_REAL:
+                fprintf(f, "%d %d %20.15g\n", i + 1, c, ((const double *) v)[at]);
+                break;
+            case BML_HIP_SINGLE_COMPLEX:
+                fprintf(f, "%d %d %20.15g %20.15g\n", i + 1, c,
+                        (double) ((const float *) v)[2 * at], (double) ((const float *) v)[2 * at + 1]);
+                break;
+            default:
+                fprintf(f, "%d %d %20.15g %20.15g\n", i + 1, c, ((const double *) v)[2 * at],
+                        ((const double *) v)[2 * at + 1]);
+                break;
+            }
+        }
+    if (fclose(f) != 0)
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: write to %s failed", what, filename);
+    f = NULL;
+out:
+    if (f)
+        fclose(f);
+    free(v);
+    free(ix);
+    free(nn);
+}
+
+/* bml_read_bml_matrix_ellpack (ellpack/bml_utilities_ellpack_typed.c:22-120):
+ * appends the file's entries to A's rows in file order (A is normally a zero
+ * matrix); a "symmetric" file also stores each off-diagonal entry at its
+ * mirror.  Values are read as %f (single real) or %lf and rounded to A's
+ * precision.  Where the reference would overrun a row, this reports the
+ * M overflow. */
+void bml_read_bml_matrix_ellpack_hip(bml_matrix_ellpack_hip_t *A, const char *filename)
+{
+    const char *what = "bml_read_bml_matrix_ellpack_hip";
+    if (!valid_ell(A, what))
+        return;
+    const int N = A->N, M = A->M, prec = A->matrix_precision;
+    const size_t es = bmlh_elem_size(prec);
+    const int cplx = prec == BML_HIP_SINGLE_COMPLEX || prec == BML_HIP_DOUBLE_COMPLEX;
+    char *v = malloc((size_t) N * M * es + 1);
+    int *ix = malloc(((size_t) N * M + 1) * sizeof(int));
+    int *nn = malloc(((size_t) N + 1) * sizeof(int));
+    FILE *f = NULL;
+    if (!v || !ix || !nn) {
+        bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+        goto out;
+    }
+    bml_ellpack_hip_get_arrays(A, v, ix, nn);
+    if (bmlh_failed())
+        goto out;
+    if (!(f = fopen(filename, "r"))) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: cannot open %s", what, filename);
+        goto out;
+    }
+    char h[5][64];
+    int n1, n2, nz;
+    if (fscanf(f, "%63s %63s %63s %63s %63s", h[0], h[1], h[2], h[3], h[4]) != 5 ||
+        fscanf(f, "%d %d %d", &n1, &n2, &nz) != 3) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: %s: read error on the header", what, filename);
+        goto out;
+    }
+    if (n1 > N || n2 > N || nz < 0) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: %s is %d x %d, the matrix %d x %d", what,
+                   filename, n1, n2, N, N);
+        goto out;
+    }
+    const int sym = strcmp(h[4], "symmetric") == 0;
+    for (int e = 0; e < nz; e++) {
+        int r, c, got;
+        double re = 0.0, im = 0.0;
+        float fv = 0.0f;
+        if (prec == BML_HIP_SINGLE_REAL)
+            got = fscanf(f, "%d %d %f", &r, &c, &fv) == 3;
+        else if (!cplx)
+            got = fscanf(f, "%d %d %lf", &r, &c, &re) == 3;
+        else
+            got = fscanf(f, "%d %d %lf %lf", &r, &c, &re, &im) == 4;
+        if (!got) {
+            bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: %s: read error at entry %d", what, filename, e);
+            goto out;
+        }
+        r--;
+        c--;
+        if (r < 0 || r >= N || c < 0 || c >= N) {
+            bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: %s: entry %d (%d, %d) out of range", what,
+                       filename, e, r + 1, c + 1);
+            goto out;
+        }
+        for (int pass = 0; pass < (sym && c != r ? 2 : 1); pass++) {
+            const int row = pass ? c : r, col = pass ? r : c;
+            if (nn[row] >= M) {
+                bmlh_error(BML_HIP_ERR_M_OVERFLOW,
+                           "%s: row %d: Number of non-zeroes per row > M", what, row);
+                goto out;
+            }
+            const size_t at = (size_t) row * M + nn[row]++;
+            ix[at] = col;
+            switch (prec) {
+            case BML_HIP_SINGLE_REAL: ((float *) v)[at] = fv; break;
+            case BML_HIP_DOUBLE_REAL: ((double *) v)[at] = re; break;
+            case BML_HIP_SINGLE_COMPLEX:
+                ((float *) v)[2 * at] = (float) re;
+                ((float *) v)[2 * at + 1] = (float) im;
+                break;
+            default:
+                ((double *) v)[2 * at] = re;
+                ((double *) v)[2 * at + 1] = im;
+                break;
+            }
+        }
+    }
+    bml_ellpack_hip_set_arrays(A, v, ix, nn, M);
+out:
+    if (f)
+        fclose(f);
+    free(v);
+    free(ix);
+    free(nn);
+}

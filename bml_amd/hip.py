@@ -28,6 +28,7 @@ ELLBLOCK_HIP = 7
 ERRORS_EXIT, ERRORS_RECORD = 0, 1
 SYNC_EACH_CALL, SYNC_DEFERRED = 0, 1
 ERR_M_OVERFLOW = 1
+ERR_ARGUMENT = 2
 
 
 class BmlError(RuntimeError):
@@ -96,6 +97,8 @@ _SIGS = {
     "bml_copy_ellpack_hip": (None, [_P, _P]),
     "bml_import_from_dense_ellpack_hip": (_P, [_I, _I, _I, _I, _P, _D, _I]),
     "bml_export_to_dense_ellpack_hip": (_P, [_P, _I]),
+    "bml_write_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
+    "bml_read_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
     "bml_ellpack_hip_set_arrays": (None, [_P, _P, _P, _P, _I]),
     "bml_ellpack_hip_get_arrays": (None, [_P, _P, _P, _P]),
     "bml_multiply_x2_ellpack_hip": (_P, [_P, _P, _D]),
@@ -311,6 +314,17 @@ def import_from_dense(dense: np.ndarray, M: int, threshold: float = 0.0,
     d = np.ascontiguousarray(dense)
     return EllpackHip(_call("bml_import_from_dense_ellpack_hip", precision_of(d.dtype), order,
                             d.shape[0], M, d.ctypes.data, threshold, 0))
+
+
+def write_matrix_market(A: EllpackHip, path: str) -> None:
+    """bml_write_bml_matrix for ellpack_hip: a Matrix Market coordinate file."""
+    _call("bml_write_bml_matrix_ellpack_hip", A.ptr, os.fsencode(path))
+
+
+def read_matrix_market(A: EllpackHip, path: str) -> EllpackHip:
+    """bml_read_bml_matrix for ellpack_hip: appends the file's entries to A."""
+    _call("bml_read_bml_matrix_ellpack_hip", A.ptr, os.fsencode(path))
+    return A
 
 
 def export_to_dense(A: EllpackHip, order: int = 0) -> np.ndarray:

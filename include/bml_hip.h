@@ -181,6 +181,12 @@ void bml_ellpack_hip_set_arrays(bml_matrix_ellpack_hip_t *A, const void *value,
                                 const int *index, const int *nnz, int src_M);
 void bml_ellpack_hip_get_arrays(const bml_matrix_ellpack_hip_t *A, void *value,
                                 int *index, int *nnz);
+/* bml_write_bml_matrix / bml_read_bml_matrix for ellpack (bml_utilities.h:38-45,
+ * ellpack/bml_utilities_ellpack_typed.c:22-190): Matrix Market coordinate
+ * files in the reference's text format; read appends the entries to A's
+ * rows (a "symmetric" file also stores the mirrors). */
+void bml_write_bml_matrix_ellpack_hip(const bml_matrix_ellpack_hip_t *A, const char *filename);
+void bml_read_bml_matrix_ellpack_hip(bml_matrix_ellpack_hip_t *A, const char *filename);
 
 /* ---- ELLPACK-R: the thresholded multiply path --------------------------- */
 

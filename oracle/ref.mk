@@ -21,7 +21,9 @@ CFLAGS_REF := -O3 -fopenmp -std=gnu99 -fPIC -w -DBML_COMPLEX -DNOBLAS \
 PRECS := SINGLE_REAL DOUBLE_REAL SINGLE_COMPLEX DOUBLE_COMPLEX
 
 # source lists, read from the reference's CMakeLists.txt files
-srclist = $(shell awk '/set\(SOURCES-/{f=1;next} f&&/\)/{f=0} f{print $$1}' $(1)/CMakeLists.txt)
+# (every *.c name inside each set(SOURCES-... ) list, the last one included,
+# which shares its line with the closing parenthesis)
+srclist = $(shell awk '/set\(SOURCES-/{f=1;next} f{for(i=1;i<=NF;i++) if ($$i ~ /[.]c/) {t=$$i; gsub(/[^A-Za-z0-9_.\/-]/,"",t); print t}} f&&/\)/{f=0}' $(1)/CMakeLists.txt)
 DIRS := C-interface C-interface/dense C-interface/ellpack C-interface/ellblock \
         C-interface/csr internal-blas
 ALLSRC := $(foreach d,$(DIRS),$(addprefix $(d)/,$(call srclist,$(SRC)/$(d))))

@@ -357,6 +357,18 @@ int main(int argc, char **argv)
         NEED(3);
         double *g = bml_gershgorin(read_ell(argv[2], 0));
         printf("gersh %a %a\n", g[0], g[1]);
+    } else if (!strcmp(op, "mm_write")) {
+        /* mm_write <in.ell> <out.mtx>: bml_write_bml_matrix */
+        NEED(4);
+        bml_write_bml_matrix(read_ell(argv[2], 0), argv[3]);
+    } else if (!strcmp(op, "mm_read")) {
+        /* mm_read <in.mtx> <precision> <N> <M> <out.ell>: bml_read_bml_matrix
+         * into a zero matrix */
+        NEED(7);
+        bml_matrix_ellpack_t *A = bml_zero_matrix(ellpack, (bml_matrix_precision_t) atoi(argv[3]),
+                                                  atoi(argv[4]), atoi(argv[5]), sequential);
+        bml_read_bml_matrix(A, argv[2]);
+        write_ell(argv[6], A);
     } else if (!strcmp(op, "eb_x2")) {
         NEED(6);
         bml_matrix_ellblock_t *X = read_blk(argv[2], 0);

@@ -21,6 +21,7 @@ DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 OUT = os.path.dirname(os.path.abspath(__file__))
 DTYPES = [np.float32, np.float64, np.complex64, np.complex128]
 TAG = {np.float32: "f32", np.float64: "f64", np.complex64: "c64", np.complex128: "c128"}
+PREC = {np.float32: 1, np.float64: 2, np.complex64: 3, np.complex128: 4}
 
 
 def run(*args):
@@ -216,12 +217,62 @@ def sp2_cases(tmp):
     return cases
 
 
+def text_bytes(path):
+    return np.frombuffer(open(path, "rb").read(), dtype=np.uint8)
+
+
+def mm_cases(tmp):
+    """Matrix Market files through bml_write_bml_matrix / bml_read_bml_matrix
+    (ellpack): the reference's text for a random matrix, what the reference
+    reads back from it, and what it reads from a "symmetric" file (lower
+    triangle listed, mirrors stored)."""
+    cases = {}
+    for dt in DTYPES:
+        t = TAG[dt]
+        N, M = 40, 12
+        A = G.random_ell(N, M, 6, 31, dt)
+        fa, fm, fo = (os.path.join(tmp, n) for n in ("a.ell", "a.mtx", "o.ell"))
+        A.write(fa)
+        d = {"N": N, "M": M}
+        pack_ell("A", A, d)
+        rc, _, _ = run("mm_write", fa, fm)
+        assert rc == 0
+        d["general_text"] = text_bytes(fm)
+        rc, _, _ = run("mm_read", fm, PREC[dt], N, M, fo)
+        assert rc == 0
+        pack_ell("general_read", Ell.read(fo), d)
+        # a symmetric file: the lower triangle of a random symmetric matrix
+        rng = np.random.default_rng(41)
+        cplx = np.issubdtype(np.dtype(dt), np.complexfloating)
+        lines, nz = [], 0
+        for i in range(N):
+            for j in rng.choice(i + 1, size=min(i + 1, 3), replace=False):
+                re, im = rng.uniform(-1, 1), rng.uniform(-1, 1)
+                lines.append(f"{i + 1} {j + 1} {re:.17g}" + (f" {im:.17g}" if cplx else ""))
+                nz += 1
+        kind = "complex" if cplx else "real"
+        text = f"%%MatrixMarket matrix coordinate {kind} symmetric\n{N} {N} {nz}\n" + \
+               "\n".join(lines) + "\n"
+        fs = os.path.join(tmp, "s.mtx")
+        open(fs, "w").write(text)
+        d["symmetric_text"] = text_bytes(fs)
+        rc, _, _ = run("mm_read", fs, PREC[dt], N, M, fo)
+        assert rc == 0
+        pack_ell("symmetric_read", Ell.read(fo), d)
+        cases[f"mm_{t}"] = d
+    return cases
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run make -C oracle ref")
+    # "mm": only the Matrix Market fixtures
+    only = sys.argv[1:]
     with tempfile.TemporaryDirectory() as tmp:
         for fname, fn in (("ellpack.npz", ellpack_cases), ("ellblock.npz", ellblock_cases),
-                          ("sp2.npz", sp2_cases)):
+                          ("sp2.npz", sp2_cases), ("mm.npz", mm_cases)):
+            if only and fname.split(".")[0] not in only:
+                continue
             cases = fn(tmp)
             flat = {f"{k}/{kk}": np.asarray(vv) for k, v in cases.items() for kk, vv in v.items()}
             np.savez_compressed(os.path.join(OUT, fname), **flat)
