@@ -686,3 +686,198 @@ void bml_normalize_ellblock_hip(bml_matrix_ellblock_hip_t *A, double mineval, do
     if (!bmlh_failed())
         bml_add_identity_ellblock_hip(A, maxeval / d, 0.0);
 }
+
+/* ---- Matrix Market files -------------------------------------------------- */
+
+/* bml_write_bml_matrix_ellblock (ellblock/bml_utilities_ellblock_typed.c:174-260):
+ * "%%%MatrixMarket matrix coordinate real general" ("complex" for complex
+ * precisions), "N N n" with n the elements of the stored blocks, then every
+ * element of every stored block (block rows in order, blocks in storage
+ * order, each block row-major, zeros included) as 1-based global row and
+ * column and %20.15e (complex: real and imaginary parts). */
+void bml_write_bml_matrix_ellblock_hip(const bml_matrix_ellblock_hip_t *A, const char *filename)
+{
+    const char *what = "bml_write_bml_matrix_ellblock_hip";
+    if (!valid_eb(A, what))
+        return;
+    const int NB = A->NB, MB = A->MB, bt = A->bt, prec = A->matrix_precision;
+    const int *bs = A->bsize_host;
+    const size_t es = bmlh_elem_size(prec), tile = (size_t) bt * bt;
+    char *v = malloc((size_t) NB * MB * tile * es + 1);
+    int *ix = malloc(((size_t) NB * MB + 1) * sizeof(int));
+    int *nn = malloc(((size_t) NB + 1) * sizeof(int));
+    int *off = malloc(((size_t) NB + 1) * sizeof(int));
+    FILE *f = NULL;
+    if (!v || !ix || !nn || !off) {
+        bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+        goto out;
+    }
+    bml_ellblock_hip_get_blocks(A, v, ix, nn, bt);
+    if (bmlh_failed())
+        goto out;
+    if (!(f = fopen(filename, "w"))) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: cannot open %s", what, filename);
+        goto out;
+    }
+    const int cplx = prec == BML_HIP_SINGLE_COMPLEX || prec == BML_HIP_DOUBLE_COMPLEX;
+    fprintf(f, "%%%%%%MatrixMarket matrix coordinate %s general\n", cplx ? "complex" : "real");
+    off[0] = 0;
+    for (int ib = 1; ib <= NB; ib++)
+        off[ib] = off[ib - 1] + bs[ib - 1];
+    long long tot = 0;
+    for (int ib = 0; ib < NB; ib++)
+        for (int jp = 0; jp < nn[ib]; jp++)
+            tot += (long long) bs[ib] * bs[ix[(size_t) ib * MB + jp]];
+    fprintf(f, "%d %d %lld\n", A->N, A->N, tot);
+    for (int ib = 0; ib < NB; ib++)
+        for (int jp = 0; jp < nn[ib]; jp++) {
+            const size_t slot = (size_t) ib * MB + jp;
+            const int jb = ix[slot];
+            for (int ii = 0; ii < bs[ib]; ii++)
+                for (int jj = 0; jj < bs[jb]; jj++) {
+                    const size_t at = slot * tile + (size_t) ii * bt + jj;
+                    const int r = off[ib] + 1 + ii, c = off[jb] + 1 + jj;
+                    switch (prec) {
+                    case BML_HIP_SINGLE_REAL:
+                        fprintf(f, "%d %d %20.15e\n", r, c, (double) ((const float *) v)[at]);
+                        break;
+                    case BML_HIP_DOUBLE_REAL:
+                        fprintf(f, "%d %d %20.15e\n", r, c, ((const double *) v)[at]);
+                        break;
+                    case BML_HIP_SINGLE_COMPLEX:
+                        fprintf(f, "%d %d %20.15e %20.15e\n", r, c,
+                                (double) ((const float *) v)[2 * at],
+                                (double) ((const float *) v)[2 * at + 1]);
+                        break;
+                    default:
+                        fprintf(f, "%d %d %20.15e %20.15e\n", r, c, ((const double *) v)[2 * at],
+                                ((const double *) v)[2 * at + 1]);
+                        break;
+                    }
+                }
+        }
+    if (fclose(f) != 0)
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: write to %s failed", what, filename);
+    f = NULL;
+out:
+    if (f)
+        fclose(f);
+    free(v);
+    free(ix);
+    free(nn);
+    free(off);
+}
+
+/* bml_read_bml_matrix_ellblock (ellblock/bml_utilities_ellblock_typed.c:24-165):
+ * each entry of the file goes to its block, which is appended to the block
+ * row (zeroed) the first time one of its elements appears.  A "symmetric"
+ * file also stores each off-diagonal element at its mirror.  The reference's
+ * own symmetric branch writes the new block's column at the wrong slot and
+ * never allocates the block, and it tests the in-block indices for the
+ * diagonal; this follows the general branch for the mirror instead.  A
+ * block row past MB is the M overflow (the reference asserts). */
+void bml_read_bml_matrix_ellblock_hip(bml_matrix_ellblock_hip_t *A, const char *filename)
+{
+    const char *what = "bml_read_bml_matrix_ellblock_hip";
+    if (!valid_eb(A, what))
+        return;
+    const int NB = A->NB, MB = A->MB, bt = A->bt, prec = A->matrix_precision, N = A->N;
+    const int *bs = A->bsize_host;
+    const int cplx = prec == BML_HIP_SINGLE_COMPLEX || prec == BML_HIP_DOUBLE_COMPLEX;
+    const size_t es = bmlh_elem_size(prec), tile = (size_t) bt * bt;
+    char *v = malloc((size_t) NB * MB * tile * es + 1);
+    int *ix = malloc(((size_t) NB * MB + 1) * sizeof(int));
+    int *nn = malloc(((size_t) NB + 1) * sizeof(int));
+    int *blk = malloc(((size_t) N + 1) * sizeof(int)), *in = malloc(((size_t) N + 1) * sizeof(int));
+    FILE *f = NULL;
+    if (!v || !ix || !nn || !blk || !in) {
+        bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+        goto out;
+    }
+    bml_ellblock_hip_get_blocks(A, v, ix, nn, bt);
+    if (bmlh_failed())
+        goto out;
+    for (int ib = 0, r = 0; ib < NB; ib++)
+        for (int ii = 0; ii < bs[ib]; ii++, r++) {
+            blk[r] = ib;
+            in[r] = ii;
+        }
+    if (!(f = fopen(filename, "r"))) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: cannot open %s", what, filename);
+        goto out;
+    }
+    char h[5][64];
+    int n1, n2, nz;
+    if (fscanf(f, "%63s %63s %63s %63s %63s", h[0], h[1], h[2], h[3], h[4]) != 5 ||
+        fscanf(f, "%d %d %d", &n1, &n2, &nz) != 3) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: %s: read error on the header", what, filename);
+        goto out;
+    }
+    if (n1 > N || n2 > N || nz < 0) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: %s is %d x %d, the matrix %d x %d", what,
+                   filename, n1, n2, N, N);
+        goto out;
+    }
+    const int sym = strcmp(h[4], "symmetric") == 0;
+    for (int e = 0; e < nz; e++) {
+        int r, c, got;
+        double re = 0.0, im = 0.0;
+        float fv = 0.0f;
+        if (prec == BML_HIP_SINGLE_REAL)
+            got = fscanf(f, "%d %d %e", &r, &c, &fv) == 3;
+        else if (!cplx)
+            got = fscanf(f, "%d %d %le", &r, &c, &re) == 3;
+        else
+            got = fscanf(f, "%d %d %le %le", &r, &c, &re, &im) == 4;
+        if (!got) {
+            bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: %s: read error at entry %d", what, filename, e);
+            goto out;
+        }
+        r--;
+        c--;
+        if (r < 0 || r >= N || c < 0 || c >= N) {
+            bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: %s: entry %d (%d, %d) out of range", what,
+                       filename, e, r + 1, c + 1);
+            goto out;
+        }
+        for (int pass = 0; pass < (sym && c != r ? 2 : 1); pass++) {
+            const int row = pass ? c : r, col = pass ? r : c;
+            const int ib = blk[row], jb = blk[col];
+            int jp = 0;
+            while (jp < nn[ib] && ix[(size_t) ib * MB + jp] != jb)
+                jp++;
+            if (jp == nn[ib]) {
+                if (nn[ib] >= MB) {
+                    bmlh_error(BML_HIP_ERR_M_OVERFLOW,
+                               "%s: block row %d: Number of non-zeroes per row > M", what, ib);
+                    goto out;
+                }
+                ix[(size_t) ib * MB + jp] = jb;
+                memset(v + ((size_t) ib * MB + jp) * tile * es, 0, tile * es);
+                nn[ib]++;
+            }
+            const size_t at = ((size_t) ib * MB + jp) * tile + (size_t) in[row] * bt + in[col];
+            switch (prec) {
+            case BML_HIP_SINGLE_REAL: ((float *) v)[at] = fv; break;
+            case BML_HIP_DOUBLE_REAL: ((double *) v)[at] = re; break;
+            case BML_HIP_SINGLE_COMPLEX:
+                ((float *) v)[2 * at] = (float) re;
+                ((float *) v)[2 * at + 1] = (float) im;
+                break;
+            default:
+                ((double *) v)[2 * at] = re;
+                ((double *) v)[2 * at + 1] = im;
+                break;
+            }
+        }
+    }
+    bml_ellblock_hip_set_blocks(A, v, ix, nn, MB, bt);
+out:
+    if (f)
+        fclose(f);
+    free(v);
+    free(ix);
+    free(nn);
+    free(blk);
+    free(in);
+}

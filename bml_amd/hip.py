@@ -99,6 +99,8 @@ _SIGS = {
     "bml_export_to_dense_ellpack_hip": (_P, [_P, _I]),
     "bml_write_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
     "bml_read_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
+    "bml_write_bml_matrix_ellblock_hip": (None, [_P, C.c_char_p]),
+    "bml_read_bml_matrix_ellblock_hip": (None, [_P, C.c_char_p]),
     "bml_ellpack_hip_set_arrays": (None, [_P, _P, _P, _P, _I]),
     "bml_ellpack_hip_get_arrays": (None, [_P, _P, _P, _P]),
     "bml_multiply_x2_ellpack_hip": (_P, [_P, _P, _D]),
@@ -316,14 +318,18 @@ def import_from_dense(dense: np.ndarray, M: int, threshold: float = 0.0,
                             d.shape[0], M, d.ctypes.data, threshold, 0))
 
 
-def write_matrix_market(A: EllpackHip, path: str) -> None:
-    """bml_write_bml_matrix for ellpack_hip: a Matrix Market coordinate file."""
-    _call("bml_write_bml_matrix_ellpack_hip", A.ptr, os.fsencode(path))
+def write_matrix_market(A, path: str) -> None:
+    """bml_write_bml_matrix for ellpack_hip / ellblock_hip: a Matrix Market
+    coordinate file."""
+    kind = "ellblock" if isinstance(A, EllblockHip) else "ellpack"
+    _call(f"bml_write_bml_matrix_{kind}_hip", A.ptr, os.fsencode(path))
 
 
-def read_matrix_market(A: EllpackHip, path: str) -> EllpackHip:
-    """bml_read_bml_matrix for ellpack_hip: appends the file's entries to A."""
-    _call("bml_read_bml_matrix_ellpack_hip", A.ptr, os.fsencode(path))
+def read_matrix_market(A, path: str):
+    """bml_read_bml_matrix for ellpack_hip / ellblock_hip: adds the file's
+    entries to A."""
+    kind = "ellblock" if isinstance(A, EllblockHip) else "ellpack"
+    _call(f"bml_read_bml_matrix_{kind}_hip", A.ptr, os.fsencode(path))
     return A
 
 

@@ -301,6 +301,12 @@ void bml_ellblock_hip_set_blocks(bml_matrix_ellblock_hip_t *A, const void *value
                                  const int *indexb, const int *nnzb, int src_MB, int bmax);
 void bml_ellblock_hip_get_blocks(const bml_matrix_ellblock_hip_t *A, void *value,
                                  int *indexb, int *nnzb, int bmax);
+/* bml_write_bml_matrix / bml_read_bml_matrix for ellblock (bml_utilities.h:38-45,
+ * ellblock/bml_utilities_ellblock_typed.c:24-260): Matrix Market coordinate
+ * files in the reference's text format (every element of every stored
+ * block); read adds each entry to its block, appending blocks as needed. */
+void bml_write_bml_matrix_ellblock_hip(const bml_matrix_ellblock_hip_t *A, const char *filename);
+void bml_read_bml_matrix_ellblock_hip(bml_matrix_ellblock_hip_t *A, const char *filename);
 /* bml_multiply_x2_ellblock (ellblock/bml_multiply_ellblock_typed.c:246-442):
  * X2 <- X*X blockwise; a product block is kept iff it is the diagonal block
  * or its max |element| > threshold (no elementwise threshold); fatal if a

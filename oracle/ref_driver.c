@@ -369,6 +369,19 @@ int main(int argc, char **argv)
                                                   atoi(argv[4]), atoi(argv[5]), sequential);
         bml_read_bml_matrix(A, argv[2]);
         write_ell(argv[6], A);
+    } else if (!strcmp(op, "eb_mm_write")) {
+        /* eb_mm_write <in.blk> <out.mtx>: bml_write_bml_matrix */
+        NEED(4);
+        bml_write_bml_matrix(read_blk(argv[2], 0), argv[3]);
+    } else if (!strcmp(op, "eb_mm_read")) {
+        /* eb_mm_read <in.mtx> <shape.blk> <out.blk>: bml_read_bml_matrix into
+         * a zero matrix of shape.blk's precision, block sizes and MB */
+        NEED(5);
+        bml_matrix_ellblock_t *S = read_blk(argv[3], 0);
+        bml_matrix_ellblock_t *A = bml_block_matrix(ellblock, S->matrix_precision, S->NB,
+                                                    S->MB, S->N, S->bsize, sequential);
+        bml_read_bml_matrix(A, argv[2]);
+        write_blk(argv[4], A);
     } else if (!strcmp(op, "eb_x2")) {
         NEED(6);
         bml_matrix_ellblock_t *X = read_blk(argv[2], 0);

@@ -260,6 +260,20 @@ def mm_cases(tmp):
         assert rc == 0
         pack_ell("symmetric_read", Ell.read(fo), d)
         cases[f"mm_{t}"] = d
+        # ellblock: every element of every stored block, and the read back
+        bsize = [3, 5, 2, 4, 1, 6, 3, 2]
+        B = G.random_blk(bsize, 5, 3, 37, dt)
+        fb, fbm, fbo = (os.path.join(tmp, n) for n in ("b.blk", "b.mtx", "o.blk"))
+        B.write(fb)
+        e = {}
+        pack_blk("A", B, e)
+        rc, _, _ = run("eb_mm_write", fb, fbm)
+        assert rc == 0
+        e["general_text"] = text_bytes(fbm)
+        rc, _, _ = run("eb_mm_read", fbm, fb, fbo)
+        assert rc == 0
+        pack_blk("general_read", Blk.read(fbo), e)
+        cases[f"ebmm_{t}"] = e
     return cases
 
 
