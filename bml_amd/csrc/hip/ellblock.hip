@@ -912,12 +912,19 @@ struct EbAddParams {
 #define BMLHIP_EB_ADD_WAVES 4
 #endif
 constexpr int EB_ADD_WAVES = BMLHIP_EB_ADD_WAVES;
+// union entries per wavefront per round: 2 (more loads in flight, 178
+// VGPRs in FP64) ran the FP64 SP2 solve of the 16x16x8-site, 32-orbital lattice at 22.3 solves/s against 25.3
+// with 1
+#ifndef BMLHIP_EB_ADD_PER
+#define BMLHIP_EB_ADD_PER 1
+#endif
+constexpr int EB_ADD_PER = BMLHIP_EB_ADD_PER;
 
 template <typename T, int BT>
 __global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
     using E = Elem<T>;
     extern __shared__ int usm[];
-    __shared__ int keep_s[EB_ADD_WAVES];
+    __shared__ int keep_s[EB_ADD_WAVES * EB_ADD_PER];
     __shared__ double nrm_s[EB_ADD_WAVES];
     if (p.skip && *p.skip) return;
     const int ib = blockIdx.x;
@@ -962,52 +969,62 @@ __global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
     constexpr int Q = (BT * BT + 63) / 64;
     int ll = 0;
     double nrm2 = 0.0;
-    for (int u0 = 0; u0 < n; u0 += EB_ADD_WAVES) {
-        const int u = u0 + w;
-        const bool live = u < n;
-        const int jb = live ? ucol[u] : 0, sa = live ? ua[u] : -1, sb = live ? ub[u] : -1;
-        const int bj = p.bsize[jb];
-        T x[Q];
-        double m = 0.0;
+    // each round: EB_ADD_PER union entries per wavefront, entry u0 + r with
+    // r = k EB_ADD_WAVES + w; all their tiles are read (and loads of the
+    // round's entries overlap) before any kept tile is written
+    for (int u0 = 0; u0 < n; u0 += EB_ADD_WAVES * EB_ADD_PER) {
+        T x[EB_ADD_PER][Q];
+        bool keep[EB_ADD_PER];
+        int jbk[EB_ADD_PER];
 #pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int e = lane + 64 * q;
-            x[q] = E::zero();
-            T y = E::zero();
-            if (e < BT * BT && e / BT < bi && e % BT < bj) {
-                if (sa >= 0) {
-                    const T a = Av[((size_t) ib * p.MA + sa) * B2 + e];
-                    x[q] = E::axpy(x[q], alpha, a);
-                    y = a;
+        for (int k = 0; k < EB_ADD_PER; k++) {
+            const int u = u0 + k * EB_ADD_WAVES + w;
+            const bool live = u < n;
+            const int jb = live ? ucol[u] : 0, sa = live ? ua[u] : -1, sb = live ? ub[u] : -1;
+            const int bj = p.bsize[jb];
+            jbk[k] = jb;
+            double m = 0.0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int e = lane + 64 * q;
+                x[k][q] = E::zero();
+                T y = E::zero();
+                if (e < BT * BT && e / BT < bi && e % BT < bj) {
+                    if (sa >= 0) {
+                        const T a = Av[((size_t) ib * p.MA + sa) * B2 + e];
+                        x[k][q] = E::axpy(x[k][q], alpha, a);
+                        y = a;
+                    }
+                    if (sb >= 0) {
+                        const T b = Bv[((size_t) ib * p.MB + sb) * B2 + e];
+                        x[k][q] = E::axpy(x[k][q], beta, b);
+                        y = E::sub(y, b);
+                    }
                 }
-                if (sb >= 0) {
-                    const T b = Bv[((size_t) ib * p.MB + sb) * B2 + e];
-                    x[q] = E::axpy(x[q], beta, b);
-                    y = E::sub(y, b);
-                }
+                const double mg = E::mag(x[k][q]);
+                m = mg > m ? mg : m;
+                if (p.mode == 2) nrm2 += E::abs2(y);
             }
-            const double mg = E::mag(x[q]);
-            m = mg > m ? mg : m;
-            if (p.mode == 2) nrm2 += E::abs2(y);
+            m = wave_max(m);
+            keep[k] = live && E::above(E::from_real(m), p.thr);
+            if (lane == 0) keep_s[k * EB_ADD_WAVES + w] = keep[k] ? 1 : 0;
         }
-        m = wave_max(m);
-        const bool keep = live && E::above(E::from_real(m), p.thr);
-        if (lane == 0) keep_s[w] = keep ? 1 : 0;
         __syncthreads();  // every tile of the round read; keep flags in
-        int pos = ll, tot = 0;
-        for (int v = 0; v < EB_ADD_WAVES; v++) {
-            pos += v < w ? keep_s[v] : 0;
-            tot += keep_s[v];
-        }
-        if (keep) {
+        int tot = 0;
+        for (int r = 0; r < EB_ADD_WAVES * EB_ADD_PER; r++) tot += keep_s[r];
+#pragma unroll
+        for (int k = 0; k < EB_ADD_PER; k++) {
+            if (!keep[k]) continue;
+            int pos = ll;
+            for (int r = 0; r < k * EB_ADD_WAVES + w; r++) pos += keep_s[r];
             if (pos < p.MA) {
                 const size_t d = ((size_t) ib * p.MA + pos) * B2;
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
                     const int e = lane + 64 * q;
-                    if (e < BT * BT) Av[d + e] = x[q];
+                    if (e < BT * BT) Av[d + e] = x[k][q];
                 }
-                if (lane == 0) p.Ai[(size_t) ib * p.MA + pos] = jb;
+                if (lane == 0) p.Ai[(size_t) ib * p.MA + pos] = jbk[k];
             } else if (lane == 0 && pos == p.MA) {
                 atomicAdd(&p.st->m_overflow_rows, 1);
                 atomicMin(&p.st->first_m_overflow, ib);
