@@ -913,8 +913,8 @@ struct EbAddParams {
 #endif
 constexpr int EB_ADD_WAVES = BMLHIP_EB_ADD_WAVES;
 // union entries per wavefront per round: 2 (more loads in flight, 178
-// VGPRs in FP64) ran the FP64 SP2 solve of the 16x16x8-site, 32-orbital lattice at 22.3 solves/s against 25.3
-// with 1
+// VGPRs in FP64) ran the FP64 SP2 solve of the 16x16x8-site, 32-orbital
+// lattice at 22.3 solves/s against 25.3 with 1
 #ifndef BMLHIP_EB_ADD_PER
 #define BMLHIP_EB_ADD_PER 1
 #endif
