@@ -88,8 +88,10 @@ def test_multiply_ab_unordered(unordered, dtype):
 
 
 def test_sp2_unordered_matches_exact(hip):
-    """SP2 on the 12^3 lattice: the unordered engine's solve takes the same
-    branches and lands on the exact engine's density matrix."""
+    """SP2 on the 12^3 lattice: the unordered engine's solve lands on the
+    exact engine's density matrix (its sums round in arrival order, so the
+    stopping iteration may differ by one when the idempotency meets the
+    tolerance within rounding)."""
     H = G.lattice_hamiltonian(12, 12, 12, M=16, hopping=-0.1, disorder=0.01)
     dH = hip.from_ell(H)
     out = {}
@@ -102,12 +104,17 @@ def test_sp2_unordered_matches_exact(hip):
             hip.set_row_engine(hip.ROW_ENGINE_EXACT)
         out[eng] = (r, X.get().to_dense())
     (re_, De), (ru, Du) = out[hip.ROW_ENGINE_EXACT], out[hip.ROW_ENGINE_UNORDERED]
-    assert ru.iterations == re_.iterations and ru.converged == re_.converged
+    assert abs(ru.iterations - re_.iterations) <= 1 and ru.converged == re_.converged
     assert abs(ru.trace_x - re_.trace_x) <= 1e-9
     assert np.abs(Du - De).max() <= 1e-9
 
 
 def test_batch_unordered_matches_exact(hip):
+    """Batched SP2 on both engines.  The unordered engine's sums round in
+    arrival order, so a system whose idempotency reaches the tolerance within
+    rounding of it may stop one iteration apart; the density matrices agree
+    to 1e-9 either way (one more step moves a converged X by about the
+    tolerance)."""
     shapes = [(4, 4, 4), (6, 6, 4), (8, 8, 8), (6, 6, 6), (10, 8, 6)]
     Hs = [G.lattice_hamiltonian(*L, M=16, hopping=-0.1, disorder=0.01, seed=40 + k)
           for k, L in enumerate(shapes)]
@@ -124,7 +131,7 @@ def test_batch_unordered_matches_exact(hip):
         res[eng] = (r, [X.get().to_dense() for X in Xs])
     (re_, De), (ru, Du) = res[hip.ROW_ENGINE_EXACT], res[hip.ROW_ENGINE_UNORDERED]
     for a, b, da, db in zip(re_, ru, De, Du):
-        assert a.iterations == b.iterations and a.converged == b.converged
+        assert abs(a.iterations - b.iterations) <= 1 and a.converged == b.converged
         assert np.abs(da - db).max() <= 1e-9
 
 
@@ -171,7 +178,7 @@ def test_sp2_complex_unordered_matches_exact(hip):
             hip.set_row_engine(hip.ROW_ENGINE_EXACT)
         out[eng] = (r, X.get().to_dense())
     (re_, De), (ru, Du) = out[hip.ROW_ENGINE_EXACT], out[hip.ROW_ENGINE_UNORDERED]
-    assert ru.iterations == re_.iterations
+    assert abs(ru.iterations - re_.iterations) <= 1  # rounding at the tolerance
     assert np.abs(Du - De).max() <= 1e-9
 
 
@@ -193,7 +200,7 @@ def test_batch_unordered_graph_matches_eager(hip):
     finally:
         hip.set_row_engine(hip.ROW_ENGINE_EXACT)
     for a, b, da, db in zip(outs[False][0], outs[True][0], outs[False][1], outs[True][1]):
-        assert a.iterations == b.iterations
+        assert abs(a.iterations - b.iterations) <= 1  # rounding at the tolerance
         assert np.abs(da - db).max() <= 1e-9
 
 
