@@ -13,12 +13,16 @@
 //                 hash; a ballot prefix count ranks the new ones), counts the
 //                 pairs per slot, and writes the pairs grouped by slot (jp
 //                 order kept within a slot: a stable counting sort).
-//  k_eb_numeric   four wavefronts per block row: each takes one slot per
-//                 round, accumulates its pairs' tile products in registers
-//                 (exact VALU dot products in the reference's order, or
-//                 MFMA), takes the block's max |element|, and the four
-//                 decide keep/drop together so the kept blocks are stored
-//                 in slot order (the reference's output order).
+//  k_eb_numeric   four wavefronts per block row claim the row's slots (MFMA
+//                 engines: batches of whole slots) from an LDS counter,
+//                 accumulate each slot's tile products in registers (exact
+//                 VALU dot products in the reference's order, or MFMA),
+//                 take the block's max |element| and store a kept tile at
+//                 its slot's position; the kept tiles are then compacted
+//                 into slot order (the reference's output order).  For the
+//                 x2 of an exactly symmetric X each block row forms only
+//                 the blocks it owns and mirrors them (k_eb_compact then
+//                 compacts).
 //
 // Keep rule (bml_multiply_ellblock_typed.c:406-408): the diagonal block, or
 // max |element| > threshold, compared in double.  A block row touching more
