@@ -376,3 +376,42 @@ def test_mfma_x2_slots_longer_than_a_unit(hip, monkeypatch, sym):
     assert rel_close(tr[1], np.trace(ref), 1e-12)
     if sym:
         assert np.array_equal(got, got.T)
+
+
+def test_mfma_symmetric_x2_ragged_blocks_and_empty_rows(hip, monkeypatch):
+    """The symmetric path on block sizes 1..32 (zero-padded 32 x 32 tiles)
+    with empty block rows: the dense product, exactly symmetric."""
+    bsize = [32, 20, 7, 32, 1, 16, 32, 9, 25, 32]
+    NB = len(bsize)
+    off = np.concatenate([[0], np.cumsum(bsize)])
+    N = int(off[-1])
+    rng = np.random.default_rng(3)
+    D = np.zeros((N, N))
+    empty = {2, 6}
+    for ib in range(NB):
+        for jb in range(ib, NB):
+            if ib in empty or jb in empty or (jb != ib and rng.uniform() < 0.5):
+                continue
+            blk = rng.uniform(-1, 1, (bsize[ib], bsize[jb]))
+            D[off[ib]:off[ib + 1], off[jb]:off[jb + 1]] = blk
+            D[off[jb]:off[jb + 1], off[ib]:off[ib + 1]] = blk.T
+    for ib in range(NB):  # symmetric diagonal blocks
+        sl = slice(off[ib], off[ib + 1])
+        D[sl, sl] = np.triu(D[sl, sl]) + np.triu(D[sl, sl], 1).T
+    H = Blk.from_dense(D, bsize, NB, bmax=32)
+    for ib in empty:
+        assert H.nnzb[ib] == 1  # from_dense keeps the (zero) diagonal block
+    H.nnzb[list(empty)] = 0     # make those block rows really empty
+    D2 = H.to_dense()
+    assert np.array_equal(D2, D2.T)
+    monkeypatch.setenv("BML_HIP_ASSUME_SYMMETRIC", "1")
+    dH = hip.from_blk(H)
+    assert eb_is_symmetric(hip, dH)
+    X2 = hip.block_matrix(H.precision, H.bsize, NB)
+    tr = hip.eb_multiply_x2(dH, X2, 0.0)
+    ref = D2 @ D2
+    got = hip.eb_export_to_dense(X2)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(got, got.T)
+    assert rel_close(tr[1], np.trace(ref), 1e-12)
+    assert all(int(n) == 0 for n in np.asarray(X2.get().nnzb)[list(empty)])
