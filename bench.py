@@ -175,7 +175,10 @@ def ellblock_sp2(hip, G):
     dH = hip.from_blk(H, MB)
     out = {"workload": "SP2 on ELLBLOCK, 16x16x8 sites x 32 orbitals (N=65536, 32x32 blocks), "
                        "onsite shift +-6, coupling 0.02, MB=160, block threshold 1e-5, "
-                       "stop at ||X^2-X||_F<=1e-8 or 60 iterations", "N": H.N}
+                       "stop at ||X^2-X||_F<=1e-8 or 60 iterations; H is exactly symmetric, so "
+                       "each X^2 forms about half of its block pairs and mirrors the rest "
+                       "(block_pairs_per_launch and the MFMA rates count the pairs formed)",
+           "N": H.N}
     runs = [("fp64", hip.BLOCK_ENGINE_MFMA, False), ("mixed_fp32", hip.BLOCK_ENGINE_MFMA, True),
             ("mixed_bf16", hip.BLOCK_ENGINE_MFMA_BF16, True)]
     for name, engine, mixed in runs:
