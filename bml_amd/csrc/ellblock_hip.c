@@ -9,6 +9,7 @@
  */
 #include "bml_hip_host.h"
 
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -880,4 +881,98 @@ out:
     free(nn);
     free(blk);
     free(in);
+}
+
+/* bml_import_from_dense_ellblock (ellblock/bml_import_ellblock_typed.c:20-96):
+ * a zero matrix with the registered block sizes, then every block (ib, jb)
+ * of the dense N x N array (row- or column-major), block columns in order,
+ * kept iff its max |element| (bml_norm_inf, bml_utilities_ellblock_typed.c:
+ * 260-277: fabs/cabs, as fabs/hypot, in the element's precision, compared
+ * in double) is
+ * above the threshold; the diagonal block is not forced.  A block row past
+ * MB is the M overflow (the reference would overrun it). */
+bml_matrix_ellblock_hip_t *bml_import_from_dense_ellblock_hip(int precision, int order, int N,
+                                                             const void *A, double threshold,
+                                                             int M, int distrib_mode)
+{
+    const char *what = "bml_import_from_dense_ellblock_hip";
+    const size_t es = bmlh_elem_size(precision);
+    if (!es || !A || (order != 0 && order != 1)) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: bad precision %d, order %d or array", what,
+                   precision, order);
+        return NULL;
+    }
+    bml_matrix_ellblock_hip_t *B = bml_zero_matrix_ellblock_hip(precision, N, M, distrib_mode);
+    if (!B)
+        return NULL;
+    const int NB = B->NB, MB = B->MB, bt = B->bt;
+    const int *bs = B->bsize_host;
+    int tot = 0;
+    for (int ib = 0; ib < NB; ib++)
+        tot += bs[ib];
+    if (tot != N) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: registered block sizes cover %d rows, not %d",
+                   what, tot, N);
+        bml_deallocate_ellblock_hip(B);
+        return NULL;
+    }
+    const size_t tile = (size_t) bt * bt;
+    char *v = calloc((size_t) NB * MB * tile + 1, es);
+    int *ix = calloc((size_t) NB * MB + 1, sizeof(int));
+    int *nn = calloc((size_t) NB + 1, sizeof(int));
+    if (!v || !ix || !nn) {
+        bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+        goto fail;
+    }
+    for (int ib = 0, r0 = 0; ib < NB; r0 += bs[ib], ib++)
+        for (int jb = 0, c0 = 0; jb < NB; c0 += bs[jb], jb++) {
+            double norm = 0.0;
+            for (int ii = 0; ii < bs[ib]; ii++)
+                for (int jj = 0; jj < bs[jb]; jj++) {
+                    const size_t at = order == 0 ? (size_t) (r0 + ii) * N + (c0 + jj)
+                                                 : (size_t) (c0 + jj) * N + (r0 + ii);
+                    double a;
+                    switch (precision) {
+                    case BML_HIP_SINGLE_REAL: a = (double) fabsf(((const float *) A)[at]); break;
+                    case BML_HIP_DOUBLE_REAL: a = fabs(((const double *) A)[at]); break;
+                    case BML_HIP_SINGLE_COMPLEX: /* cabsf: glibc's is hypotf */
+                        a = (double) hypotf(((const float *) A)[2 * at], ((const float *) A)[2 * at + 1]);
+                        break;
+                    default: /* cabs: hypot */
+                        a = hypot(((const double *) A)[2 * at], ((const double *) A)[2 * at + 1]);
+                        break;
+                    }
+                    if (a > norm)
+                        norm = a;
+                }
+            if (!(norm > threshold))
+                continue;
+            if (nn[ib] >= MB) {
+                bmlh_error(BML_HIP_ERR_M_OVERFLOW,
+                           "%s: block row %d: Number of non-zeroes per row > M", what, ib);
+                goto fail;
+            }
+            const size_t slot = (size_t) ib * MB + nn[ib]++;
+            ix[slot] = jb;
+            for (int ii = 0; ii < bs[ib]; ii++)
+                for (int jj = 0; jj < bs[jb]; jj++) {
+                    const size_t at = order == 0 ? (size_t) (r0 + ii) * N + (c0 + jj)
+                                                 : (size_t) (c0 + jj) * N + (r0 + ii);
+                    memcpy(v + (slot * tile + (size_t) ii * bt + jj) * es,
+                           (const char *) A + at * es, es);
+                }
+        }
+    bml_ellblock_hip_set_blocks(B, v, ix, nn, MB, bt);
+    if (bmlh_failed())
+        goto fail;
+    free(v);
+    free(ix);
+    free(nn);
+    return B;
+fail:
+    free(v);
+    free(ix);
+    free(nn);
+    bml_deallocate_ellblock_hip(B);
+    return NULL;
 }

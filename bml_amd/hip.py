@@ -141,6 +141,7 @@ _SIGS.update({
     "bml_ellblock_hip_set_blocks": (None, [_P, _P, _P, _P, _I, _I]),
     "bml_ellblock_hip_get_blocks": (None, [_P, _P, _P, _P, _I]),
     "bml_export_to_dense_ellblock_hip": (_P, [_P, _I]),
+    "bml_import_from_dense_ellblock_hip": (_P, [_I, _I, _I, _P, C.c_double, _I, _I]),
     "bml_multiply_x2_ellblock_hip": (_P, [_P, _P, _D]),
     "bml_multiply_ellblock_hip": (None, [_P, _P, _P, _D, _D, _D]),
     "bml_multiply_AB_ellblock_hip": (None, [_P, _P, _P, _D]),
@@ -562,6 +563,22 @@ def block_matrix(precision: int, bsize, MB: int, M: int = 0) -> EllblockHip:
 
 def from_blk(A: Blk, MB: int | None = None) -> EllblockHip:
     return block_matrix(A.precision, A.bsize, MB or A.MB).set(A)
+
+
+def eb_import_from_dense(dense: np.ndarray, M: int, threshold: float = 0.0,
+                         order: int = 0) -> EllblockHip:
+    """bml_import_from_dense for ellblock_hip (the registered block sizes)."""
+    d = np.ascontiguousarray(dense)
+    return EllblockHip(_call("bml_import_from_dense_ellblock_hip", precision_of(d.dtype), order,
+                             d.shape[0], d.ctypes.data, threshold, M, 0))
+
+
+def block_sizes(N: int, M: int) -> np.ndarray:
+    """The registered block sizes (registering the default for N, M when
+    none are set)."""
+    p = _call("bml_hip_get_block_sizes", N, M)
+    nb = _call("bml_hip_get_nb")
+    return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_int)), shape=(nb,)).copy()
 
 
 def eb_export_to_dense(A: EllblockHip, order: int = 0) -> np.ndarray:

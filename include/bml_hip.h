@@ -344,6 +344,13 @@ void bml_scale_inplace_ellblock_hip(const void *scale_factor, bml_matrix_ellbloc
 /* bml_add_identity_ellblock (ellblock/bml_add_ellblock_typed.c:354-370):
  * A <- A + beta*I through the blockwise add. */
 void bml_add_identity_ellblock_hip(bml_matrix_ellblock_hip_t *A, double beta, double threshold);
+/* bml_import_from_dense_ellblock (ellblock/bml_import_ellblock_typed.c:20-96):
+ * a matrix with the registered block sizes (which must cover N rows) holding
+ * the blocks of the dense N x N array whose max |element| exceeds threshold,
+ * in block-column order; order 0 row-major, 1 column-major. */
+bml_matrix_ellblock_hip_t *bml_import_from_dense_ellblock_hip(int precision, int order, int N,
+                                                             const void *A, double threshold,
+                                                             int M, int distrib_mode);
 /* bml_export_to_dense_ellblock (ellblock/bml_export_ellblock_typed.c):
  * returns a malloc'd N*N host array. */
 void *bml_export_to_dense_ellblock_hip(const bml_matrix_ellblock_hip_t *A, int order);

@@ -415,3 +415,40 @@ def test_mfma_symmetric_x2_ragged_blocks_and_empty_rows(hip, monkeypatch):
     assert np.array_equal(got, got.T)
     assert rel_close(tr[1], np.trace(ref), 1e-12)
     assert all(int(n) == 0 for n in np.asarray(X2.get().nnzb)[list(empty)])
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("order", [0, 1])
+def test_import_from_dense(hip, dtype, order):
+    """bml_import_from_dense for ellblock (ellblock/bml_import_ellblock_typed.c:
+    20-96): the registered block sizes; blocks in block-column order, kept
+    iff max |element| > threshold (bml_norm_inf), the diagonal block not
+    forced.  The registry is set once per process, so N (and the block
+    slots per row) follow it; a block row with more kept blocks than slots
+    is the M overflow."""
+    bs = hip.block_sizes(40, 40)
+    MB = hip._call("bml_hip_get_mb")
+    N = int(bs.sum())
+    off = np.concatenate([[0], np.cumsum(bs)])
+    rng = np.random.default_rng(9)
+    D = rng.uniform(-1, 1, (N, N)) * (rng.uniform(0, 1, (N, N)) < 0.2)
+    if np.issubdtype(np.dtype(dtype), np.complexfloating):
+        D = D + 1j * rng.uniform(-1, 1, (N, N)) * (D != 0)
+    D = D.astype(dtype)
+    thr = 0.6
+    arr = D if order == 0 else np.ascontiguousarray(D.T)  # D stored column-major
+    want = [[jb for jb in range(len(bs))
+             if float(np.abs(D[off[ib]:off[ib + 1], off[jb]:off[jb + 1]]).max(initial=0.0)) > thr]
+            for ib in range(len(bs))]
+    if max(len(w) for w in want) > MB:
+        with pytest.raises(hip.BmlError) as e:
+            hip.eb_import_from_dense(arr, MB, thr, order)
+        assert e.value.code == hip.ERR_M_OVERFLOW
+        return
+    got = hip.eb_import_from_dense(arr, MB, thr, order).get(int(bs.max()))
+    for ib in range(len(bs)):
+        n = int(got.nnzb[ib])
+        assert got.indexb[ib, :n].tolist() == want[ib]
+        for jp, jb in enumerate(want[ib]):
+            blk = D[off[ib]:off[ib + 1], off[jb]:off[jb + 1]]
+            np.testing.assert_array_equal(got.value[ib, jp, :blk.shape[0], :blk.shape[1]], blk)
