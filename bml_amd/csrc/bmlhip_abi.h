@@ -128,7 +128,7 @@ int bmlhip_add(int prec, int N, int MA, void *d_Av, int *d_Ai, int *d_An,
                double alpha, double beta, double threshold, int mode);
 int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *d_skip);
 /* *symmetric = 1 when every stored (i, j, v) has (j, i, conj(v)) stored,
- * bit for bit (synchronous; not for use under graph capture) */
+ * equal by value, +0.0 == -0.0 (synchronous; not for use under graph capture) */
 int bmlhip_is_symmetric(int prec, int N, int M, const void *d_Av, const int *d_Ai,
                         const int *d_An, int *symmetric);
 /* as bmlhip_add with alpha = d_ab[0], beta = d_ab[1] read on the device and

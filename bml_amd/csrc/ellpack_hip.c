@@ -10,6 +10,7 @@
 #include "bml_hip_tune.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -67,6 +68,12 @@ static bml_matrix_ellpack_hip_t *alloc_ell(int prec, int N, int M, int mode, int
     }
     if (N < 0 || M <= 0) {
         bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: bad shape N=%d M=%d", what, N, M);
+        return NULL;
+    }
+    /* the row kernels address slot i*M + jp in 32-bit unsigned arithmetic */
+    if ((unsigned long long) N * (unsigned long long) M > (unsigned long long) UINT32_MAX) {
+        bmlh_error(BML_HIP_ERR_UNSUPPORTED, "%s: N*M = %llu slots exceeds 2^32 - 1", what,
+                   (unsigned long long) N * (unsigned long long) M);
         return NULL;
     }
     bml_matrix_ellpack_hip_t *A = calloc(1, sizeof *A);
@@ -282,9 +289,8 @@ void bml_ellpack_hip_get_arrays(const bml_matrix_ellpack_hip_t *A, void *value, 
 }
 
 bml_matrix_ellpack_hip_t *bml_import_from_dense_ellpack_hip(int precision, int order, int N,
-                                                            int M, const void *A,
-                                                            double threshold,
-                                                            int distrib_mode)
+                                                            const void *A, double threshold,
+                                                            int M, int distrib_mode)
 {
     const char *what = "bml_import_from_dense_ellpack_hip";
     size_t es = bmlh_elem_size(precision);

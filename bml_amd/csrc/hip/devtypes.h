@@ -166,6 +166,14 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
     return xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
 }
 
+// Value equality (==, so +0.0 equals -0.0; complex by component): the
+// symmetry tests compare a stored entry with the conjugate of its mirror,
+// and conj turns a +0.0 imaginary part into -0.0.
+__device__ __forceinline__ bool same_value(float a, float b) { return a == b; }
+__device__ __forceinline__ bool same_value(double a, double b) { return a == b; }
+__device__ __forceinline__ bool same_value(cf32 a, cf32 b) { return a.x == b.x && a.y == b.y; }
+__device__ __forceinline__ bool same_value(cf64 a, cf64 b) { return a.x == b.x && a.y == b.y; }
+
 // 64-lane wavefront helpers
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return (int) __lane_id(); }

@@ -188,6 +188,8 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
         running += __shfl(inc, 63, 64);
     }
     if (lane == 0) off[count] = running;
+    // the loop below reads fill[s - 1], written by lane s - 1 above
+    __syncthreads();
     // batch starts: bfirst[k] = the first slot s with off[s] >= EB_BATCH k
     // (count for the end), k <= running / EB_BATCH + 1
     int *bf = p.bfirst + (size_t) ib * (p.maxp / EB_BATCH + 2);
@@ -1489,10 +1491,6 @@ int eb_identity_typed(int NB, int bt, const int *bsize, int MB, void *Av, int *A
 // symmetry test: one wavefront per block row; every stored block (ib, jb)
 // needs a stored (jb, ib) holding its conjugate transpose (values compared
 // with ==, so +0 and -0 match and a NaN never does)
-__device__ __forceinline__ bool same_value(float a, float b) { return a == b; }
-__device__ __forceinline__ bool same_value(double a, double b) { return a == b; }
-__device__ __forceinline__ bool same_value(cf32 a, cf32 b) { return a.x == b.x && a.y == b.y; }
-__device__ __forceinline__ bool same_value(cf64 a, cf64 b) { return a.x == b.x && a.y == b.y; }
 
 template <typename T, int BT>
 __global__ __launch_bounds__(64) void k_eb_check_sym(int NB, int MB, const T *Av, const int *Ai,

@@ -1964,8 +1964,8 @@ int identity_typed(int N, int M, void *Av, int *Ai, int *An) {
     return report("k_identity");
 }
 
-// Symmetry test: entry (i, j, v) needs (j, i, conj(v)) in row j, bit for bit.
-// One thread per stored entry; mismatches are counted.
+// Symmetry test: entry (i, j, v) needs (j, i, conj(v)) in row j, equal by
+// value (same_value).  One thread per stored entry; mismatches are counted.
 template <typename T>
 __global__ __launch_bounds__(256) void k_check_sym(int N, int M, const T *Av, const int *Ai,
                                                    const int *An, int *bad) {
@@ -1982,17 +1982,7 @@ __global__ __launch_bounds__(256) void k_check_sym(int N, int M, const T *Av, co
     const int n = An[j];
     bool ok = false;
     for (int k = 0; k < n && !ok; k++)
-        if (Ai[(size_t) j * M + k] == i) {
-            union U {
-                T t;
-                int w[sizeof(T) / 4];
-            } x, y;
-            x.t = Av[(size_t) j * M + k];
-            y.t = v;
-            ok = true;
-#pragma unroll
-            for (int q = 0; q < (int) (sizeof(T) / 4); q++) ok = ok & (x.w[q] == y.w[q]);
-        }
+        if (Ai[(size_t) j * M + k] == i) ok = same_value(Av[(size_t) j * M + k], v);
     if (!ok) atomicAdd(bad, 1);
 }
 

@@ -467,7 +467,8 @@ void bml_hip_sp2_ellpack_batch(int nsys, bml_matrix_ellpack_hip_t *const *H,
              * shape or the precision changes (the first pass runs eagerly so
              * every scratch buffer is sized before capture) */
             const int same = graph && sh.group == gshape.group && sh.table == gshape.table &&
-                             sh.bucket == gshape.bucket && sh.remap == gshape.remap;
+                             sh.bucket == gshape.bucket && sh.remap == gshape.remap &&
+                             sh.waves == gshape.waves && sh.engine == gshape.engine;
             if (!same) {
                 bmlhip_graph_destroy(graph);
                 graph = NULL;

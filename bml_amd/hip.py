@@ -95,7 +95,7 @@ _SIGS = {
     "bml_clear_ellpack_hip": (None, [_P]),
     "bml_copy_ellpack_hip_new": (_P, [_P]),
     "bml_copy_ellpack_hip": (None, [_P, _P]),
-    "bml_import_from_dense_ellpack_hip": (_P, [_I, _I, _I, _I, _P, _D, _I]),
+    "bml_import_from_dense_ellpack_hip": (_P, [_I, _I, _I, _P, _D, _I, _I]),
     "bml_export_to_dense_ellpack_hip": (_P, [_P, _I]),
     "bml_write_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
     "bml_read_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
@@ -316,7 +316,7 @@ def import_from_dense(dense: np.ndarray, M: int, threshold: float = 0.0,
                       order: int = 0) -> EllpackHip:
     d = np.ascontiguousarray(dense)
     return EllpackHip(_call("bml_import_from_dense_ellpack_hip", precision_of(d.dtype), order,
-                            d.shape[0], M, d.ctypes.data, threshold, 0))
+                            d.shape[0], d.ctypes.data, threshold, M, 0))
 
 
 def write_matrix_market(A, path: str) -> None:

@@ -166,10 +166,11 @@ void bml_clear_ellpack_hip(bml_matrix_ellpack_hip_t *A);
  * B may have a different M as long as every row fits. */
 bml_matrix_ellpack_hip_t *bml_copy_ellpack_hip_new(const bml_matrix_ellpack_hip_t *A);
 void bml_copy_ellpack_hip(const bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B);
-/* bml_import_from_dense_ellpack (ellpack/bml_import_ellpack_typed.c:28-76):
+/* bml_import_from_dense_ellpack (ellpack/bml_import_ellpack.h:6-13,
+ * typed ellpack/bml_import_ellpack_typed.c:28-76), same argument order:
  * entries with |a| > threshold are kept, in column order. */
 bml_matrix_ellpack_hip_t *bml_import_from_dense_ellpack_hip(
-    int precision, int order, int N, int M, const void *A, double threshold,
+    int precision, int order, int N, const void *A, double threshold, int M,
     int distrib_mode);
 /* bml_export_to_dense_ellpack (ellpack/bml_export_ellpack_typed.c:18-73):
  * returns a malloc'd N*N host array. */
