@@ -223,7 +223,8 @@ static int s_row_engine = BML_HIP_ROW_ENGINE_EXACT;
 
 void bml_hip_set_row_engine(int engine)
 {
-    if (engine != BML_HIP_ROW_ENGINE_EXACT && engine != BML_HIP_ROW_ENGINE_UNORDERED) {
+    if (engine != BML_HIP_ROW_ENGINE_EXACT && engine != BML_HIP_ROW_ENGINE_UNORDERED &&
+        engine != BML_HIP_ROW_ENGINE_WINDOW) {
         bmlh_error(BML_HIP_ERR_ARGUMENT, "bml_hip_set_row_engine: unknown engine %d", engine);
         return;
     }

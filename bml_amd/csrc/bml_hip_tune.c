@@ -29,7 +29,7 @@
 #include <stdlib.h>
 
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
-                      int hint_max_a, bmlh_spgemm_shape *out)
+                      int hint_max_a, int square, bmlh_spgemm_shape *out)
 {
     int g = 8;
     while (g < 64 && 3 * g < max_b)
@@ -101,7 +101,9 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
             tm -= 64;
         t = tm;
     }
-    int engine = bml_hip_get_row_engine() == BML_HIP_ROW_ENGINE_UNORDERED && g == 64
+    const int sel = bml_hip_get_row_engine();
+    int engine = (sel == BML_HIP_ROW_ENGINE_UNORDERED || sel == BML_HIP_ROW_ENGINE_WINDOW) &&
+                         g == 64
                      ? BMLHIP_ROW_UNORDERED
                      : BMLHIP_ROW_EXACT;
     const char *e;
@@ -153,6 +155,30 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
                 t = tw;
                 waves = wv;
             }
+        }
+    }
+    /* Window engine (squares): forced by the WINDOW setting, or on the
+     * unordered setting when a row's distinct columns would pass what an
+     * LDS hash table holds at two blocks per CU (beyond that the hash
+     * engine's rows fall to the global-table pass).  Windows of 2^12
+     * columns, eight wavefronts per row; at most 64 windows. */
+    out->wbits = 0;
+    if (square && (sel == BML_HIP_ROW_ENGINE_WINDOW ||
+                   (sel == BML_HIP_ROW_ENGINE_UNORDERED && engine == BMLHIP_ROW_UNORDERED))) {
+        int wb = (e = getenv("BML_HIP_WINDOW_BITS")) ? atoi(e) : 12;
+        if (wb < 11)
+            wb = 11;
+        if (wb > 13)
+            wb = 13;
+        /* smallest window with at most 64 windows over N columns */
+        while (wb < 13 && ((long long) N + (1ll << wb) - 1) >> wb > BMLHIP_WIN_MAX_WINDOWS)
+            wb++;
+        const int fits = (((long long) N + (1ll << wb) - 1) >> wb) <= BMLHIP_WIN_MAX_WINDOWS;
+        const long long wmin = (e = getenv("BML_HIP_WINDOW_MIN")) ? atoll(e) : 3000;
+        if (fits && (sel == BML_HIP_ROW_ENGINE_WINDOW || est > wmin)) {
+            engine = BMLHIP_ROW_WINDOW;
+            out->wbits = wb;
+            waves = 8;
         }
     }
     out->engine = engine;

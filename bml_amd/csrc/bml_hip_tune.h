@@ -10,14 +10,17 @@ typedef struct bmlh_spgemm_shape {
     int bucket; /* slots per probe */
     int remap;  /* XCD-aware block order */
     int waves;  /* wavefronts per row (long rows; unordered engine) */
-    int engine; /* BMLHIP_ROW_EXACT or BMLHIP_ROW_UNORDERED */
+    int engine; /* BMLHIP_ROW_EXACT, _UNORDERED or _WINDOW */
+    int wbits;  /* _WINDOW: log2 of the column window */
 } bmlh_spgemm_shape;
 
 /* Launch shape for a product whose left rows hold up to max_a entries and
  * right rows up to max_b; touched_hint is the largest distinct-column count
  * seen the last time (0 = unknown), when the left rows held up to
- * hint_max_a entries.  The engine follows bml_hip_set_row_engine. */
+ * hint_max_a entries.  The engine follows bml_hip_set_row_engine; square is
+ * 1 for a product X*X (x2, the SP2 iterations), the only one the window
+ * engine takes. */
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
-                      int hint_max_a, bmlh_spgemm_shape *out);
+                      int hint_max_a, int square, bmlh_spgemm_shape *out);
 
 #endif

@@ -61,7 +61,7 @@ void *bmlhip_stream(void);
 /* Library-owned scratch, grown on demand; slot selects one of a few
  * independent buffers.  The pointer stays valid until the next call with
  * the same slot and a larger size. */
-#define BMLHIP_SCRATCH_SLOTS 12
+#define BMLHIP_SCRATCH_SLOTS 16
 void *bmlhip_scratch(int slot, size_t bytes);
 /* as bmlhip_scratch, zero-filled whenever the buffer is (re)allocated */
 void *bmlhip_scratch_zeroed(int slot, size_t bytes);
@@ -111,13 +111,25 @@ typedef struct bmlhip_spgemm_args {
     int engine;  /* BMLHIP_ROW_EXACT: sums in the reference's jp order, first-touch
                     output order; BMLHIP_ROW_UNORDERED (group 64): waves
                     wavefronts per row with LDS atomic adds, table order */
-    int sym;     /* x2 of a symmetric / Hermitian X on the unordered engine:
-                    rows form columns k >= i, the rest is mirrored */
+    int sym;     /* x2 of a symmetric / Hermitian X on the unordered or window
+                    engine: rows form columns k >= i, the rest is mirrored */
+    int wbits;   /* BMLHIP_ROW_WINDOW: log2 of the column window (11..13) */
     const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
     const bmlhip_batch *batch; /* not NULL: per-system traces into batch->d_trx/d_trx2 */
 } bmlhip_spgemm_args;
 #define BMLHIP_ROW_EXACT 0
 #define BMLHIP_ROW_UNORDERED 1
+#define BMLHIP_ROW_WINDOW 2 /* x2 only (A = B): waves wavefronts per row add
+                               into a dense LDS window of 2^wbits columns at a
+                               time; rows come out sorted by column */
+/* scratch slots of the window engine: packed rows, window offsets, window
+ * masks, per-row norms */
+#define BMLHIP_WIN_PACK_SLOT 12
+#define BMLHIP_WIN_BOFF_SLOT 13
+#define BMLHIP_WIN_MASK_SLOT 14
+#define BMLHIP_WIN_NORM_SLOT 15
+/* window engine limits: at most 64 windows per row range */
+#define BMLHIP_WIN_MAX_WINDOWS 64
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);
 
 /* A <- alpha*A + beta*B (mode 0), or that plus the norm over the union of
@@ -163,8 +175,20 @@ typedef struct bmlhip_sp2_state {
  * lands tr closer to nocc, store alpha/beta, record history[it*4..]. */
 int bmlhip_sp2_decide(bmlhip_sp2_state *d_state, double nocc, double *d_hist, int maxit);
 /* After add (status scalars[2] holds ||X - X^2||^2): record the norm, set
- * done when it is <= tol, advance it. */
-int bmlhip_sp2_check(bmlhip_sp2_state *d_state, double tol, double *d_hist, int maxit);
+ * done when it is <= tol, advance it; touched = 1 also records the status
+ * block's longest pre-threshold row (when the product ran after the
+ * decision, as in the fused window step). */
+int bmlhip_sp2_check(bmlhip_sp2_state *d_state, double tol, double *d_hist, int maxit,
+                     int touched);
+/* One SP2 iteration of a symmetric (Hermitian) X on the window engine, X
+ * updated in place: tr X and tr X^2 = sum |X_ik|^2 reduced from X, the
+ * branch chosen (bmlhip_sp2_decide), then one fused pass forms the upper
+ * triangle of X^2 (kept as bml_multiply_x2 keeps it), writes alpha X + beta
+ * X^2 thresholded as bml_add_norm does, mirrors it and reduces ||X - X^2||^2
+ * into scalars[2]; then bmlhip_sp2_check.  Nothing runs once d_state->done. */
+int bmlhip_sp2_window_step(int prec, int N, int M, void *d_Xv, int *d_Xi, int *d_Xn,
+                           double threshold, int wbits, int waves, bmlhip_sp2_state *d_state,
+                           double nocc, double tol, double *d_hist, int maxit);
 /* Capture the kernels queued on the library stream between begin and end
  * into an executable graph (no allocation, memset or host sync may happen
  * in between); launch replays them on the library stream. */

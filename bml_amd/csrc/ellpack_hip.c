@@ -428,13 +428,14 @@ static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
     bmlh_spgemm_shape sh;
     bmlh_pick_spgemm(a.prec, A->N, max_nnz_of(A), max_nnz_of(B),
                      hint_owner ? hint_owner->touched_hint : 0,
-                     hint_owner ? hint_owner->hint_max_nnz : 0, &sh);
+                     hint_owner ? hint_owner->hint_max_nnz : 0, A == B, &sh);
     a.group = sh.group;
     a.table = sh.table;
     a.bucket = sh.bucket;
     a.remap = sh.remap;
     a.waves = sh.waves;
     a.engine = sh.engine;
+    a.wbits = sh.wbits;
     /* BML_HIP_ASSUME_SYMMETRIC=1: the caller asserts X is symmetric
      * (Hermitian) and x2 forms the upper triangle and mirrors it, as the SP2
      * drivers do for a symmetric H (a test hook for that path) */

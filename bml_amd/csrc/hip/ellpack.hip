@@ -986,6 +986,391 @@ __global__ __launch_bounds__(256) void k_sym_finish(SpParams p, int *fill) {
     p.rowstat[k].x += f;
 }
 
+// ---------------------------------------------------------------------------
+// Window engine, for wide rows (X^2 of a matrix whose product rows touch more
+// distinct columns than an LDS hash table holds: the 32^3 lattice density
+// matrix at hopping -0.5 keeps ~2600 entries per row and its X^2 rows touch
+// ~10^4 of the 32768 columns).  The column range is cut into windows of WC =
+// 2^WB columns.  A pack pass (k_win_pack) copies every row of X with its
+// entries grouped by window, as (value, column) records in consecutive
+// 32-bit words (one load per entry), with per-row window offsets (boff) and
+// a mask of the windows the row touches.  Then one block per output row i
+// takes the windows its rows j touch, in column order; per window its
+// wavefronts take the entries (a, j) of row i round-robin and add a * X[j,k]
+// for the segment of row j in the window straight into a dense LDS window
+// (ds_add_f64: no hashing, no probing, no capacity limit), and the epilogue
+// walks the window in column order: output rows come out sorted by column.
+// Sums are formed in arrival order, as on the unordered engine.
+//
+// SYM (X symmetric / Hermitian): row i forms only its columns k >= i (the
+// windows below i's are skipped, lanes with k < i sit idle) and
+// k_sym_mirror fills the rest.  FUSE (SP2): the epilogue applies the SP2
+// update X <- alpha X + beta X^2 (alpha, beta chosen beforehand from tr X and
+// tr X^2 = sum |X_ik|^2, which the pack pass reduces) with the thresholds of
+// bml_multiply_x2 and bml_add_norm, accumulates ||X - X^2||_F^2 over the row,
+// and writes the new row in place of X's (the products read only the packed
+// copy).
+
+// A packed entry: the value's words, then the column.
+template <typename T>
+struct PkW {
+    static constexpr int n = (int) (sizeof(T) / 4) + 1;
+};
+template <int NW>
+struct __attribute__((aligned(4))) Words {
+    unsigned w[NW];
+};
+template <typename T>
+__device__ __forceinline__ void pk_get(const unsigned *P, size_t e, T &v, int &c) {
+    constexpr int NW = PkW<T>::n;
+    const Words<NW> x = *reinterpret_cast<const Words<NW> *>(P + e * NW);
+    union {
+        T t;
+        unsigned u[NW - 1];
+    } cv;
+#pragma unroll
+    for (int q = 0; q < NW - 1; q++) cv.u[q] = x.w[q];
+    v = cv.t;
+    c = (int) x.w[NW - 1];
+}
+template <typename T>
+__device__ __forceinline__ int pk_col(const unsigned *P, size_t e) {
+    return (int) P[e * PkW<T>::n + PkW<T>::n - 1];
+}
+template <typename T>
+__device__ __forceinline__ void pk_put(unsigned *P, size_t e, T v, int c) {
+    constexpr int NW = PkW<T>::n;
+    union {
+        T t;
+        unsigned u[NW - 1];
+    } cv;
+    cv.t = v;
+    Words<NW> x;
+#pragma unroll
+    for (int q = 0; q < NW - 1; q++) x.w[q] = cv.u[q];
+    x.w[NW - 1] = (unsigned) c;
+    *reinterpret_cast<Words<NW> *>(P + e * NW) = x;
+}
+
+__device__ __forceinline__ unsigned long long or64_wave(unsigned long long m) {
+    unsigned lo = (unsigned) m, hi = (unsigned) (m >> 32);
+    for (int off = 32; off > 0; off >>= 1) {
+        lo |= (unsigned) __shfl_xor((int) lo, off, 64);
+        hi |= (unsigned) __shfl_xor((int) hi, off, 64);
+    }
+    return ((unsigned long long) hi << 32) | lo;
+}
+
+struct WinParams {
+    int N, MP, nwin, MC;
+    const unsigned *P;               // packed rows: N x MP entries
+    const int *boff;                 // [N][nwin + 1] window offsets within each packed row
+    const unsigned long long *wmask; // [N] windows each row touches
+    void *Cv;
+    int *Ci, *Cn;
+    double thr;
+    double *rowtr;    // [2N] x2: per-row tr X, tr X^2 partials
+    double *rownorm;  // [N] FUSE: per-row ||X - X^2||^2 partials
+    const double *ab; // FUSE: (alpha, beta) per system (index row_sys[i], or 0)
+    int4 *rowstat;
+    const int *skip;
+    const int *row_sys, *sys_done;
+    int remap;
+};
+
+// Pack pass: one wavefront per row.  Row j's entries are grouped by window
+// (counted, scanned, scattered with LDS counters; the order within a window
+// is the arrival order), boff[j][c] is the first entry of window c,
+// boff[j][nwin] the row length, wmask[j] the windows with entries.  With
+// rowtr the row's tr X (its diagonal) and sum |X_jk|^2 partials are written
+// too.
+template <typename T>
+__global__ __launch_bounds__(256) void k_win_pack(int N, int M, const T *Xv, const int *Xi,
+                                                  const int *Xn, int MP, int nwin, int wb,
+                                                  unsigned *P, int *boff,
+                                                  unsigned long long *wmask, double *rowtr,
+                                                  const int *skip, const int *row_sys,
+                                                  const int *sys_done) {
+    __shared__ int cnt[4][65];
+    if (skip && *skip) return;  // uniform
+    const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + w;
+    const bool live = j < N && !(row_sys && sys_done[row_sys[j]]);
+    const int n = live ? Xn[j] : 0;
+    const size_t rowX = live ? (size_t) j * M : 0;
+    cnt[w][lane] = 0;
+    __syncthreads();
+    double trx = 0.0, sq = 0.0;
+    for (int e = lane; e < n; e += 64) {
+        const int c = Xi[rowX + e];
+        atomicAdd(&cnt[w][c >> wb], 1);
+        if (rowtr) {
+            const T v = Xv[rowX + e];
+            if (c == j) trx += Elem<T>::re(v);
+            sq += Elem<T>::abs2(v);
+        }
+    }
+    __syncthreads();
+    const int x = lane < nwin ? cnt[w][lane] : 0;
+    int inc = x;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    const unsigned long long m = ballot64(x > 0);
+    __syncthreads();
+    cnt[w][lane] = inc - x;  // next free position of window `lane`
+    if (live) {
+        int *bo = boff + (size_t) j * (nwin + 1);
+        if (lane < nwin) bo[lane] = inc - x;
+        if (lane == 0) {
+            bo[nwin] = n;
+            wmask[j] = m;
+        }
+    }
+    __syncthreads();
+    const size_t rowP = (size_t) (live ? j : 0) * MP;
+    for (int e = lane; e < n; e += 64) {
+        const int c = Xi[rowX + e];
+        const T v = Xv[rowX + e];
+        const int pos = atomicAdd(&cnt[w][c >> wb], 1);
+        pk_put<T>(P, rowP + pos, v, c);
+    }
+    if (rowtr && live) {
+        for (int off = 32; off > 0; off >>= 1) {
+            trx += __shfl_down(trx, off, 64);
+            sq += __shfl_down(sq, off, 64);
+        }
+        if (lane == 0) {
+            rowtr[j] = trx;
+            rowtr[N + j] = sq;
+        }
+    }
+}
+
+// One block of W wavefronts per output row; LDS: the window's accumulators
+// (double components) and, with FUSE, X's row i in the window.
+template <typename T, int W, int WB, bool SYM, bool FUSE>
+__global__ __launch_bounds__(64 * W) void k_spgemm_win(WinParams p) {
+    using E = Elem<T>;
+    using A = typename UAcc<T>::type;
+    constexpr int WC = 1 << WB;
+    constexpr int NT = 64 * W;
+    constexpr int PER = WC / W;  // epilogue columns per wavefront
+    static_assert(PER % 64 == 0, "window too small for the block");
+    extern __shared__ __align__(16) unsigned char smem[];
+    A *win = (A *) smem;  // WC accumulators, then one dummy slot per lane
+    T *xwin = (T *) (smem + sizeof(A) * (WC + 64));
+    const int dummy = WC + (int) (threadIdx.x & 63);
+    __shared__ unsigned long long mask_s;
+    __shared__ int cnt_s[W], dflag_s;
+    __shared__ long long prods_s[W];
+    __shared__ double nrm_s[W];
+    __shared__ T trx_s, d2_s;
+    if (p.skip && *p.skip) return;  // uniform
+    const int i = p.remap ? xcd_block(blockIdx.x, gridDim.x) : (int) blockIdx.x;
+    if (i >= p.N) return;
+    const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+    if (p.row_sys && p.sys_done[p.row_sys[i]]) {  // uniform
+        if (tid == 0) p.rowstat[i] = make_int4(-1, 0, 0, 0);
+        return;
+    }
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const unsigned *P = p.P;
+    const size_t rowP = (size_t) i * p.MP;
+    const int nw1 = p.nwin + 1;
+    const int *bo_i = p.boff + (size_t) i * nw1;
+    const int nA = bo_i[p.nwin];
+    if (tid == 0) {
+        mask_s = 0ull;
+        trx_s = E::zero();
+        d2_s = E::zero();
+    }
+    __syncthreads();
+    // the windows row i touches: the union of its rows' masks
+    unsigned long long m = 0ull;
+    for (int e = tid; e < nA; e += NT) {
+        T a;
+        int j;
+        pk_get<T>(P, rowP + e, a, j);
+        m |= p.wmask[j];
+        if (!FUSE && j == i) trx_s = a;  // one entry per row holds the diagonal
+    }
+    m = or64_wave(m);
+    if (lane == 0 && m) atomicOr(&mask_s, m);
+    __syncthreads();
+    m = mask_s;
+    if constexpr (SYM) m &= ~((1ull << (i >> WB)) - 1ull);
+    double alpha = 0.0, beta = 0.0;
+    if constexpr (FUSE) {
+        const int sys = p.row_sys ? p.row_sys[i] : 0;
+        alpha = p.ab[2 * sys];
+        beta = p.ab[2 * sys + 1];
+    }
+    const int MC = p.MC;
+    const size_t rowC = (size_t) i * MC;
+    T *Cv = (T *) p.Cv;
+    int kept = 0, distinct = 0;  // block-uniform running counts
+    long long prods = 0;         // this wavefront's products
+    double nrm = 0.0;            // this lane's norm partial
+    while (m) {
+        const int c = __ffsll((long long) m) - 1;
+        m &= m - 1ull;
+        const int cbase = c << WB;
+        for (int s = tid; s < WC; s += NT) {
+            win[s] = Elem<A>::zero();
+            if constexpr (FUSE) xwin[s] = E::zero();
+        }
+        if (tid == 0) dflag_s = SYM ? (nA > 0) : 0;
+        bool dtouch = false;  // this lane added into column i (!SYM)
+        __syncthreads();
+        // products: wavefront w takes groups g = w, w + W, ... of 64 entries
+        // of row i; lane q of a group holds entry q's (a, j) and the bounds
+        // of row j's segment in window c
+        for (int g0 = w * 64; g0 < nA; g0 += NT) {
+            const int e = g0 + lane;
+            T aq = E::zero();
+            int jq = 0, lo = 0, hi = 0;
+            if (e < nA) {
+                pk_get<T>(P, rowP + e, aq, jq);
+                const int *bo = p.boff + (size_t) jq * nw1;
+                lo = bo[c];
+                hi = bo[c + 1];
+            }
+            uint64_t todo = ballot64(hi > lo);
+            while (todo) {
+                const int q = __ffsll((long long) todo) - 1;
+                todo &= todo - 1ull;
+                const T a = bcast<64>(aq, q);
+                const int j = __builtin_amdgcn_readlane(jq, q);
+                const int l0 = __builtin_amdgcn_readlane(lo, q);
+                const int l1 = __builtin_amdgcn_readlane(hi, q);
+                const size_t rb = (size_t) j * p.MP;
+                prods += l1 - l0;
+                int k0 = l0 + lane;
+                // two entries per lane per round, branch-free: both loads
+                // issue before the adds, and (SYM) the products of columns
+                // below i go to the lane's own dummy slot past the window
+                for (; k0 + 64 < l1; k0 += 128) {
+                    T b0, b1;
+                    int c0, c1;
+                    pk_get<T>(P, rb + k0, b0, c0);
+                    pk_get<T>(P, rb + k0 + 64, b1, c1);
+                    const int s0 = (!SYM || c0 >= i) ? c0 - cbase : dummy;
+                    const int s1 = (!SYM || c1 >= i) ? c1 - cbase : dummy;
+                    lds_add(&win[s0], UAcc<T>::to(E::mul(a, b0)));
+                    lds_add(&win[s1], UAcc<T>::to(E::mul(a, b1)));
+                    dtouch = dtouch | (c0 == i) | (c1 == i);
+                }
+                if (k0 < l1) {
+                    T b0;
+                    int c0;
+                    pk_get<T>(P, rb + k0, b0, c0);
+                    const int s0 = (!SYM || c0 >= i) ? c0 - cbase : dummy;
+                    lds_add(&win[s0], UAcc<T>::to(E::mul(a, b0)));
+                    dtouch = dtouch | (c0 == i);
+                }
+            }
+        }
+        if constexpr (!SYM) {
+            if (__any(dtouch) && lane == 0) dflag_s = 1;
+        }
+        if constexpr (FUSE) {
+            // X's own row i in this window
+            for (int e = bo_i[c] + tid; e < bo_i[c + 1]; e += NT) {
+                T v;
+                int k;
+                pk_get<T>(P, rowP + e, v, k);
+                xwin[k - cbase] = v;
+            }
+        }
+        __syncthreads();
+        const bool dtouched = dflag_s != 0;
+        // epilogue: wavefront w walks columns [w PER, (w+1) PER) of the
+        // window twice, counting, then writing at its offset
+        auto entry = [&](int s, T &out, bool &keep, bool &nz, double &dn) {
+            const int k = cbase + s;
+            const bool valid = k < p.N && (!SYM || k >= i);
+            const T y = UAcc<T>::from(win[s]);
+            nz = valid && (E::mag(y) != 0.0 || k == i);
+            const bool ykeep = valid && (k == i ? dtouched : E::above(y, p.thr));
+            dn = 0.0;
+            if constexpr (FUSE) {
+                const T x = xwin[s];
+                const T yk = ykeep ? y : E::zero();
+                if (valid) dn = E::abs2(E::sub(x, yk)) * ((SYM && k > i) ? 2.0 : 1.0);
+                out = E::axpy(E::axpy(E::zero(), alpha, x), beta, yk);
+                keep = valid && E::above(out, p.thr);
+            } else {
+                out = y;
+                keep = ykeep;
+            }
+        };
+        int cw = 0, dw = 0;
+        for (int s = w * PER + lane; s < (w + 1) * PER; s += 64) {
+            T v;
+            bool keep, nz;
+            double dn;
+            entry(s, v, keep, nz, dn);
+            nrm += dn;
+            cw += popc64(ballot64(keep));
+            dw += popc64(ballot64(nz));
+            if (!FUSE && cbase + s == i) d2_s = v;
+        }
+        if (lane == 0) {
+            cnt_s[w] = cw;
+            prods_s[w] = dw;  // distinct counts, summed below
+        }
+        __syncthreads();
+        int pos0 = kept, tot = 0, dtot = 0;
+        for (int q = 0; q < W; q++) {
+            pos0 += q < w ? cnt_s[q] : 0;
+            tot += cnt_s[q];
+            dtot += (int) prods_s[q];
+        }
+        for (int s = w * PER + lane; s < (w + 1) * PER; s += 64) {
+            T v;
+            bool keep, nz;
+            double dn;
+            entry(s, v, keep, nz, dn);
+            const uint64_t km = ballot64(keep);
+            const int pos = pos0 + popc64(km & ltmask);
+            if (keep && pos < MC) {
+                p.Ci[rowC + pos] = cbase + s;
+                Cv[rowC + pos] = v;
+            }
+            pos0 += popc64(km);
+        }
+        kept += tot;
+        distinct += dtot;
+        __syncthreads();  // the window and cnt_s are reused by the next window
+    }
+    // row records, traces, norm
+    if (lane == 0) prods_s[w] = prods;
+    double r = nrm;
+    if constexpr (FUSE) {
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
+        if (lane == 0) nrm_s[w] = r;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        long long pr = 0;
+        double nr = 0.0;
+        for (int q = 0; q < W; q++) {
+            pr += prods_s[q];
+            if constexpr (FUSE) nr += nrm_s[q];
+        }
+        p.Cn[i] = kept < MC ? kept : MC;
+        p.rowstat[i] = make_int4(kept, distinct, (int) (pr < INT_MAX ? pr : INT_MAX), nA);
+        if constexpr (FUSE) {
+            p.rownorm[i] = nr;
+        } else {
+            p.rowtr[i] = E::re(trx_s);
+            p.rowtr[p.N + i] = E::re(d2_s);
+        }
+    }
+}
+
 // First pass: table in LDS, 64/GROUP rows per 64-thread block.
 template <typename T, int GROUP, int BK>
 __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
@@ -1670,8 +2055,216 @@ int launch_spgemm_bk(const SpParams &p, unsigned tsize, int bucket, hipStream_t 
     }
 }
 
+// ---- window engine launchers ----------------------------------------------
+
+template <typename T, int W, int WB, bool SYM, bool FUSE>
+int launch_win(const WinParams &wp, hipStream_t s) {
+    using A = typename UAcc<T>::type;
+    const size_t lds = sizeof(A) * (((size_t) 1 << WB) + 64) + (FUSE ? sizeof(T) * ((size_t) 1 << WB) : 0);
+    if (lds > 160 * 1024 - 1024)
+        return fail_msg(5, "window engine: a window of 2^%d columns needs %zu bytes of LDS", WB, lds);
+    if (lds > 64 * 1024)
+        (void) hipFuncSetAttribute((const void *) k_spgemm_win<T, W, WB, SYM, FUSE>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+    if (wp.N > 0)
+        hipLaunchKernelGGL((k_spgemm_win<T, W, WB, SYM, FUSE>), dim3(wp.N), dim3(64 * W), lds, s,
+                           wp);
+    return report("k_spgemm_win");
+}
+
+template <typename T, bool SYM, bool FUSE>
+int launch_win_shape(const WinParams &wp, int wbits, int waves, hipStream_t s) {
+    if (waves == 8) {
+        switch (wbits) {
+        case 11: return launch_win<T, 8, 11, SYM, FUSE>(wp, s);
+        case 12: return launch_win<T, 8, 12, SYM, FUSE>(wp, s);
+        case 13: return launch_win<T, 8, 13, SYM, FUSE>(wp, s);
+        default: break;
+        }
+    } else if (waves == 4) {
+        switch (wbits) {
+        case 11: return launch_win<T, 4, 11, SYM, FUSE>(wp, s);
+        case 12: return launch_win<T, 4, 12, SYM, FUSE>(wp, s);
+        case 13: return launch_win<T, 4, 13, SYM, FUSE>(wp, s);
+        default: break;
+        }
+    }
+    return fail_msg(2, "window engine: window 2^%d with %d wavefronts not built (2^11..2^13, 4 or 8)",
+                    wbits, waves);
+}
+
+// Window count of an N-column product, or -1 when it passes the limit.
+inline int win_count(int N, int wbits) {
+    const long long n = ((long long) N + (1ll << wbits) - 1) >> wbits;
+    return n <= BMLHIP_WIN_MAX_WINDOWS ? (int) (n < 1 ? 1 : n) : -1;
+}
+
+// Pack X (N rows, stride M) into the window scratch; traces: the row
+// partials of tr X and sum |X_ik|^2 into rowtr.  Fills wp's P, boff, wmask.
+template <typename T>
+int win_pack(int N, int M, const void *Xv, const int *Xi, const int *Xn, int wbits,
+             double *rowtr, const int *skip, const int *row_sys, const int *sys_done,
+             WinParams &wp, hipStream_t s) {
+    const int nwin = win_count(N, wbits);
+    if (nwin < 0)
+        return fail_msg(5, "window engine: N=%d needs more than %d windows of 2^%d columns", N,
+                        BMLHIP_WIN_MAX_WINDOWS, wbits);
+    unsigned *P = (unsigned *) bmlhip_scratch(BMLHIP_WIN_PACK_SLOT,
+                                              sizeof(unsigned) * PkW<T>::n * ((size_t) N * M + 1));
+    int *boff = (int *) bmlhip_scratch(BMLHIP_WIN_BOFF_SLOT,
+                                       sizeof(int) * ((size_t) N * (nwin + 1) + 1));
+    unsigned long long *wmask = (unsigned long long *) bmlhip_scratch(
+        BMLHIP_WIN_MASK_SLOT, sizeof(unsigned long long) * ((size_t) N + 1));
+    if (!P || !boff || !wmask) return fail_msg(4, "scratch allocation failed");
+    wp.N = N;
+    wp.MP = M;
+    wp.nwin = nwin;
+    wp.P = P;
+    wp.boff = boff;
+    wp.wmask = wmask;
+    if (N > 0)
+        hipLaunchKernelGGL(k_win_pack<T>, dim3((N + 3) / 4), dim3(256), 0, s, N, M, (const T *) Xv,
+                           Xi, Xn, M, nwin, wbits, P, boff, wmask, rowtr, skip, row_sys, sys_done);
+    return report("k_win_pack");
+}
+
+// X^2 on the window engine (a->d_Av == a->d_Bv): pack, rows, mirror (sym),
+// row records and traces, as spgemm_typed's other engines.
+template <typename T>
+int spgemm_win_typed(const bmlhip_spgemm_args *a) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    WinParams wp;
+    memset(&wp, 0, sizeof wp);
+    const int *row_sys = a->batch ? a->batch->d_row_sys : nullptr;
+    const int *sys_done = a->batch ? a->batch->d_done : nullptr;
+    if (a->N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
+    if ((rc = win_pack<T>(a->N, a->MB, a->d_Bv, a->d_Bi, a->d_Bn, a->wbits, nullptr, a->d_skip,
+                          row_sys, sys_done, wp, s)))
+        return rc;
+    wp.MC = a->MC;
+    wp.Cv = a->d_Cv;
+    wp.Ci = a->d_Ci;
+    wp.Cn = a->d_Cn;
+    wp.thr = a->threshold;
+    wp.rowtr = a->batch ? a->batch->d_rowtr
+                        : (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
+    wp.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (a->N + 1));
+    wp.skip = a->d_skip;
+    wp.row_sys = row_sys;
+    wp.sys_done = sys_done;
+    wp.remap = a->remap;
+    if (!wp.rowtr || !wp.rowstat) return fail_msg(4, "scratch allocation failed");
+    const bool sym = a->sym != 0;
+    int *fill = nullptr;
+    if (sym) {
+        fill = (int *) bmlhip_scratch_zeroed(BMLHIP_SYMFILL_SLOT, sizeof(int) * (size_t) (a->N + 1));
+        if (!fill) return fail_msg(4, "scratch allocation failed");
+        rc = launch_win_shape<T, true, false>(wp, a->wbits, a->waves, s);
+    } else {
+        rc = launch_win_shape<T, false, false>(wp, a->wbits, a->waves, s);
+    }
+    if (rc) return rc;
+    SpParams p;
+    memset(&p, 0, sizeof p);
+    p.N = a->N;
+    p.MC = a->MC;
+    p.Cv = a->d_Cv;
+    p.Ci = a->d_Ci;
+    p.Cn = a->d_Cn;
+    p.skip = a->d_skip;
+    p.rowstat = wp.rowstat;
+    p.row_sys = row_sys;
+    p.sys_done = sys_done;
+    if (sym && a->N > 0) {
+        hipLaunchKernelGGL(k_sym_mirror<T>, dim3((a->N + 3) / 4), dim3(256), 0, s, p, fill);
+        hipLaunchKernelGGL(k_sym_finish, dim3((a->N + 255) / 256), dim3(256), 0, s, p, fill);
+        if ((rc = report("k_sym_mirror"))) return rc;
+    }
+    if (a->N > 0) bmlhip_prof_mark_end();
+    hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(a->N)), dim3(256), 0, s,
+                       (const int4 *) wp.rowstat, a->N, a->MC,
+                       BMLHIP_RS_OVF_KEPT | BMLHIP_RS_STORED | BMLHIP_RS_PROF,
+                       (int) PrecOf<T>::value, a->d_skip, bmlhip_status_dev());
+    if ((rc = report("k_reduce_rowstat"))) return rc;
+    bmlhip_status *st = bmlhip_status_dev();
+    if (a->batch) {
+        const bmlhip_batch *b = a->batch;
+        hipLaunchKernelGGL(k_seg_sum, dim3(b->nsys), dim3(256), 0, s, wp.rowtr, b->d_row_off,
+                           b->d_done, b->d_trx);
+        hipLaunchKernelGGL(k_seg_sum, dim3(b->nsys), dim3(256), 0, s, wp.rowtr + a->N,
+                           b->d_row_off, b->d_done, b->d_trx2);
+        return report("k_seg_sum");
+    }
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, wp.rowtr, a->N, &st->scalars[0]);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, wp.rowtr + a->N, a->N, &st->scalars[1]);
+    return report("k_sum");
+}
+
+// One fused SP2 iteration on the window engine (symmetric X, in place).
+template <typename T>
+int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int wbits, int waves,
+                     bmlhip_sp2_state *d_state, double nocc, double tol, double *d_hist,
+                     int maxit) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    int rc = bmlhip_status_reset();
+    if (rc) return rc;
+    bmlhip_status *st = bmlhip_status_dev();
+    const int *skip = &d_state->done;
+    double *rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (N + 1));
+    double *rownorm = (double *) bmlhip_scratch(BMLHIP_WIN_NORM_SLOT, sizeof(double) * (size_t) (N + 1));
+    int4 *rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (N + 1));
+    int *fill = (int *) bmlhip_scratch_zeroed(BMLHIP_SYMFILL_SLOT, sizeof(int) * (size_t) (N + 1));
+    if (!rowtr || !rownorm || !rowstat || !fill) return fail_msg(4, "scratch allocation failed");
+    WinParams wp;
+    memset(&wp, 0, sizeof wp);
+    if (N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
+    if ((rc = win_pack<T>(N, M, Xv, Xi, Xn, wbits, rowtr, skip, nullptr, nullptr, wp, s))) return rc;
+    // the branch needs tr X and tr X^2 before the product
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rowtr, N, &st->scalars[0]);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rowtr + N, N, &st->scalars[1]);
+    if ((rc = report("k_sum"))) return rc;
+    if ((rc = bmlhip_sp2_decide(d_state, nocc, d_hist, maxit))) return rc;
+    wp.MC = M;
+    wp.Cv = Xv;
+    wp.Ci = Xi;
+    wp.Cn = Xn;
+    wp.thr = thr;
+    wp.rownorm = rownorm;
+    wp.ab = d_state->ab;
+    wp.rowstat = rowstat;
+    wp.skip = skip;
+    if ((rc = launch_win_shape<T, true, true>(wp, wbits, waves, s))) return rc;
+    SpParams p;
+    memset(&p, 0, sizeof p);
+    p.N = N;
+    p.MC = M;
+    p.Cv = Xv;
+    p.Ci = Xi;
+    p.Cn = Xn;
+    p.skip = skip;
+    p.rowstat = rowstat;
+    if (N > 0) {
+        hipLaunchKernelGGL(k_sym_mirror<T>, dim3((N + 3) / 4), dim3(256), 0, s, p, fill);
+        hipLaunchKernelGGL(k_sym_finish, dim3((N + 255) / 256), dim3(256), 0, s, p, fill);
+        if ((rc = report("k_sym_mirror"))) return rc;
+        bmlhip_prof_mark_end();
+    }
+    hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(N)), dim3(256), 0, s,
+                       (const int4 *) rowstat, N, M,
+                       BMLHIP_RS_OVF_KEPT | BMLHIP_RS_STORED | BMLHIP_RS_PROF,
+                       (int) PrecOf<T>::value, skip, st);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rownorm, N, &st->scalars[2]);
+    if ((rc = report("k_sum"))) return rc;
+    return bmlhip_sp2_check(d_state, tol, d_hist, maxit, 1);
+}
+
 template <typename T>
 int spgemm_typed(const bmlhip_spgemm_args *a) {
+    if (a->engine == BMLHIP_ROW_WINDOW && a->kind == 0 && a->d_Av == a->d_Bv &&
+        a->d_Ai == a->d_Bi && a->d_An == a->d_Bn && a->MA == a->MB)
+        return spgemm_win_typed<T>(a);
     hipStream_t s = (hipStream_t) bmlhip_stream();
     // the row kernels hash columns with a 24-bit multiply and address rows
     // with 32-bit element offsets
@@ -2039,6 +2632,15 @@ int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *skip) {
                        (hipStream_t) bmlhip_stream(), (const int4 *) rs, N, MC, what, prof, skip,
                        bmlhip_status_dev());
     return report("k_reduce_rowstat");
+}
+
+int bmlhip_sp2_window_step(int prec, int N, int M, void *Xv, int *Xi, int *Xn, double thr,
+                           int wbits, int waves, bmlhip_sp2_state *d_state, double nocc,
+                           double tol, double *d_hist, int maxit) {
+#define CALL(T) sp2_window_typed<T>(N, M, Xv, Xi, Xn, thr, wbits, waves, d_state, nocc, tol, \
+                                    d_hist, maxit)
+    DISPATCH(prec, CALL)
+#undef CALL
 }
 
 int bmlhip_is_symmetric(int prec, int N, int M, const void *Av, const int *Ai, const int *An,

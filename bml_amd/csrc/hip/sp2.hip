@@ -34,11 +34,12 @@ __global__ void k_sp2_decide(bmlhip_sp2_state *s, const bmlhip_status *st, doubl
 }
 
 __global__ void k_sp2_check(bmlhip_sp2_state *s, const bmlhip_status *st, double tol,
-                            double *hist, int maxit) {
+                            double *hist, int maxit, int touched) {
     if (threadIdx.x != 0 || s->done) return;
     const double nrm = sqrt(st->scalars[2]);
     s->norm = nrm;
     s->max_nnz = st->max_nnz;
+    if (touched) s->max_touched = st->max_touched;
     if (hist && s->it < maxit) hist[4 * s->it + 2] = nrm;
     s->it++;
     if (nrm <= tol) s->done = 1;
@@ -104,9 +105,10 @@ int bmlhip_sp2_decide(bmlhip_sp2_state *d_state, double nocc, double *d_hist, in
     return launch_error("k_sp2_decide");
 }
 
-int bmlhip_sp2_check(bmlhip_sp2_state *d_state, double tol, double *d_hist, int maxit) {
+int bmlhip_sp2_check(bmlhip_sp2_state *d_state, double tol, double *d_hist, int maxit,
+                     int touched) {
     hipLaunchKernelGGL(k_sp2_check, dim3(1), dim3(64), 0, (hipStream_t) bmlhip_stream(), d_state,
-                       bmlhip_status_dev(), tol, d_hist, maxit);
+                       bmlhip_status_dev(), tol, d_hist, maxit, touched);
     return launch_error("k_sp2_check");
 }
 

@@ -75,20 +75,28 @@ static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *
     a.d_skip = &d_state->done;
     int mx = X->aux->stats_valid ? X->aux->max_nnz : X->M;
     bmlh_spgemm_shape sh;
-    bmlh_pick_spgemm(a.prec, X->N, mx, mx, X->aux->touched_hint, X->aux->hint_max_nnz, &sh);
+    bmlh_pick_spgemm(a.prec, X->N, mx, mx, X->aux->touched_hint, X->aux->hint_max_nnz, 1, &sh);
     a.group = sh.group;
     a.table = sh.table;
     a.bucket = sh.bucket;
     a.remap = sh.remap;
     a.waves = sh.waves;
     a.engine = sh.engine;
+    a.wbits = sh.wbits;
     a.sym = sym;
+    if (sh.engine == BMLHIP_ROW_WINDOW && sym)
+        /* one fused pass: X^2's upper triangle, the update and the norm,
+         * X rewritten in place (X2 is not used) */
+        return bmlh_ok(bmlhip_sp2_window_step(X->matrix_precision, X->N, X->M, X->value,
+                                              X->index, X->nnz, o->threshold, sh.wbits,
+                                              sh.waves, d_state, nocc, o->tol, d_hist, o->maxit),
+                       what);
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
            bmlh_ok(bmlhip_add_dev(X->matrix_precision, X->N, X->M, X->value, X->index, X->nnz,
                                   X2->M, X2->value, X2->index, X2->nnz, d_state->ab,
                                   o->threshold, 2, &d_state->done), what) &&
-           bmlh_ok(bmlhip_sp2_check(d_state, o->tol, d_hist, o->maxit), what);
+           bmlh_ok(bmlhip_sp2_check(d_state, o->tol, d_hist, o->maxit, 0), what);
 }
 
 void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_hip_t *X,
@@ -132,7 +140,7 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     /* A symmetric (Hermitian) H gives symmetric iterates: on the unordered
      * engine, X^2 then forms its upper triangle and mirrors it */
     int sym = 0;
-    if (bml_hip_get_row_engine() == BML_HIP_ROW_ENGINE_UNORDERED &&
+    if (bml_hip_get_row_engine() != BML_HIP_ROW_ENGINE_EXACT &&
         !bmlh_ok(bmlhip_is_symmetric(H->matrix_precision, H->N, H->M, H->value, H->index,
                                      H->nnz, &sym), what))
         return;
@@ -324,6 +332,7 @@ static int batch_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t
     a.remap = sh->remap;
     a.waves = sh->waves;
     a.engine = sh->engine;
+    a.wbits = sh->wbits;
     a.sym = d->sym;
     a.batch = &d->b;
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
@@ -433,7 +442,7 @@ void bml_hip_sp2_ellpack_batch(int nsys, bml_matrix_ellpack_hip_t *const *H,
         Xc->aux->max_nnz = hmax + 1;
     }
     d.sym = 0;
-    if (bml_hip_get_row_engine() == BML_HIP_ROW_ENGINE_UNORDERED &&
+    if (bml_hip_get_row_engine() != BML_HIP_ROW_ENGINE_EXACT &&
         !bmlh_ok(bmlhip_is_symmetric(prec, N, M, Xc->value, Xc->index, Xc->nnz, &d.sym), what))
         goto out;
     if (!bmlh_ok(bmlhip_gershgorin_batch(prec, N, M, Xc->value, Xc->index, Xc->nnz, &d.b,
@@ -461,7 +470,7 @@ void bml_hip_sp2_ellpack_batch(int nsys, bml_matrix_ellpack_hip_t *const *H,
         bml_matrix_ellpack_hip_t *cur = single ? Xs : Xc, *cur2 = single ? X2s : X2c;
         const int mx = cur->aux->stats_valid ? cur->aux->max_nnz : cur->M;
         bmlh_spgemm_shape sh;
-        bmlh_pick_spgemm(cur->matrix_precision, N, mx, mx, touched, hint_max, &sh);
+        bmlh_pick_spgemm(cur->matrix_precision, N, mx, mx, touched, hint_max, 1, &sh);
         if (o.graph) {
             /* replay the iteration as a graph; capture anew when the launch
              * shape or the precision changes (the first pass runs eagerly so
@@ -605,7 +614,7 @@ static int eb_sp2_iteration(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hi
                                  X->indexb, X->nnzb, X2->MB, X2->value, X2->indexb, X2->nnzb,
                                  0.0, 0.0, o->threshold, 2, d_state->ab, &d_state->done),
                    what) &&
-           bmlh_ok(bmlhip_sp2_check(d_state, o->tol, d_hist, o->maxit), what);
+           bmlh_ok(bmlhip_sp2_check(d_state, o->tol, d_hist, o->maxit, 0), what);
 }
 
 /* values, block columns and counts of an ELLBLOCK matrix into one of the

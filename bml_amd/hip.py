@@ -157,7 +157,7 @@ _SIGS.update({
 _OPTIONAL: dict = {}
 
 BLOCK_ENGINE_AUTO, BLOCK_ENGINE_EXACT, BLOCK_ENGINE_MFMA, BLOCK_ENGINE_MFMA_BF16 = -1, 0, 1, 2
-ROW_ENGINE_EXACT, ROW_ENGINE_UNORDERED = 0, 1
+ROW_ENGINE_EXACT, ROW_ENGINE_UNORDERED, ROW_ENGINE_WINDOW = 0, 1, 2
 
 
 def header_symbols() -> list[str]:
@@ -547,7 +547,7 @@ def set_block_engine(engine: int) -> None:
 
 
 def set_row_engine(engine: int) -> None:
-    """bml_hip_set_row_engine: ROW_ENGINE_EXACT or ROW_ENGINE_UNORDERED."""
+    """bml_hip_set_row_engine: ROW_ENGINE_EXACT, ROW_ENGINE_UNORDERED or ROW_ENGINE_WINDOW."""
     _call("bml_hip_set_row_engine", engine)
 
 

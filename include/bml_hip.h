@@ -139,10 +139,20 @@ const char *bml_hip_version(void);
  *              accumulators, so a column's sum is formed in arrival order
  *              (agrees with EXACT to rounding and may vary in the last bits
  *              between runs) and the kept columns are written in table
- *              order.  Rows of shorter groups use EXACT.
+ *              order.  Rows of shorter groups use EXACT.  Squares (x2 and
+ *              the SP2 iterations) whose rows touch more distinct columns
+ *              than an LDS hash table holds (estimated from the previous
+ *              square of the same matrix) use WINDOW;
+ *   WINDOW     squares only, whatever their fill: the column range is cut
+ *              into windows of 2^11..2^13 columns and each output row adds
+ *              its products into a dense LDS window per window it touches
+ *              (no hashing), summed in arrival order like UNORDERED, kept
+ *              columns written sorted; other products run UNORDERED.  The
+ *              windows of one product number at most 64 (N <= 2^19).
  * No bml counterpart. */
 #define BML_HIP_ROW_ENGINE_EXACT 0
 #define BML_HIP_ROW_ENGINE_UNORDERED 1
+#define BML_HIP_ROW_ENGINE_WINDOW 2
 void bml_hip_set_row_engine(int engine);
 int bml_hip_get_row_engine(void);
 

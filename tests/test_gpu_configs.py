@@ -1,0 +1,259 @@
+"""Parity and property tests at the measured workloads' own parameters.
+
+* 32^3 SP2 lattice (onsite +-1 checkerboard, hopping -0.5 +- 0.05, threshold
+  1e-5): at M = 256 the reference exits with "Number of non-zeroes per row >
+  M" (bml_multiply_ellpack_typed.c:325-330) -- the oracle and the library
+  both raise it; with M sized to the fill (~2600 entries per row) the full
+  lattice runs on the window engine and is checked by size-independent
+  properties, and by the unfused path (x2, then add_norm) on the same
+  iterate.
+* 512-system batch (hopping -0.5, onsite +-1, M = 128, threshold 1e-5):
+  every system's density matrix fills beyond 128 entries per row, so M =
+  128 raises as the reference does; with M sized to the fill, systems drawn
+  from the batch's own size list are solved in one batch and each is
+  compared with the oracle's SP2 of that system.
+* 64^3 complex Peierls lattice (X = SP2 iterate 3, M = 512, threshold
+  1e-6): the X^2 of the same construction at L = 12 against the oracle, and
+  at full size (N = 262144) the Hermitian identities of X^2.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import orc
+from bml_amd import generators as G
+from bml_amd.ell import Ell
+from helpers import assert_ell_close, rel_close
+
+pytestmark = pytest.mark.gpu
+
+
+def csr(A: Ell):
+    r, c, v = A.to_coo()
+    return sp.csr_matrix((v, (r, c)), shape=(A.N, A.N))
+
+
+@pytest.fixture
+def engine(hip):
+    yield hip
+    hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+
+
+# ---- 32^3 SP2 lattice, hopping -0.5 +- 0.05 -------------------------------------
+
+def test_lattice32_m256_overflows_as_reference(engine):
+    """M = 256 as written: the oracle (the reference's x2 loop) and the
+    library's exact engine both stop with the M-overflow error once the
+    iterate outgrows 256 entries per row (iteration 3)."""
+    hip = engine
+    H = G.lattice_hamiltonian(32, 32, 32, M=16, hopping=-0.5, disorder=0.05)
+    with pytest.raises(RuntimeError):
+        orc.sp2(H, 256, H.N / 2, 1e-5, tol=1e-8, maxit=6)
+    X = hip.zero_matrix(H.precision, H.N, 256)
+    with pytest.raises(hip.BmlError) as e:
+        hip.sp2(hip.from_ell(H, 256), X, H.N / 2, threshold=1e-5, tol=1e-8, maxit=6)
+    assert e.value.code == 1
+
+
+@pytest.fixture(scope="module")
+def lattice32(hip):
+    """The 32^3 lattice after 14 FP64 SP2 iterations on the window engine
+    (M = 3200 holds the fill), with its history."""
+    hip.set_row_engine(hip.ROW_ENGINE_UNORDERED)
+    try:
+        H = G.lattice_hamiltonian(32, 32, 32, M=16, hopping=-0.5, disorder=0.05)
+        dH = hip.from_ell(H, 3200)
+        X = hip.zero_matrix(H.precision, H.N, 3200)
+        r, h = hip.sp2(dH, X, H.N / 2, threshold=1e-5, tol=1e-8, maxit=14)
+        return H, dH, X, r, h
+    finally:
+        hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+
+
+def test_lattice32_window_sp2_properties(lattice32):
+    H, dH, X, r, h = lattice32
+    assert r.iterations == 14 and not r.converged
+    A = X.get()
+    n = A.nnz
+    assert n.max() <= 3200 and n.min() > 1000  # the fill this workload reaches
+    S = csr(A)
+    assert (S != S.T).nnz == 0, "iterate not exactly symmetric"
+    m = A.live_mask()
+    assert np.all(np.abs(A.value[m]) > 1e-5), "an entry at or below the threshold was kept"
+    # the electron count: tr X -> N/2 (measured: within 2e-3 from iteration 12)
+    assert abs(h[-1, 0] - H.N / 2) < 1e-2 and abs(h[-1, 1] - H.N / 2) < 1e-2
+    # idempotency falls to the threshold floor (~1e-2 .. 1e-3 here)
+    assert h[-1, 2] < 1.0 and h[-1, 2] < h[5, 2]
+
+
+def test_lattice32_x2_hermitian_identities(lattice32, engine):
+    """X^2 of the full-size iterate on the window engine (full product, no
+    symmetry assumed): tr X = the sum of X's diagonal, tr X^2 = sum |X_ij|^2
+    for a symmetric X, and X^2 symmetric to rounding."""
+    hip = engine
+    H, dH, X, r, h = lattice32
+    hip.set_row_engine(hip.ROW_ENGINE_WINDOW)
+    X2 = hip.zero_matrix(H.precision, H.N, 4096)
+    tr = hip.multiply_x2(X, X2, 1e-5)
+    A = X.get()
+    S = csr(A)
+    assert rel_close(tr[0], float(S.diagonal().sum()), 1e-12)
+    assert rel_close(tr[1], float(np.sum(np.abs(A.value[A.live_mask()]) ** 2)), 1e-12)
+    S2 = csr(X2.get())
+    assert abs(S2 - S2.T).max() <= 1e-12 * abs(S2).max()
+
+
+def test_lattice32_fused_sp2_matches_client_steps(lattice32, engine):
+    """The fused window SP2 (product, update and norm in one pass, upper
+    triangle mirrored) against the steps a bml client runs, driven from the
+    host at full size: gershgorin, normalize, then per iteration
+    bml_multiply_x2 (window engine, full product), the branch from its
+    traces, bml_add_norm.  Same branches; traces, norms and X to rounding."""
+    hip = engine
+    H, dH, X, r, h = lattice32
+    hip.set_row_engine(hip.ROW_ENGINE_WINDOW)
+    nocc = H.N / 2
+    lo, hi = hip.gershgorin(dH)
+    Y = hip.copy_new(dH)
+    hip.normalize(Y, lo, hi)
+    Y2 = hip.zero_matrix(H.precision, H.N, 4096)
+    hist = []
+    for _ in range(h.shape[0]):
+        tr = hip.multiply_x2(Y, Y2, 1e-5)
+        br = 0 if abs(tr[1] - nocc) < abs(2 * tr[0] - tr[1] - nocc) else 1
+        a, b = (2.0, -1.0) if br else (0.0, 1.0)
+        n2 = hip.add_norm(Y, Y2, a, b, 1e-5)
+        hist.append((tr[0], tr[1], np.sqrt(n2), br))
+    hist = np.array(hist)
+    np.testing.assert_array_equal(h[:, 3], hist[:, 3], err_msg="branch history")
+    np.testing.assert_allclose(h[:, :2], hist[:, :2], rtol=1e-12)
+    np.testing.assert_allclose(h[:, 2], hist[:, 2], rtol=1e-9)
+    Sa, Sb = csr(X.get()), csr(Y.get())
+    assert abs(Sa - Sb).max() <= 1e-12 * abs(Sb).max()
+
+
+# ---- 512-system batch, hopping -0.5, M = 128 --------------------------------------
+
+def test_batch_m128_overflows_as_reference(engine):
+    """M = 128 as written: the density matrices of these systems fill beyond
+    128 entries per row (a 6x6x7 system's whole 252 columns), so the oracle
+    and the batched solve both stop with the M-overflow error."""
+    hip = engine
+    n = sorted(G.batch_sizes(512))[:3]
+    Hs = [G.lattice_hamiltonian(*G.near_cubic(int(k)), M=128, hopping=-0.5, disorder=0.0)
+          for k in n]
+    with pytest.raises(RuntimeError):
+        orc.sp2(Hs[0], 128, Hs[0].N // 2, 1e-5, tol=1e-8, maxit=60)
+    Xs = [hip.zero_matrix(H.precision, H.N, 128) for H in Hs]
+    with pytest.raises(hip.BmlError) as e:
+        hip.sp2_batch([hip.from_ell(H) for H in Hs], Xs, [H.N // 2 for H in Hs],
+                      threshold=1e-5, tol=1e-8, maxit=60)
+    assert e.value.code == 1
+
+
+@pytest.fixture(scope="module")
+def batch_draw():
+    """Systems drawn from the batch's own size list (every 64th size, so the
+    draw spans the log-uniform range; those up to 1400 sites, one per lattice
+    shape), hopping -0.5, no disorder, onsite +-1, and the oracle's SP2 of
+    each (threshold 1e-5, nocc = floor(N/2), M = N: the fill reaches whole
+    rows, 60 iterations)."""
+    shapes = []
+    for k in G.batch_sizes(512)[::64]:
+        L = G.near_cubic(int(k))
+        if L[0] * L[1] * L[2] <= 1400 and L not in shapes:
+            shapes.append(L)
+    Hs = [G.lattice_hamiltonian(*L, M=16, hopping=-0.5, disorder=0.0) for L in shapes]
+    refs = [orc.sp2(H, H.N, float(H.N // 2), 1e-5, tol=1e-8, maxit=60) for H in Hs]
+    return Hs, refs
+
+
+@pytest.mark.parametrize("row_engine", ["exact", "unordered"])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_systems_match_oracle(engine, batch_draw, row_engine, mixed):
+    """The drawn systems solved in one batch, each against the oracle's SP2.
+    FP64 on the exact engine: X bit-identical and the same iterations (its
+    sums are the reference's).  FP64 on the unordered engine: X within 1e-10
+    x max|X| (the odd lattices, 9^3 and 11^3, have no gap at the Fermi level
+    and do not converge; there the oracle alone moves X by 6e-12 after 60
+    iterations when H is perturbed by 1e-15).  Mixed precision (FP32
+    iterations, FP64 refinement): X within 2e-5 x max|X| of the oracle's
+    FP64 density matrix."""
+    hip = engine
+    Hs, refs = batch_draw
+    assert len(Hs) >= 3
+    hip.set_row_engine(hip.ROW_ENGINE_EXACT if row_engine == "exact"
+                       else hip.ROW_ENGINE_UNORDERED)
+    nocc = [float(H.N // 2) for H in Hs]
+    Xs = [hip.zero_matrix(H.precision, H.N, H.N) for H in Hs]
+    res = hip.sp2_batch([hip.from_ell(H, H.N) for H in Hs], Xs, nocc, threshold=1e-5,
+                        tol=1e-8, maxit=60, mixed=mixed)
+    for H, X, r, (ref, rit, rh) in zip(Hs, Xs, res, refs):
+        D, R = X.get().to_dense(), ref.to_dense()
+        scale = np.abs(R).max()
+        if mixed:
+            assert np.abs(D - R).max() <= 2e-5 * scale, f"N={H.N}"
+        elif row_engine == "exact":
+            assert r.iterations == rit, f"N={H.N}"
+            assert np.array_equal(D, R), f"N={H.N}: not bit-identical"
+        else:
+            assert r.iterations == rit, f"N={H.N}"
+            assert np.abs(D - R).max() <= 1e-10 * scale, f"N={H.N}"
+
+
+# ---- 64^3 complex Peierls lattice, X = SP2 iterate 3, M = 512, thr 1e-6 ---------
+
+def _peierls_iterate_oracle(L):
+    H = G.peierls_hamiltonian(L, M=16)
+    X, it, _ = orc.sp2(H, 512, H.N / 2, 1e-6, tol=1e-8, maxit=3)
+    return X
+
+
+@pytest.mark.parametrize("row_engine", ["exact", "unordered", "window"])
+def test_peierls_iterate_x2_matches_oracle(engine, row_engine):
+    """The Peierls workload's X^2 at L = 12 (X = SP2 iterate 3 built by the
+    oracle, M = 512, threshold 1e-6) against the oracle's X^2: exact engine
+    bit for bit, the others within 1e-12 x max|X^2|."""
+    hip = engine
+    X = _peierls_iterate_oracle(12)
+    ref, rtr = orc.x2(X, 512, 1e-6)
+    hip.set_row_engine({"exact": hip.ROW_ENGINE_EXACT, "unordered": hip.ROW_ENGINE_UNORDERED,
+                        "window": hip.ROW_ENGINE_WINDOW}[row_engine])
+    X2 = hip.zero_matrix(X.precision, X.N, 512)
+    tr = hip.multiply_x2(hip.from_ell(X), X2, 1e-6)
+    got = X2.get()
+    if row_engine == "exact":
+        from helpers import assert_ell_identical
+        assert_ell_identical(got, ref, "Peierls x2")
+    else:
+        assert_ell_close(got, ref, 1e-12, 1e-6, "Peierls x2")
+    assert rel_close(tr[0], rtr[0], 1e-12) and rel_close(tr[1], rtr[1], 1e-12)
+
+
+def test_peierls64_x2_hermitian_identities(engine):
+    """Full size (N = 262144): X = SP2 iterate 3 on the GPU, X^2 by the
+    default engine and the unordered engine; X^2 is Hermitian to rounding
+    and tr X^2 = sum |X_ij|^2 (X Hermitian), both within 1e-12; the two
+    engines' X^2 agree within 1e-12 x max|X^2|."""
+    hip = engine
+    H = G.peierls_hamiltonian(64, M=16)
+    dH = hip.from_ell(H, 512)
+    hip.set_row_engine(hip.ROW_ENGINE_UNORDERED)
+    X = hip.zero_matrix(H.precision, H.N, 512)
+    hip.sp2(dH, X, H.N / 2, threshold=1e-6, tol=1e-8, maxit=3)
+    A = X.get()
+    SA = csr(A)
+    assert abs(SA - SA.conj().T).max() == 0.0
+    out = {}
+    for eng in (hip.ROW_ENGINE_EXACT, hip.ROW_ENGINE_UNORDERED):
+        hip.set_row_engine(eng)
+        X2 = hip.zero_matrix(H.precision, H.N, 512)
+        tr = hip.multiply_x2(X, X2, 1e-6)
+        S = csr(X2.get())
+        X2.free()
+        assert abs(S - S.conj().T).max() <= 1e-12 * abs(S).max()
+        assert rel_close(tr[1], float(np.sum(np.abs(A.value[A.live_mask()]) ** 2)), 1e-12)
+        assert rel_close(tr[0], float(SA.diagonal().real.sum()), 1e-12)
+        out[eng] = S
+    d = abs(out[hip.ROW_ENGINE_EXACT] - out[hip.ROW_ENGINE_UNORDERED])
+    assert d.max() <= 1e-12 * abs(out[hip.ROW_ENGINE_EXACT]).max()

@@ -1,0 +1,181 @@
+"""The window row engine (bml_hip_set_row_engine(WINDOW), and the unordered
+setting's choice for squares whose rows touch more columns than an LDS hash
+table holds).
+
+Each output row adds its products into a dense LDS window of 2^11..2^13
+columns per window it touches, so sums are formed in arrival order and rows
+come out sorted by column.  Results are compared with the oracle (the
+reference's sums, bml_multiply_ellpack_typed.c:130-349 and the SP2 steps a
+bml client runs) as sets per row: values within 1e-12 x max|ref| in double
+precision (2e-5 in single), patterns equal except entries that close to the
+threshold, traces to the same relative tolerance.
+"""
+import numpy as np
+import pytest
+
+import orc
+from bml_amd import generators as G
+from helpers import DTYPES, assert_ell_close, rel_close
+
+pytestmark = pytest.mark.gpu
+
+
+def vtol(dtype):
+    return 1e-12 if np.dtype(dtype) in (np.float64, np.complex128) else 2e-5
+
+
+@pytest.fixture
+def window(hip):
+    hip.set_row_engine(hip.ROW_ENGINE_WINDOW)
+    assert hip.get_row_engine() == hip.ROW_ENGINE_WINDOW
+    yield hip
+    hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(600, 160, 120), (2500, 128, 100), (257, 64, 40)])
+def test_x2_window_matches_oracle(window, dtype, shape):
+    hip = window
+    N, M, fill = shape
+    A = G.random_ell(N, M, fill, 700 + N, dtype)
+    thr = 0.05
+    ref, rtr = orc.x2(A, N, thr)
+    X2 = hip.zero_matrix(A.precision, N, N)
+    tr = hip.multiply_x2(hip.from_ell(A), X2, thr)
+    got = X2.get()
+    assert_ell_close(got, ref, vtol(dtype), thr, f"x2 {shape}")
+    for i in range(N):  # the window engine writes each row sorted by column
+        cols = got.index[i, : got.nnz[i]]
+        assert np.all(np.diff(cols) > 0), f"row {i} not sorted"
+    assert rel_close(tr[0], rtr[0], vtol(dtype)) and rel_close(tr[1], rtr[1], vtol(dtype))
+
+
+@pytest.mark.parametrize("wbits", [11, 12, 13])
+@pytest.mark.parametrize("waves", [4, 8])
+def test_window_sizes(window, monkeypatch, wbits, waves):
+    """Windows of 2^11..2^13 columns over N = 9000 (2..5 windows), four or
+    eight wavefronts per row."""
+    hip = window
+    A = G.random_ell(9000, 96, 80, 31, np.float64)
+    ref, rtr = orc.x2(A, 9000, 0.02)
+    monkeypatch.setenv("BML_HIP_WINDOW_BITS", str(wbits))
+    monkeypatch.setenv("BML_HIP_WAVES", str(waves))
+    X2 = hip.zero_matrix(A.precision, A.N, 4096)
+    tr = hip.multiply_x2(hip.from_ell(A), X2, 0.02)
+    assert_ell_close(X2.get(), ref, 1e-12, 0.02, f"window 2^{wbits} waves {waves}")
+    assert rel_close(tr[0], rtr[0], 1e-12) and rel_close(tr[1], rtr[1], 1e-12)
+
+
+def test_window_empty_rows_and_no_diagonal(window):
+    """Rows with no entries, rows whose product never reaches the diagonal
+    (no diagonal entry is stored then, as the reference's first-touch list
+    never holds it), N = 1."""
+    hip = window
+    A = G.random_ell(700, 64, 30, 41, np.float64, diag=False)
+    for i in range(0, 700, 7):
+        A.nnz[i] = 0
+    ref, rtr = orc.x2(A, 700, 0.01)
+    X2 = hip.zero_matrix(A.precision, 700, 700)
+    tr = hip.multiply_x2(hip.from_ell(A), X2, 0.01)
+    assert_ell_close(X2.get(), ref, 1e-12, 0.01, "empty rows")
+    assert rel_close(tr[1], rtr[1], 1e-12)
+    one = G.random_ell(1, 4, 1, 42, np.float64)
+    ref1, _ = orc.x2(one, 1, 0.0)
+    Y = hip.zero_matrix(one.precision, 1, 4)
+    hip.multiply_x2(hip.from_ell(one), Y, 0.0)
+    assert_ell_close(Y.get(), ref1, 1e-12, 0.0, "N=1")
+
+
+def test_window_m_overflow_raises(window):
+    """A product row with more kept entries than X2's M is the reference's
+    fatal "Number of non-zeroes per row > M" (bml_multiply_ellpack_typed.c:
+    325-330); the library stays usable afterwards."""
+    hip = window
+    A = G.random_ell(500, 64, 60, 43, np.float64)
+    X2 = hip.zero_matrix(A.precision, 500, 32)
+    with pytest.raises(hip.BmlError) as e:
+        hip.multiply_x2(hip.from_ell(A), X2, 0.0)
+    assert e.value.code == 1
+    ref, _ = orc.x2(A, 500, 0.0)
+    Y = hip.zero_matrix(A.precision, 500, 500)
+    hip.multiply_x2(hip.from_ell(A), Y, 0.0)
+    assert_ell_close(Y.get(), ref, 1e-12, 0.0, "after overflow")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
+def test_window_symmetric_x2(window, monkeypatch, dtype):
+    """The symmetric path the SP2 drivers take (upper triangle formed, the
+    rest mirrored): a lattice SP2 iterate, squared."""
+    hip = window
+    if np.dtype(dtype).kind == "c":
+        H = G.peierls_hamiltonian(L=8, M=64, dtype=dtype)
+    else:
+        H = G.lattice_hamiltonian(10, 10, 10, M=64, hopping=-0.5, disorder=0.05, dtype=dtype)
+    # X = the normalized H, squared twice (a symmetric matrix with wide rows)
+    lo, hi = orc.gershgorin(H)
+    X = orc.normalize(H.with_m(H.N), lo, hi)
+    X, _ = orc.x2(X, X.N, 1e-6)
+    X, _ = orc.x2(X, X.N, 1e-6)
+    ref, rtr = orc.x2(X, X.N, 1e-6)
+    monkeypatch.setenv("BML_HIP_ASSUME_SYMMETRIC", "1")
+    X2 = hip.zero_matrix(X.precision, X.N, X.N)
+    tr = hip.multiply_x2(hip.from_ell(X), X2, 1e-6)
+    got = X2.get()
+    assert_ell_close(got, ref, vtol(dtype), 1e-6, "symmetric x2")
+    D = got.to_dense()
+    assert np.array_equal(D, D.conj().T), "mirrored product not exactly Hermitian"
+    assert rel_close(tr[1], rtr[1], vtol(dtype))
+
+
+def test_sp2_window_matches_oracle_hop05(window):
+    """SP2 on the 10^3 lattice with the hopping of the 32^3 SP2 workload
+    (-0.5 +- 0.05 disorder, onsite +-1, threshold 1e-5): the fused window
+    step (X^2's upper triangle, the update, the norm, in place) against the
+    oracle's SP2 (bml_multiply_x2 + bml_add_norm per iteration, the
+    reference's sums).  The branch history agrees exactly; X, traces and the
+    idempotency measure agree to rounding (1e-12 x max|X|; measured on the
+    oracle alone, a 1e-15 perturbation of H moves this solve's X by 1e-15)."""
+    hip = window
+    H = G.lattice_hamiltonian(10, 10, 10, M=64, hopping=-0.5, disorder=0.05)
+    ref, rit, rh = orc.sp2(H, 1000, 500.0, 1e-5, tol=1e-8, maxit=40)
+    X = hip.zero_matrix(H.precision, H.N, 1000)
+    r, h = hip.sp2(hip.from_ell(H), X, 500.0, threshold=1e-5, tol=1e-8, maxit=40)
+    assert r.iterations == rit
+    np.testing.assert_array_equal(h[:, 3], rh[:, 3], err_msg="branch history")
+    np.testing.assert_allclose(h[:, :2], rh[:, :2], rtol=1e-12)
+    np.testing.assert_allclose(h[:, 2], rh[:, 2], rtol=1e-9, atol=1e-13)
+    assert_ell_close(X.get(), ref, 1e-12, 1e-5, "SP2 density matrix")
+
+
+def test_sp2_window_two_windows(window, monkeypatch):
+    """SP2 on the 16^3 lattice (hopping -0.1) with windows of 2^11 columns
+    (two windows per row range) against the oracle: converged, same
+    iterations, X within 1e-12 x max|X|."""
+    hip = window
+    monkeypatch.setenv("BML_HIP_WINDOW_BITS", "11")
+    H = G.lattice_hamiltonian(16, 16, 16, M=16, hopping=-0.1, disorder=0.01)
+    ref, rit, rh = orc.sp2(H, 512, H.N / 2, 1e-7, tol=1e-9, maxit=60)
+    X = hip.zero_matrix(H.precision, H.N, 512)
+    r, h = hip.sp2(hip.from_ell(H), X, H.N / 2, threshold=1e-7, tol=1e-9, maxit=60)
+    assert r.converged and abs(r.iterations - rit) <= 1
+    np.testing.assert_array_equal(h[: min(rit, r.iterations), 3],
+                                  rh[: min(rit, r.iterations), 3])
+    assert_ell_close(X.get(), ref, 1e-9, 1e-7, "SP2 two windows")
+
+
+def test_sp2_window_mixed_against_oracle(window):
+    """Mixed precision on the window engine (single-precision iterations,
+    FP64 refinement) against the oracle's FP64 SP2 on the 10^3 lattice at
+    hopping -0.5: the density matrix within 2e-5 of the FP64 one (the
+    single-precision phase leaves about eps_f32 x ||H|| / gap in the occupied
+    subspace, which FP64 polynomial steps cannot rotate back), the trace
+    within 1e-6 of the oracle's."""
+    hip = window
+    H = G.lattice_hamiltonian(10, 10, 10, M=64, hopping=-0.5, disorder=0.05)
+    ref, rit, rh = orc.sp2(H, 1000, 500.0, 1e-5, tol=1e-8, maxit=40)
+    X = hip.zero_matrix(H.precision, H.N, 1000)
+    r, h = hip.sp2(hip.from_ell(H), X, 500.0, threshold=1e-5, tol=1e-8, maxit=40, mixed=True)
+    assert r.single_iterations > 0 and r.iterations == 40
+    D, R = X.get().to_dense(), ref.to_dense()
+    assert np.abs(D - R).max() <= 2e-5 * np.abs(R).max()
+    assert abs(np.trace(D) - np.trace(R)) <= 1e-6 * 500
