@@ -1188,14 +1188,19 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_win(WinParams p) {
     }
     __syncthreads();
     // the windows row i touches: the union of its rows' masks
+    // (and the row's partial products, sum of nnz(j): the full product's
+    // count, also when SYM forms half of them)
     unsigned long long m = 0ull;
+    long long prods = 0;  // this thread's share, then the wavefront's
     for (int e = tid; e < nA; e += NT) {
         T a;
         int j;
         pk_get<T>(P, rowP + e, a, j);
         m |= p.wmask[j];
+        prods += p.boff[(size_t) j * nw1 + p.nwin];
         if (!FUSE && j == i) trx_s = a;  // one entry per row holds the diagonal
     }
+    for (int off = 32; off > 0; off >>= 1) prods += __shfl_down(prods, off, 64);
     m = or64_wave(m);
     if (lane == 0 && m) atomicOr(&mask_s, m);
     __syncthreads();
@@ -1211,7 +1216,6 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_win(WinParams p) {
     const size_t rowC = (size_t) i * MC;
     T *Cv = (T *) p.Cv;
     int kept = 0, distinct = 0;  // block-uniform running counts
-    long long prods = 0;         // this wavefront's products
     double nrm = 0.0;            // this lane's norm partial
     while (m) {
         const int c = __ffsll((long long) m) - 1;
@@ -1246,7 +1250,6 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_win(WinParams p) {
                 const int l0 = __builtin_amdgcn_readlane(lo, q);
                 const int l1 = __builtin_amdgcn_readlane(hi, q);
                 const size_t rb = (size_t) j * p.MP;
-                prods += l1 - l0;
                 int k0 = l0 + lane;
                 // two entries per lane per round, branch-free: both loads
                 // issue before the adds, and (SYM) the products of columns
