@@ -1205,6 +1205,9 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_win(WinParams p) {
     if (lane == 0 && m) atomicOr(&mask_s, m);
     __syncthreads();
     m = mask_s;
+    // the update alpha X + beta X^2 also keeps X's own entries in windows
+    // that no product reaches (a row without its diagonal entry)
+    if constexpr (FUSE) m |= p.wmask[i];
     if constexpr (SYM) m &= ~((1ull << (i >> WB)) - 1ull);
     double alpha = 0.0, beta = 0.0;
     if constexpr (FUSE) {
