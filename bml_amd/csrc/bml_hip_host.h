@@ -44,5 +44,8 @@ int bmlh_block_engine_for(const bml_matrix_ellblock_hip_t *A);
 void bmlh_put_scalar(int prec, double s, void *buf);
 struct bml_hip_aux *bmlh_aux_new(void);
 void bmlh_set_stats(struct bml_hip_aux *aux, const bmlhip_status *st);
+/* The row-major dense N x N host array (malloc'd) of an ellpack_hip or
+ * ellblock_hip matrix A, its elements cast to precision; N into *N_out. */
+void *bmlh_dense_as(const void *A, int precision, int *N_out, const char *what);
 
 #endif

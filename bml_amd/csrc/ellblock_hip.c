@@ -976,3 +976,67 @@ fail:
     bml_deallocate_ellblock_hip(B);
     return NULL;
 }
+
+/* ---- bml_convert ------------------------------------------------------------ */
+
+bml_matrix_ellblock_hip_t *bml_convert_ellblock_hip(const void *A, int precision, int M,
+                                                    int distrib_mode)
+{
+    const char *what = "bml_convert_ellblock_hip";
+    int N = 0;
+    char *D = bmlh_dense_as(A, precision, &N, what);
+    if (!D)
+        return NULL;
+    /* the registered block sizes (set from N if none are); every block of
+     * the NB x NB block grid is stored, zeros included, in block-column order */
+    const int *bsize = bml_hip_get_block_sizes(N, N);
+    const int NB = bml_hip_get_nb();
+    int tot = 0;
+    for (int ib = 0; bsize && ib < NB; ib++)
+        tot += bsize[ib];
+    if (!bsize || tot != N) {
+        free(D);
+        if (bsize)
+            bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: registered block sizes cover %d rows, not %d",
+                       what, tot, N);
+        return NULL;
+    }
+    bml_matrix_ellblock_hip_t *B =
+        bml_block_matrix_ellblock_hip(precision, NB, NB, M, bsize, distrib_mode);
+    if (!B) {
+        free(D);
+        return NULL;
+    }
+    const int bt = B->bt;
+    const size_t es = bmlh_elem_size(precision), tile = (size_t) bt * bt;
+    char *v = calloc((size_t) NB * NB * tile + 1, es);
+    int *ix = calloc((size_t) NB * NB + 1, sizeof(int));
+    int *nn = calloc((size_t) NB + 1, sizeof(int));
+    if (!v || !ix || !nn) {
+        bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+        bml_deallocate_ellblock_hip(B);
+        B = NULL;
+        goto out;
+    }
+    for (int ib = 0, r0 = 0; ib < NB; r0 += bsize[ib], ib++) {
+        nn[ib] = NB;
+        for (int jb = 0, c0 = 0; jb < NB; c0 += bsize[jb], jb++) {
+            const size_t slot = (size_t) ib * NB + jb;
+            ix[slot] = jb;
+            for (int ii = 0; ii < bsize[ib]; ii++)
+                memcpy(v + (slot * tile + (size_t) ii * bt) * es,
+                       D + ((size_t) (r0 + ii) * N + c0) * es, (size_t) bsize[jb] * es);
+        }
+    }
+    bml_ellblock_hip_set_blocks(B, v, ix, nn, NB, bt);
+    if (bmlh_failed()) {
+        bml_deallocate_ellblock_hip(B);
+        B = NULL;
+    }
+out:
+    free(v);
+    free(ix);
+    free(nn);
+    free(D);
+    return B;
+}

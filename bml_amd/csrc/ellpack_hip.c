@@ -865,3 +865,128 @@ out:
     free(ix);
     free(nn);
 }
+
+/* ---- bml_convert ------------------------------------------------------------ */
+
+/* One element of precision pin at src, as an element of precision pout at dst:
+ * float <-> double as C converts them, a real value gets a zero imaginary
+ * part, and a complex value keeps its real part (C's complex-to-real
+ * conversion). */
+static void cast_elem(int pin, const void *src, int pout, void *dst)
+{
+    double re, im = 0.0;
+    switch (pin) {
+    case BML_HIP_SINGLE_REAL: re = *(const float *) src; break;
+    case BML_HIP_DOUBLE_REAL: re = *(const double *) src; break;
+    case BML_HIP_SINGLE_COMPLEX:
+        re = ((const float *) src)[0];
+        im = ((const float *) src)[1];
+        break;
+    default:
+        re = ((const double *) src)[0];
+        im = ((const double *) src)[1];
+        break;
+    }
+    switch (pout) {
+    case BML_HIP_SINGLE_REAL: *(float *) dst = (float) re; break;
+    case BML_HIP_DOUBLE_REAL: *(double *) dst = re; break;
+    case BML_HIP_SINGLE_COMPLEX:
+        ((float *) dst)[0] = (float) re;
+        ((float *) dst)[1] = (float) im;
+        break;
+    default:
+        ((double *) dst)[0] = re;
+        ((double *) dst)[1] = im;
+        break;
+    }
+}
+
+void *bmlh_dense_as(const void *A, int precision, int *N_out, const char *what)
+{
+    if (!A) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: matrix is NULL", what);
+        return NULL;
+    }
+    const size_t es = bmlh_elem_size(precision);
+    if (!es) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: unknown precision %d", what, precision);
+        return NULL;
+    }
+    const int type = *(const int *) A;
+    int N, pin;
+    void *D;
+    if (type == BML_HIP_TYPE_ELLPACK) {
+        const bml_matrix_ellpack_hip_t *E = A;
+        N = E->N;
+        pin = E->matrix_precision;
+        D = bml_export_to_dense_ellpack_hip(E, BML_HIP_DENSE_ROW_MAJOR);
+    } else if (type == BML_HIP_TYPE_ELLBLOCK) {
+        const bml_matrix_ellblock_hip_t *E = A;
+        N = E->N;
+        pin = E->matrix_precision;
+        D = bml_export_to_dense_ellblock_hip(E, BML_HIP_DENSE_ROW_MAJOR);
+    } else {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: unknown matrix type %d", what, type);
+        return NULL;
+    }
+    if (!D || bmlh_failed()) {
+        free(D);
+        return NULL;
+    }
+    if (pin != precision) {
+        const size_t nn = (size_t) N * N, ein = bmlh_elem_size(pin);
+        char *C = malloc(nn * es + 1);
+        if (!C) {
+            free(D);
+            bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+            return NULL;
+        }
+        for (size_t k = 0; k < nn; k++)
+            cast_elem(pin, (const char *) D + k * ein, precision, C + k * es);
+        free(D);
+        D = C;
+    }
+    *N_out = N;
+    return D;
+}
+
+bml_matrix_ellpack_hip_t *bml_convert_ellpack_hip(const void *A, int precision, int M,
+                                                  int distrib_mode)
+{
+    const char *what = "bml_convert_ellpack_hip";
+    int N = 0;
+    char *D = bmlh_dense_as(A, precision, &N, what);
+    if (!D)
+        return NULL;
+    /* every row holds all N columns, zeros included (bml_set_element per
+     * element), so M must hold N */
+    if (N > M) {
+        free(D);
+        bmlh_error(BML_HIP_ERR_M_OVERFLOW, "%s: row 0: Number of non-zeroes per row > M", what);
+        return NULL;
+    }
+    const size_t es = bmlh_elem_size(precision);
+    bml_matrix_ellpack_hip_t *B = bml_zero_matrix_ellpack_hip(precision, N, M, distrib_mode);
+    char *v = B ? calloc((size_t) N * M + 1, es) : NULL;
+    int *ix = B ? calloc((size_t) N * M + 1, sizeof(int)) : NULL;
+    int *nn = B ? calloc((size_t) N + 1, sizeof(int)) : NULL;
+    if (B && (!v || !ix || !nn)) {
+        bmlh_error(BML_HIP_ERR_ALLOC, "%s: host allocation failed", what);
+        bml_deallocate_ellpack_hip(B);
+        B = NULL;
+    }
+    if (B) {
+        for (int i = 0; i < N; i++) {
+            memcpy(v + (size_t) i * M * es, D + (size_t) i * N * es, (size_t) N * es);
+            for (int j = 0; j < N; j++)
+                ix[(size_t) i * M + j] = j;
+            nn[i] = N;
+        }
+        bml_ellpack_hip_set_arrays(B, v, ix, nn, M);
+    }
+    free(v);
+    free(ix);
+    free(nn);
+    free(D);
+    return B;
+}

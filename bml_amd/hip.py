@@ -97,6 +97,7 @@ _SIGS = {
     "bml_copy_ellpack_hip": (None, [_P, _P]),
     "bml_import_from_dense_ellpack_hip": (_P, [_I, _I, _I, _P, _D, _I, _I]),
     "bml_export_to_dense_ellpack_hip": (_P, [_P, _I]),
+    "bml_convert_ellpack_hip": (_P, [_P, _I, _I, _I]),
     "bml_write_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
     "bml_read_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
     "bml_write_bml_matrix_ellblock_hip": (None, [_P, C.c_char_p]),
@@ -141,6 +142,7 @@ _SIGS.update({
     "bml_ellblock_hip_set_blocks": (None, [_P, _P, _P, _P, _I, _I]),
     "bml_ellblock_hip_get_blocks": (None, [_P, _P, _P, _P, _I]),
     "bml_export_to_dense_ellblock_hip": (_P, [_P, _I]),
+    "bml_convert_ellblock_hip": (_P, [_P, _I, _I, _I]),
     "bml_import_from_dense_ellblock_hip": (_P, [_I, _I, _I, _P, C.c_double, _I, _I]),
     "bml_multiply_x2_ellblock_hip": (_P, [_P, _P, _D]),
     "bml_multiply_ellblock_hip": (None, [_P, _P, _P, _D, _D, _D]),
@@ -341,6 +343,18 @@ def export_to_dense(A: EllpackHip, order: int = 0) -> np.ndarray:
     out = np.frombuffer(buf.tobytes(), A.dtype).reshape(A.N, A.N)
     _libc.free(p)
     return out
+
+
+def convert(A, precision: int, M: int) -> EllpackHip:
+    """bml_convert to ellpack_hip: every element of A (ellpack_hip or
+    ellblock_hip) stored, row by row in column order."""
+    return EllpackHip(_call("bml_convert_ellpack_hip", A.ptr, precision, M, 0))
+
+
+def eb_convert(A, precision: int, M: int) -> "EllblockHip":
+    """bml_convert to ellblock_hip: every block of the registered block grid
+    stored."""
+    return EllblockHip(_call("bml_convert_ellblock_hip", A.ptr, precision, M, 0))
 
 
 def copy_new(A: EllpackHip) -> EllpackHip:

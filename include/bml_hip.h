@@ -185,6 +185,15 @@ bml_matrix_ellpack_hip_t *bml_import_from_dense_ellpack_hip(
 /* bml_export_to_dense_ellpack (ellpack/bml_export_ellpack_typed.c:18-73):
  * returns a malloc'd N*N host array. */
 void *bml_export_to_dense_ellpack_hip(const bml_matrix_ellpack_hip_t *A, int order);
+/* bml_convert_ellpack (ellpack/bml_convert_ellpack_typed.c:13-40, reached
+ * through bml_convert.c:21-51 with matrix_type ellpack_hip): a new N x M
+ * matrix whose row i holds every column j = 0..N-1 in order with A(i, j),
+ * zeros included (bml_set_element for each element).  A is an ellpack_hip or
+ * ellblock_hip matrix; elements are cast to precision as C converts them
+ * (the reference reinterprets A's element bytes as the new precision).
+ * N > M is the M-overflow error (the reference writes past the row). */
+bml_matrix_ellpack_hip_t *bml_convert_ellpack_hip(const void *A, int precision, int M,
+                                                  int distrib_mode);
 /* Upload/download the ELLPACK-R arrays of a host bml ellpack matrix
  * (value/index/nnz of bml_matrix_ellpack_t, bml_types_ellpack.h:19-26).
  * src_M is the row stride of the host arrays; rows must fit A->M. */
@@ -365,6 +374,12 @@ bml_matrix_ellblock_hip_t *bml_import_from_dense_ellblock_hip(int precision, int
 /* bml_export_to_dense_ellblock (ellblock/bml_export_ellblock_typed.c):
  * returns a malloc'd N*N host array. */
 void *bml_export_to_dense_ellblock_hip(const bml_matrix_ellblock_hip_t *A, int order);
+/* bml_convert_ellblock (ellblock/bml_convert_ellblock_typed.c:16-72): a new
+ * matrix with the registered block sizes (bml_get_block_sizes(N, N)) and
+ * MB = NB, every block of the block grid stored, zeros included, in
+ * block-column order.  A and the element cast as for bml_convert_ellpack_hip. */
+bml_matrix_ellblock_hip_t *bml_convert_ellblock_hip(const void *A, int precision, int M,
+                                                    int distrib_mode);
 
 /* ---- SP2 purification ----------------------------------------------------
  * No bml counterpart: client codes (PROGRESS, LATTE) build SP2 from

@@ -233,3 +233,42 @@ def eb_threshold(A: Blk, thr: float) -> Blk:
 
 def eb_trace(A: Blk) -> float:
     return float(_fn("eb_trace", A.dtype, C.c_double)(*_blk_head(A), *_blk_args(A)))
+
+
+def convert(A, M: int) -> Ell:
+    """bml_convert to ellpack (ellpack/bml_convert_ellpack_typed.c:13-40):
+    bml_set_element(B, i, j, bml_get_element(A, i, j)) for every (i, j) in
+    row-major order.  bml_get_element yields A(i, j) or zero
+    (ellpack/bml_getters_ellpack_typed.c:19-55), and bml_set_element appends
+    a column not yet in the row (ellpack/bml_setters_ellpack_typed.c:63-107),
+    so row i holds all N columns in order, zeros included.  A is an Ell or a
+    Blk of B's precision.  N > M raises (the reference writes past the row)."""
+    D = A.to_dense()
+    N = D.shape[0]
+    if N > M:
+        raise OracleError("convert: Number of non-zeroes per row > M")
+    value = np.zeros((N, M), D.dtype)
+    index = np.zeros((N, M), np.int32)
+    value[:, :N] = D
+    index[:, :N] = np.arange(N, dtype=np.int32)
+    return Ell(value, index, np.full(N, N, np.int32))
+
+
+def eb_convert(A, bsize, bmax: int | None = None) -> Blk:
+    """bml_convert to ellblock (ellblock/bml_convert_ellblock_typed.c:16-72):
+    the block grid of bsize (bml_get_block_sizes(N, N)) with MB = NB and
+    every block stored, zeros included, in block-column order, each filled
+    element by element from bml_get_element(A, i, j)."""
+    D = A.to_dense()
+    bsize = np.asarray(bsize, np.int32)
+    NB = len(bsize)
+    if int(bsize.sum()) != D.shape[0]:
+        raise OracleError("eb_convert: block sizes do not cover N")
+    out = Blk.zeros(bsize, NB, D.dtype, bmax or int(bsize.max()))
+    off = np.concatenate([[0], np.cumsum(bsize)])
+    for ib in range(NB):
+        out.nnzb[ib] = NB
+        for jb in range(NB):
+            out.indexb[ib, jb] = jb
+            out.value[ib, jb, :bsize[ib], :bsize[jb]] = D[off[ib]:off[ib + 1], off[jb]:off[jb + 1]]
+    return out

@@ -382,6 +382,33 @@ int main(int argc, char **argv)
                                                     S->MB, S->N, S->bsize, sequential);
         bml_read_bml_matrix(A, argv[2]);
         write_blk(argv[4], A);
+    } else if (!strcmp(op, "convert")) {
+        /* convert <in.ell> <M> <out.ell>: bml_convert to ellpack */
+        NEED(5);
+        bml_matrix_ellpack_t *A = read_ell(argv[2], 0);
+        bml_matrix_ellpack_t *B = bml_convert(A, ellpack, A->matrix_precision, atoi(argv[3]),
+                                              sequential);
+        write_ell(argv[4], B);
+    } else if (!strcmp(op, "convert_blk")) {
+        /* convert_blk <in.blk> <M> <out.ell>: bml_convert of an ellblock
+         * matrix to ellpack */
+        NEED(5);
+        bml_matrix_ellblock_t *A = read_blk(argv[2], 0);
+        bml_matrix_ellpack_t *B = bml_convert(A, ellpack, A->matrix_precision, atoi(argv[3]),
+                                              sequential);
+        write_ell(argv[4], B);
+    } else if (!strcmp(op, "eb_convert")) {
+        /* eb_convert <in.ell> <M> <out.blk>: bml_convert of an ellpack matrix
+         * to ellblock (the default block sizes of the registry) */
+        NEED(5);
+        bml_matrix_ellpack_t *A = read_ell(argv[2], 0);
+        bml_matrix_ellblock_t *B = bml_convert(A, ellblock, A->matrix_precision, atoi(argv[3]),
+                                               sequential);
+        g_bmax = 0;
+        for (int ib = 0; ib < B->NB; ib++)
+            if (B->bsize[ib] > g_bmax)
+                g_bmax = B->bsize[ib];
+        write_blk(argv[4], B);
     } else if (!strcmp(op, "eb_x2")) {
         NEED(6);
         bml_matrix_ellblock_t *X = read_blk(argv[2], 0);

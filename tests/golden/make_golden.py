@@ -277,6 +277,34 @@ def mm_cases(tmp):
     return cases
 
 
+def convert_cases(tmp):
+    """bml_convert: ellpack -> ellpack, ellblock -> ellpack, and ellpack ->
+    ellblock with the registry's default block sizes for N (4-row blocks:
+    N = 22 gives 4, 4, 4, 4, 4, 2), each a fresh reference process."""
+    cases = {}
+    for dt in DTYPES:
+        t = TAG[dt]
+        A = G.random_ell(22, 8, 4, 51, dt)
+        fa, fo, fb = (os.path.join(tmp, n) for n in ("a.ell", "o.ell", "b.blk"))
+        A.write(fa)
+        d = {"N": 22, "M": 24}
+        pack_ell("A", A, d)
+        rc, _, _ = run("convert", fa, 24, fo)
+        assert rc == 0
+        pack_ell("ell", Ell.read(fo), d)
+        rc, _, _ = run("eb_convert", fa, 22, os.path.join(tmp, "o.blk"))
+        assert rc == 0
+        pack_blk("eb", Blk.read(os.path.join(tmp, "o.blk")), d)
+        B = G.random_blk([3, 5, 2, 4, 1, 6, 3, 2], 5, 3, 53, dt)
+        B.write(fb)
+        pack_blk("B", B, d)
+        rc, _, _ = run("convert_blk", fb, 26, fo)
+        assert rc == 0
+        pack_ell("blk", Ell.read(fo), d)
+        cases[f"cv_{t}"] = d
+    return cases
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run make -C oracle ref")
@@ -284,7 +312,8 @@ def main():
     only = sys.argv[1:]
     with tempfile.TemporaryDirectory() as tmp:
         for fname, fn in (("ellpack.npz", ellpack_cases), ("ellblock.npz", ellblock_cases),
-                          ("sp2.npz", sp2_cases), ("mm.npz", mm_cases)):
+                          ("sp2.npz", sp2_cases), ("mm.npz", mm_cases),
+                          ("convert.npz", convert_cases)):
             if only and fname.split(".")[0] not in only:
                 continue
             cases = fn(tmp)

@@ -211,3 +211,29 @@ def test_scale_add_identity_is_dense_alpha_a_plus_beta_i(dtype):
     tol = 1e-6 if np.dtype(dtype).itemsize <= 8 and dtype != np.float64 else 1e-15
     np.testing.assert_allclose(got.to_dense(), ref, atol=tol)
     assert ((got.to_dense() != 0) == (ref != 0)).all()
+
+
+CONV = load_cases("convert.npz")
+
+
+@pytest.mark.parametrize("case", sorted(CONV))
+def test_convert_bit_identical(case):
+    """bml_convert (ellpack/bml_convert_ellpack_typed.c:13-40,
+    ellblock/bml_convert_ellblock_typed.c:16-72): the oracle against the
+    reference's output, and the closed form (the same dense matrix, every
+    element stored, zeros included)."""
+    d = CONV[case]
+    A, B = ell_of(d, "A"), blk_of(d, "B")
+    ref = ell_of(d, "ell")
+    got = orc.convert(A, 24)
+    assert_ell_identical(got, ref, "ellpack -> ellpack")
+    assert np.all(ref.nnz == A.N) and np.array_equal(ref.to_dense(), A.to_dense())
+    refb = ell_of(d, "blk")
+    assert_ell_identical(orc.convert(B, 26), refb, "ellblock -> ellpack")
+    assert np.array_equal(refb.to_dense(), B.to_dense())
+    refe = blk_of(d, "eb")
+    assert refe.bsize.tolist() == [4, 4, 4, 4, 4, 2]  # the default 4-row blocks
+    assert_blk_identical(orc.eb_convert(A, refe.bsize, refe.bmax), refe, "ellpack -> ellblock")
+    assert np.all(refe.nnzb == refe.NB) and np.array_equal(refe.to_dense(), A.to_dense())
+    with pytest.raises(orc.OracleError):
+        orc.convert(A, A.N - 1)
