@@ -57,14 +57,14 @@ def test_lattice32_m256_overflows_as_reference(engine):
 
 @pytest.fixture(scope="module")
 def lattice32(hip):
-    """The 32^3 lattice after 14 FP64 SP2 iterations on the window engine
-    (M = 3200 holds the fill), with its history."""
+    """The 32^3 lattice after 24 FP64 SP2 iterations on the window engine
+    (M = 3200 holds the fill of ~2440 entries per row), with its history."""
     hip.set_row_engine(hip.ROW_ENGINE_UNORDERED)
     try:
         H = G.lattice_hamiltonian(32, 32, 32, M=16, hopping=-0.5, disorder=0.05)
         dH = hip.from_ell(H, 3200)
         X = hip.zero_matrix(H.precision, H.N, 3200)
-        r, h = hip.sp2(dH, X, H.N / 2, threshold=1e-5, tol=1e-8, maxit=14)
+        r, h = hip.sp2(dH, X, H.N / 2, threshold=1e-5, tol=1e-8, maxit=24)
         return H, dH, X, r, h
     finally:
         hip.set_row_engine(hip.ROW_ENGINE_EXACT)
@@ -72,7 +72,7 @@ def lattice32(hip):
 
 def test_lattice32_window_sp2_properties(lattice32):
     H, dH, X, r, h = lattice32
-    assert r.iterations == 14 and not r.converged
+    assert r.iterations == 24 and not r.converged
     A = X.get()
     n = A.nnz
     assert n.max() <= 3200 and n.min() > 1000  # the fill this workload reaches
@@ -80,10 +80,11 @@ def test_lattice32_window_sp2_properties(lattice32):
     assert (S != S.T).nnz == 0, "iterate not exactly symmetric"
     m = A.live_mask()
     assert np.all(np.abs(A.value[m]) > 1e-5), "an entry at or below the threshold was kept"
-    # the electron count: tr X -> N/2 (measured: within 2e-3 from iteration 12)
+    # the electron count: tr X -> N/2 (measured: within 1e-3 from iteration 20)
     assert abs(h[-1, 0] - H.N / 2) < 1e-2 and abs(h[-1, 1] - H.N / 2) < 1e-2
-    # idempotency falls to the threshold floor (~1e-2 .. 1e-3 here)
-    assert h[-1, 2] < 1.0 and h[-1, 2] < h[5, 2]
+    # idempotency falls to the threshold floor (~1e-2 here: 1e-5 drops entries
+    # of ~2440-entry rows)
+    assert h[-1, 2] < 0.05 and h[-1, 2] < h[5, 2] / 100
 
 
 def test_lattice32_x2_hermitian_identities(lattice32, engine):
@@ -185,14 +186,21 @@ def test_batch_systems_match_oracle(engine, batch_draw, row_engine, mixed):
     hip.set_row_engine(hip.ROW_ENGINE_EXACT if row_engine == "exact"
                        else hip.ROW_ENGINE_UNORDERED)
     nocc = [float(H.N // 2) for H in Hs]
-    Xs = [hip.zero_matrix(H.precision, H.N, H.N) for H in Hs]
-    res = hip.sp2_batch([hip.from_ell(H, H.N) for H in Hs], Xs, nocc, threshold=1e-5,
+    Mx = max(H.N for H in Hs)  # one M for every X of a batch; whole rows fit
+    Xs = [hip.zero_matrix(H.precision, H.N, Mx) for H in Hs]
+    res = hip.sp2_batch([hip.from_ell(H, Mx) for H in Hs], Xs, nocc, threshold=1e-5,
                         tol=1e-8, maxit=60, mixed=mixed)
     for H, X, r, (ref, rit, rh) in zip(Hs, Xs, res, refs):
         D, R = X.get().to_dense(), ref.to_dense()
         scale = np.abs(R).max()
-        if mixed:
+        if mixed and rit < 60:
             assert np.abs(D - R).max() <= 2e-5 * scale, f"N={H.N}"
+        elif mixed:
+            # no gap at the Fermi level (the odd lattices): the occupied
+            # subspace is not fixed by H, so single-precision iterations
+            # settle elsewhere in it (measured 2e-4 .. 3e-2 from the FP64
+            # density matrix); the electron count still holds
+            assert abs(r.trace_x - H.N // 2) < 0.5, f"N={H.N}"
         elif row_engine == "exact":
             assert r.iterations == rit, f"N={H.N}"
             assert np.array_equal(D, R), f"N={H.N}: not bit-identical"
@@ -243,7 +251,11 @@ def test_peierls64_x2_hermitian_identities(engine):
     hip.sp2(dH, X, H.N / 2, threshold=1e-6, tol=1e-8, maxit=3)
     A = X.get()
     SA = csr(A)
-    assert abs(SA - SA.conj().T).max() == 0.0
+    # Hermitian to rounding: the first iterations' rows are short and run
+    # the full product on the exact kernel (the mirrored upper triangle
+    # needs 64-lane rows), whose (i, k) and (k, i) sums differ in the last
+    # bit; later diagonals inherit imaginary parts near 1e-17
+    assert abs(SA - SA.conj().T).max() <= 1e-15 * abs(SA).max()
     out = {}
     for eng in (hip.ROW_ENGINE_EXACT, hip.ROW_ENGINE_UNORDERED):
         hip.set_row_engine(eng)
