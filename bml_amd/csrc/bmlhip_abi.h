@@ -49,6 +49,12 @@ typedef struct bmlhip_status {
 int bmlhip_init(int device);
 void bmlhip_shutdown(void);
 const char *bmlhip_errmsg(void);
+/* The fixed workspace of the entry points that allocate nothing: a
+ * file-scope device array of BMLHIP_NOALLOC_BYTES, one per GPU, held until
+ * the process ends; its address on the current device (taken once at
+ * bmlhip_init), NULL before.  Calls that overlap on one GPU share it. */
+#define BMLHIP_NOALLOC_BYTES ((size_t) 64 << 20)
+void *bmlhip_noalloc_workspace(size_t *bytes);
 void bmlhip_set_errmsg(const char *msg);
 int bmlhip_malloc(void **d_ptr, size_t bytes);
 void bmlhip_free(void *d_ptr);
@@ -116,6 +122,10 @@ typedef struct bmlhip_spgemm_args {
     int wbits;   /* BMLHIP_ROW_WINDOW: log2 of the column window (11..13) */
     const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
     const bmlhip_batch *batch; /* not NULL: per-system traces into batch->d_trx/d_trx2 */
+    void *d_ws;       /* not NULL: the product takes its scratch from these
+                         ws_bytes of device memory and allocates nothing
+                         (exact and unordered engines, no batch, no sym) */
+    size_t ws_bytes;
 } bmlhip_spgemm_args;
 #define BMLHIP_ROW_EXACT 0
 #define BMLHIP_ROW_UNORDERED 1

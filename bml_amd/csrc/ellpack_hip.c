@@ -403,9 +403,10 @@ void bml_ellpack_hip_row_stats(const bml_matrix_ellpack_hip_t *A, int *max_nnz,
 
 /* C = A*B into C's storage.  kind 0 = x2 (traces, kept-count check), 1 = AB
  * (touched-count check), 2 = AB with the kept-count check (adjust). */
-static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
-                       bml_matrix_ellpack_hip_t *C, double threshold, int kind,
-                       struct bml_hip_aux *hint_owner, const char *what)
+static int run_product_ws(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
+                          bml_matrix_ellpack_hip_t *C, double threshold, int kind,
+                          struct bml_hip_aux *hint_owner, void *ws, size_t ws_bytes,
+                          const char *what)
 {
     bmlhip_spgemm_args a;
     memset(&a, 0, sizeof a);
@@ -426,9 +427,10 @@ static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
     a.d_Cn = C->nnz;
     a.threshold = threshold;
     bmlh_spgemm_shape sh;
+    /* on a workspace the window engine (squares only) is not offered */
     bmlh_pick_spgemm(a.prec, A->N, max_nnz_of(A), max_nnz_of(B),
                      hint_owner ? hint_owner->touched_hint : 0,
-                     hint_owner ? hint_owner->hint_max_nnz : 0, A == B, &sh);
+                     hint_owner ? hint_owner->hint_max_nnz : 0, A == B && !ws, &sh);
     a.group = sh.group;
     a.table = sh.table;
     a.bucket = sh.bucket;
@@ -440,8 +442,17 @@ static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
      * (Hermitian) and x2 forms the upper triangle and mirrors it, as the SP2
      * drivers do for a symmetric H (a test hook for that path) */
     const char *e = getenv("BML_HIP_ASSUME_SYMMETRIC");
-    a.sym = kind == 0 && e && atoi(e) != 0;
+    a.sym = kind == 0 && !ws && e && atoi(e) != 0;
+    a.d_ws = ws;
+    a.ws_bytes = ws_bytes;
     return bmlh_ok(bmlhip_spgemm(&a), what);
+}
+
+static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
+                       bml_matrix_ellpack_hip_t *C, double threshold, int kind,
+                       struct bml_hip_aux *hint_owner, const char *what)
+{
+    return run_product_ws(A, B, C, threshold, kind, hint_owner, NULL, 0, what);
 }
 
 void *bml_multiply_x2_ellpack_hip(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *X2,
@@ -470,6 +481,36 @@ void *bml_multiply_x2_ellpack_hip(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpac
     trace[0] = st.scalars[0];
     trace[1] = st.scalars[1];
     return trace;
+}
+
+void bml_multiply_x2_ellpack_hip_noalloc(bml_matrix_ellpack_hip_t *X,
+                                         bml_matrix_ellpack_hip_t *X2, double threshold,
+                                         double *traces)
+{
+    const char *what = "bml_multiply_x2_ellpack_hip_noalloc";
+    if (!valid_ell(X, what) || !valid_ell(X2, what) || !same_shape(X, X2, what))
+        return;
+    if (X == X2 || !traces) {
+        bmlh_error(BML_HIP_ERR_ARGUMENT, "%s: X and X2 must be distinct and traces not NULL",
+                   what);
+        return;
+    }
+    size_t ws_bytes = 0;
+    void *ws = bmlhip_noalloc_workspace(&ws_bytes);
+    if (!ws) {
+        bmlh_error(BML_HIP_ERR_DEVICE, "%s: the library is not initialised on this GPU", what);
+        return;
+    }
+    if (!run_product_ws(X, X, X2, threshold, 0, X->aux, ws, ws_bytes, what))
+        return;
+    bmlhip_status st;
+    if (!bmlh_finish_sync(what, &st))
+        return;
+    bmlh_set_stats(X2->aux, &st);
+    X->aux->touched_hint = st.max_touched;
+    X->aux->hint_max_nnz = max_nnz_of(X);
+    traces[0] = st.scalars[0];
+    traces[1] = st.scalars[1];
 }
 
 void bml_multiply_AB_ellpack_hip(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,

@@ -2268,7 +2268,7 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
 
 template <typename T>
 int spgemm_typed(const bmlhip_spgemm_args *a) {
-    if (a->engine == BMLHIP_ROW_WINDOW && a->kind == 0 && a->d_Av == a->d_Bv &&
+    if (!a->d_ws && a->engine == BMLHIP_ROW_WINDOW && a->kind == 0 && a->d_Av == a->d_Bv &&
         a->d_Ai == a->d_Bi && a->d_An == a->d_Bn && a->MA == a->MB)
         return spgemm_win_typed<T>(a);
     hipStream_t s = (hipStream_t) bmlhip_stream();
@@ -2301,10 +2301,6 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     p.st = bmlhip_status_dev();
     p.row_sys = a->batch ? a->batch->d_row_sys : nullptr;
     p.sys_done = a->batch ? a->batch->d_done : nullptr;
-    p.rowtr = a->batch ? a->batch->d_rowtr
-                       : (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
-    p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
-    p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (a->N + 1));
     p.remap = a->remap;
     p.sym = 0;
     // global-table pass: one table per block, >= 2x the row bound
@@ -2317,7 +2313,39 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     int gblocks = (int) (FALLBACK_BYTES / per_block);
     if (gblocks > FALLBACK_BLOCKS) gblocks = FALLBACK_BLOCKS;
     if (gblocks < 1) gblocks = 1;
-    unsigned char *gtab = (unsigned char *) bmlhip_scratch(2, (size_t) gblocks * per_block + 256);
+    unsigned char *gtab;
+    if (a->d_ws) {
+        // the caller's workspace: the per-row arrays first (256-byte
+        // aligned), the global-pass tables in what is left; nothing is
+        // allocated and nothing in it needs to start at zero (each kernel
+        // writes what a later one reads, the global tables are cleared by
+        // their own kernel)
+        if (a->batch || a->sym || a->engine == BMLHIP_ROW_WINDOW)
+            return fail_msg(2, "row product on a workspace: no batch, symmetric or window path");
+        unsigned char *w = (unsigned char *) a->d_ws;
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            void *q = w + off;
+            off += (bytes + 255) & ~(size_t) 255;
+            return q;
+        };
+        p.rowtr = (double *) take(sizeof(double) * 2 * (size_t) (a->N + 1));
+        p.ovf = (int *) take(sizeof(int) * (size_t) (a->N + 64));
+        p.rowstat = (int4 *) take(sizeof(int4) * (size_t) (a->N + 1));
+        if (off + per_block > a->ws_bytes)
+            return fail_msg(5, "row product: N=%d (%zu bytes of row records) and one %zu-byte "
+                               "global table exceed the %zu-byte workspace", a->N, off,
+                            per_block, a->ws_bytes);
+        const size_t fit = (a->ws_bytes - off) / per_block;
+        if ((size_t) gblocks > fit) gblocks = (int) fit;
+        gtab = w + off;
+    } else {
+        p.rowtr = a->batch ? a->batch->d_rowtr
+                           : (double *) bmlhip_scratch(0, sizeof(double) * 2 * (size_t) (a->N + 1));
+        p.ovf = (int *) bmlhip_scratch(1, sizeof(int) * (size_t) (a->N + 64));
+        p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (a->N + 1));
+        gtab = (unsigned char *) bmlhip_scratch(2, (size_t) gblocks * per_block + 256);
+    }
     int *cursor = &p.st->cursor;
     if (!p.rowtr || !p.ovf || !p.rowstat || !gtab) {
         return fail_msg(4, "scratch allocation failed");

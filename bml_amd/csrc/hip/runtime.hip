@@ -21,6 +21,7 @@ struct Runtime {
     void *scratch[BMLHIP_SCRATCH_SLOTS] = {};
     size_t scratch_bytes[BMLHIP_SCRATCH_SLOTS] = {};
     char msg[512] = {0};
+    void *noalloc_ws[64] = {};  // address of `scratch` on each device
 };
 
 Runtime g;
@@ -31,6 +32,14 @@ int fail(hipError_t e, const char *what) {
 }
 
 }  // namespace
+
+// Workspace of the entry points that allocate nothing (for callers that
+// cannot call hipMalloc: graph capture, tight serving loops).  Each GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// the name is only a stand-in: the device address comes from
+// hipGetSymbolAddress at bmlhip_init (noalloc_ws), and only that is passed
+// to kernels.
+__device__ __align__(256) unsigned char scratch[BMLHIP_NOALLOC_BYTES];
 
 // Clears the per-operation fields of the status block on the library stream
 // (a kernel rather than a memset node, so a captured graph replays it
@@ -72,6 +81,11 @@ int bmlhip_init(int device) {
         return 3;
     }
     CHECK(hipSetDevice(device));
+    if (device < 64 && !g.noalloc_ws[device]) {
+        void *ws = nullptr;
+        CHECK(hipGetSymbolAddress(&ws, HIP_SYMBOL(scratch)));
+        g.noalloc_ws[device] = ws;
+    }
     CHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     CHECK(hipMalloc((void **) &g.status, sizeof(bmlhip_status)));
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(64), 0, g.stream, g.status, 1);
@@ -97,6 +111,11 @@ void bmlhip_shutdown(void) {
 }
 
 void *bmlhip_stream(void) { return (void *) g.stream; }
+
+void *bmlhip_noalloc_workspace(size_t *bytes) {
+    if (bytes) *bytes = BMLHIP_NOALLOC_BYTES;
+    return g.device >= 0 && g.device < 64 ? g.noalloc_ws[g.device] : nullptr;
+}
 
 int bmlhip_malloc(void **p, size_t bytes) {
     *p = nullptr;

@@ -29,6 +29,7 @@ ERRORS_EXIT, ERRORS_RECORD = 0, 1
 SYNC_EACH_CALL, SYNC_DEFERRED = 0, 1
 ERR_M_OVERFLOW = 1
 ERR_ARGUMENT = 2
+ERR_UNSUPPORTED = 5
 
 
 class BmlError(RuntimeError):
@@ -98,6 +99,7 @@ _SIGS = {
     "bml_import_from_dense_ellpack_hip": (_P, [_I, _I, _I, _P, _D, _I, _I]),
     "bml_export_to_dense_ellpack_hip": (_P, [_P, _I]),
     "bml_convert_ellpack_hip": (_P, [_P, _I, _I, _I]),
+    "bml_multiply_x2_ellpack_hip_noalloc": (None, [_P, _P, _D, _P]),
     "bml_write_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
     "bml_read_bml_matrix_ellpack_hip": (None, [_P, C.c_char_p]),
     "bml_write_bml_matrix_ellblock_hip": (None, [_P, C.c_char_p]),
@@ -379,6 +381,14 @@ def _take2(p):
 def multiply_x2(X: EllpackHip, X2: EllpackHip, threshold: float):
     """X2 <- X*X; returns (tr X, tr X^2)."""
     return _take2(_call("bml_multiply_x2_ellpack_hip", X.ptr, X2.ptr, threshold))
+
+
+def multiply_x2_noalloc(X: EllpackHip, X2: EllpackHip, threshold: float):
+    """bml_multiply_x2_ellpack_hip_noalloc: the product on the library's
+    fixed device workspace; returns (tr X, tr X^2)."""
+    tr = np.zeros(2)
+    _call("bml_multiply_x2_ellpack_hip_noalloc", X.ptr, X2.ptr, threshold, tr.ctypes.data)
+    return float(tr[0]), float(tr[1])
 
 
 def multiply(A: EllpackHip, B: EllpackHip, Cm: EllpackHip, alpha: float = 1.0,

@@ -215,6 +215,20 @@ void bml_read_bml_matrix_ellpack_hip(bml_matrix_ellpack_hip_t *A, const char *fi
  * returns malloc'd double[2] = {tr X, tr X^2}. */
 void *bml_multiply_x2_ellpack_hip(bml_matrix_ellpack_hip_t *X,
                                   bml_matrix_ellpack_hip_t *X2, double threshold);
+/* No counterpart in bml: bml_multiply_x2_ellpack_hip for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same product on the exact or unordered row
+ * engine (never the window engine, never the symmetric shortcut), its
+ * scratch taken from a fixed 64 MiB device array that the library holds on
+ * each GPU until the process ends, tr X and tr X^2 written to traces[0..1]
+ * (no host allocation either).  Inputs whose per-row records (36 bytes per
+ * row) and one global table of the second pass (2 * min(M*M, N) + 64 slots
+ * of value + 8 bytes) do not fit in it are refused with
+ * BML_HIP_ERR_UNSUPPORTED.  Calls on one GPU share the array: calls that
+ * overlap (several host threads or processes on one GPU) race. */
+void bml_multiply_x2_ellpack_hip_noalloc(bml_matrix_ellpack_hip_t *X,
+                                         bml_matrix_ellpack_hip_t *X2, double threshold,
+                                         double *traces);
 /* bml_multiply_ellpack (bml_multiply_ellpack_typed.c:55-118):
  * C <- alpha*A*B + beta*C. */
 void bml_multiply_ellpack_hip(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
