@@ -1188,19 +1188,18 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_win(WinParams p) {
     }
     __syncthreads();
     // the windows row i touches: the union of its rows' masks
-    // (and the row's partial products, sum of nnz(j): the full product's
-    // count, also when SYM forms half of them)
     unsigned long long m = 0ull;
-    long long prods = 0;  // this thread's share, then the wavefront's
+    // the entries of B streamed by this wavefront (wave-uniform): the
+    // segments of the windows taken, so with SYM about half of the full
+    // product's partial products
+    long long prods = 0;
     for (int e = tid; e < nA; e += NT) {
         T a;
         int j;
         pk_get<T>(P, rowP + e, a, j);
         m |= p.wmask[j];
-        prods += p.boff[(size_t) j * nw1 + p.nwin];
         if (!FUSE && j == i) trx_s = a;  // one entry per row holds the diagonal
     }
-    for (int off = 32; off > 0; off >>= 1) prods += __shfl_down(prods, off, 64);
     m = or64_wave(m);
     if (lane == 0 && m) atomicOr(&mask_s, m);
     __syncthreads();
@@ -1253,6 +1252,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_win(WinParams p) {
                 const int l0 = __builtin_amdgcn_readlane(lo, q);
                 const int l1 = __builtin_amdgcn_readlane(hi, q);
                 const size_t rb = (size_t) j * p.MP;
+                prods += l1 - l0;
                 int k0 = l0 + lane;
                 // two entries per lane per round, branch-free: both loads
                 // issue before the adds, and (SYM) the products of columns
@@ -2145,10 +2145,11 @@ int spgemm_win_typed(const bmlhip_spgemm_args *a) {
     memset(&wp, 0, sizeof wp);
     const int *row_sys = a->batch ? a->batch->d_row_sys : nullptr;
     const int *sys_done = a->batch ? a->batch->d_done : nullptr;
-    if (a->N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
     if ((rc = win_pack<T>(a->N, a->MB, a->d_Bv, a->d_Bi, a->d_Bn, a->wbits, nullptr, a->d_skip,
                           row_sys, sys_done, wp, s)))
         return rc;
+    // the profiling window spans the product and the mirror
+    if (a->N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
     wp.MC = a->MC;
     wp.Cv = a->d_Cv;
     wp.Ci = a->d_Ci;
@@ -2225,7 +2226,6 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
     if (!rowtr || !rownorm || !rowstat || !fill) return fail_msg(4, "scratch allocation failed");
     WinParams wp;
     memset(&wp, 0, sizeof wp);
-    if (N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
     if ((rc = win_pack<T>(N, M, Xv, Xi, Xn, wbits, rowtr, skip, nullptr, nullptr, wp, s))) return rc;
     // the branch needs tr X and tr X^2 before the product
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rowtr, N, &st->scalars[0]);
@@ -2241,6 +2241,8 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
     wp.ab = d_state->ab;
     wp.rowstat = rowstat;
     wp.skip = skip;
+    // the profiling window spans the product and the mirror
+    if (N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
     if ((rc = launch_win_shape<T, true, true>(wp, wbits, waves, s))) return rc;
     SpParams p;
     memset(&p, 0, sizeof p);

@@ -25,6 +25,7 @@ void bml_hip_sp2_defaults(bml_hip_sp2_options *o)
     o->poll = 1;
     o->graph = 1;
     o->mixed_switch = 1;
+    o->resume = 0;
 }
 
 static int single_of(int prec)
@@ -53,8 +54,10 @@ static int stalled(const bml_hip_sp2_options *o, stall_track *t, double norm)
     return stall;
 }
 
-/* one SP2 iteration on (X, X2), all queued on the library stream */
-static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *X2,
+/* one SP2 iteration on (X, *X2p), all queued on the library stream; *X2p
+ * (the product's storage, of X's shape) is allocated on first use: the
+ * fused window step updates X in place and needs none */
+static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t **X2p,
                          bmlhip_sp2_state *d_state, double *d_hist,
                          const bml_hip_sp2_options *o, double nocc, int sym, const char *what)
 {
@@ -64,13 +67,9 @@ static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *
     a.kind = 0;
     a.N = X->N;
     a.MA = a.MB = X->M;
-    a.MC = X2->M;
     a.d_Av = a.d_Bv = X->value;
     a.d_Ai = a.d_Bi = X->index;
     a.d_An = a.d_Bn = X->nnz;
-    a.d_Cv = X2->value;
-    a.d_Ci = X2->index;
-    a.d_Cn = X2->nnz;
     a.threshold = o->threshold;
     a.d_skip = &d_state->done;
     int mx = X->aux->stats_valid ? X->aux->max_nnz : X->M;
@@ -91,6 +90,14 @@ static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *
                                               X->index, X->nnz, o->threshold, sh.wbits,
                                               sh.waves, d_state, nocc, o->tol, d_hist, o->maxit),
                        what);
+    if (!*X2p && !(*X2p = bml_noinit_matrix_ellpack_hip(X->matrix_precision, X->N, X->M,
+                                                        X->distribution_mode)))
+        return 0;
+    bml_matrix_ellpack_hip_t *X2 = *X2p;
+    a.MC = X2->M;
+    a.d_Cv = X2->value;
+    a.d_Ci = X2->index;
+    a.d_Cn = X2->nnz;
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
            bmlh_ok(bmlhip_add_dev(X->matrix_precision, X->N, X->M, X->value, X->index, X->nnz,
@@ -130,13 +137,16 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     bml_hip_sp2_result r;
     memset(&r, 0, sizeof r);
 
-    /* X0 = (emax I - H) / (emax - emin) */
-    double *g = bml_gershgorin_ellpack_hip(H);
-    if (!g)
-        return;
-    r.emin = g[0];
-    r.emax = g[1];
-    free(g);
+    /* X0 = (emax I - H) / (emax - emin), unless X holds the iterate to
+     * continue from */
+    if (!o.resume) {
+        double *g = bml_gershgorin_ellpack_hip(H);
+        if (!g)
+            return;
+        r.emin = g[0];
+        r.emax = g[1];
+        free(g);
+    }
     /* A symmetric (Hermitian) H gives symmetric iterates: on the unordered
      * engine, X^2 then forms its upper triangle and mirrors it */
     int sym = 0;
@@ -144,19 +154,18 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
         !bmlh_ok(bmlhip_is_symmetric(H->matrix_precision, H->N, H->M, H->value, H->index,
                                      H->nnz, &sym), what))
         return;
-    bml_copy_ellpack_hip(H, X);
-    if (bmlh_failed())
-        return;
-    bml_normalize_ellpack_hip(X, r.emin, r.emax);
-    if (bmlh_failed())
-        return;
+    if (!o.resume) {
+        bml_copy_ellpack_hip(H, X);
+        if (bmlh_failed())
+            return;
+        bml_normalize_ellpack_hip(X, r.emin, r.emax);
+        if (bmlh_failed())
+            return;
+    }
 
     bml_matrix_ellpack_hip_t *X2 = NULL, *Xs = NULL, *X2s = NULL;
     bmlhip_sp2_state *d_state = NULL;
     double *d_hist = NULL;
-    X2 = bml_noinit_matrix_ellpack_hip(prec, N, M, X->distribution_mode);
-    if (!X2)
-        goto out;
     if (!bmlh_ok(bmlhip_malloc((void **) &d_state, sizeof(bmlhip_sp2_state)), what) ||
         !bmlh_ok(bmlhip_malloc((void **) &d_hist, sizeof(double) * 4 * (size_t) o.maxit), what))
         goto out;
@@ -166,8 +175,7 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     int single = 0;
     if (o.mixed && o.maxit > o.refine_iters) {
         Xs = bml_noinit_matrix_ellpack_hip(sprec, N, M, X->distribution_mode);
-        X2s = bml_noinit_matrix_ellpack_hip(sprec, N, M, X->distribution_mode);
-        if (!Xs || !X2s)
+        if (!Xs)
             goto out;
         if (!bmlh_ok(bmlhip_convert(prec, sprec, (size_t) N * M, X->value, Xs->value), what) ||
             !bmlh_ok(bmlhip_d2d(Xs->index, X->index, (size_t) N * M * sizeof(int)), what) ||
@@ -181,8 +189,8 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     memset(&st, 0, sizeof st);
     stall_track stall = { 0, 0.0, 0.0 };
     for (int it = 0; it < o.maxit; it++) {
-        bml_matrix_ellpack_hip_t *cur = single ? Xs : X, *cur2 = single ? X2s : X2;
-        if (!sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, sym, what))
+        bml_matrix_ellpack_hip_t *cur = single ? Xs : X;
+        if (!sp2_iteration(cur, single ? &X2s : &X2, d_state, d_hist, &o, nocc, sym, what))
             goto out;
         int last_single = single && it + 1 >= o.maxit - o.refine_iters;
         if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {

@@ -56,7 +56,7 @@ class _Blk(C.Structure):
 class Sp2Options(C.Structure):
     _fields_ = [("threshold", C.c_double), ("tol", C.c_double), ("maxit", C.c_int),
                 ("mixed", C.c_int), ("refine_iters", C.c_int), ("poll", C.c_int),
-                ("graph", C.c_int), ("mixed_switch", C.c_int)]
+                ("graph", C.c_int), ("mixed_switch", C.c_int), ("resume", C.c_int)]
 
 
 class Sp2Result(C.Structure):
@@ -448,13 +448,14 @@ def normalize(A: EllpackHip, mineval: float, maxeval: float) -> None:
 
 def sp2(H: EllpackHip, X: EllpackHip, nocc: float, threshold: float = 1e-5,
         tol: float = 1e-8, maxit: int = 60, mixed: bool = False, refine_iters: int = 4,
-        poll: int = 1, mixed_switch: bool = True):
-    """X <- SP2 density matrix of H; returns (Sp2Result, history[it, 4])."""
+        poll: int = 1, mixed_switch: bool = True, resume: bool = False):
+    """X <- SP2 density matrix of H; returns (Sp2Result, history[it, 4]).
+    resume: continue from the iterate X holds."""
     o = Sp2Options()
     lib().bml_hip_sp2_defaults(C.byref(o))
     o.threshold, o.tol, o.maxit = threshold, tol, maxit
     o.mixed, o.refine_iters, o.poll = int(mixed), refine_iters, poll
-    o.mixed_switch = int(mixed_switch)
+    o.mixed_switch, o.resume = int(mixed_switch), int(resume)
     r = Sp2Result()
     hist = np.zeros((maxit, 4))
     _call("bml_hip_sp2_ellpack", H.ptr, X.ptr, nocc, C.byref(o), C.byref(r), hist.ctypes.data)

@@ -413,6 +413,9 @@ typedef struct bml_hip_sp2_options {
     int mixed_switch;  /* mixed: also move to double as soon as the single-
                           precision iterate stops improving (1, needs poll 1),
                           or only for the last refine_iters iterations (0) */
+    int resume;        /* single solves: 1 continues the iteration from the
+                          iterate X holds (no Gershgorin bounds, no
+                          normalization; H is only tested for symmetry) */
 } bml_hip_sp2_options;
 
 typedef struct bml_hip_sp2_result {

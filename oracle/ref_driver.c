@@ -503,6 +503,32 @@ int main(int argc, char **argv)
         }
         qsort(t, reps, sizeof(double), cmp_d);
         printf("median_ms %.6f\n", t[reps / 2]);
+    } else if (!strcmp(op, "bench_x2_rows")) {
+        /* bench_x2_rows <in.ell> <M2> <thr> <rows> <reps>: bml_multiply_x2 on
+         * the row block [0, rows) (the matrix's domain, which the reference's
+         * loop runs over: bml_multiply_ellpack_typed.c:155-157); prints the
+         * median time and the block's partial products */
+        NEED(7);
+        bml_matrix_ellpack_t *X = read_ell(argv[2], 0);
+        bml_matrix_ellpack_t *X2 = bml_zero_matrix(ellpack, X->matrix_precision,
+                                                   X->N, atoi(argv[3]), sequential);
+        int rows = atoi(argv[5]), reps = atoi(argv[6]);
+        if (rows < 1 || rows > X->N)
+            die("rows out of range");
+        X->domain->localRowMin[0] = 0;
+        X->domain->localRowMax[0] = rows;
+        long long prods = 0;
+        for (int i = 0; i < rows; i++)
+            for (int jp = 0; jp < X->nnz[i]; jp++)
+                prods += X->nnz[X->index[(size_t) i * X->M + jp]];
+        double *t = malloc(sizeof(double) * (reps + 1));
+        for (int r = 0; r < reps; r++) {
+            double t0 = now_ms();
+            bml_free_memory(bml_multiply_x2(X, X2, num(argv[4])));
+            t[r] = now_ms() - t0;
+        }
+        qsort(t, reps, sizeof(double), cmp_d);
+        printf("median_ms %.6f products %lld\n", t[reps / 2], prods);
     } else if (!strcmp(op, "bench_sp2")) {
         NEED(9);
         bml_matrix_ellpack_t *H = read_ell(argv[2], 0);

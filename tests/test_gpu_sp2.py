@@ -32,8 +32,11 @@ def test_sp2_matches_reference_driven_golden(hip):
     np.testing.assert_allclose(hist[:, :3], ref_hist[:, :3], rtol=1e-10, atol=1e-13)
 
 
-@pytest.mark.parametrize("L,hop", [(8, -0.1), (10, -0.25)])
+@pytest.mark.parametrize("L,hop", [(8, -0.1), (10, -0.25), (10, -0.5)])
 def test_sp2_matches_oracle(hip, L, hop):
+    """Exact engine against the oracle, bit for bit; (10, -0.5) is the 32^3
+    SP2 workload's Hamiltonian (onsite +-1, hopping -0.5 +- 0.05) on a 10^3
+    lattice, whose density matrix fills whole rows."""
     H = G.lattice_hamiltonian(L, L, L, M=16, hopping=hop, disorder=0.1 * abs(hop))
     M = H.N
     ref, it, rhist = orc.sp2(H, M, H.N / 2, 1e-5, 1e-8, 25)
@@ -132,3 +135,22 @@ def test_batch_mixed_precision(hip):
     for a, b, r in zip(Xd, Xm, rm):
         assert np.abs(a.get().to_dense() - b.get().to_dense()).max() < 1e-4
         assert r.idempotency < 1e-3
+
+
+def test_sp2_unordered_matches_oracle_workload_hamiltonian(hip):
+    """The unordered row engine (the library's fast setting) on the 32^3 SP2
+    workload's Hamiltonian at L = 10 (onsite +-1, hopping -0.5 +- 0.05,
+    threshold 1e-5): the same branches as the oracle's SP2 and X within
+    1e-12 x max|X| (sums in arrival order)."""
+    from helpers import assert_ell_close
+    H = G.lattice_hamiltonian(10, 10, 10, M=16, hopping=-0.5, disorder=0.05)
+    ref, it, rhist = orc.sp2(H, H.N, H.N / 2, 1e-5, 1e-8, 25)
+    hip.set_row_engine(hip.ROW_ENGINE_UNORDERED)
+    try:
+        X, r, hist = run_sp2(hip, H, H.N, H.N / 2, threshold=1e-5, tol=1e-8, maxit=25)
+    finally:
+        hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+    assert r.iterations == it
+    np.testing.assert_array_equal(hist[:, 3], rhist[:, 3])
+    np.testing.assert_allclose(hist[:, :2], rhist[:, :2], rtol=1e-12)
+    assert_ell_close(X, ref, 1e-12, 1e-5, "unordered SP2 vs oracle")
