@@ -163,13 +163,19 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
      * engine's rows fall to the global-table pass).  Windows of 2^11
      * columns (10% faster than 2^12 on the 32^3 lattice at hopping -0.5:
      * smaller windows keep more blocks per CU), eight wavefronts per row;
-     * at most 64 windows. */
+     * at most 64 windows.  Blocks of 4 consecutive output rows share each
+     * streamed row of X (the launcher takes fewer when 4 rows' columns
+     * would not fit its LDS union table), with windows of 2^10 columns. */
     out->wbits = 0;
+    out->wrows = 1;
     if (square && (sel == BML_HIP_ROW_ENGINE_WINDOW ||
                    (sel == BML_HIP_ROW_ENGINE_UNORDERED && engine == BMLHIP_ROW_UNORDERED))) {
-        int wb = (e = getenv("BML_HIP_WINDOW_BITS")) ? atoi(e) : 11;
-        if (wb < 11)
-            wb = 11;
+        int wr = (e = getenv("BML_HIP_WIN_ROWS")) ? atoi(e) : 4;
+        if (wr != 1 && wr != 2)
+            wr = 4;
+        int wb = (e = getenv("BML_HIP_WINDOW_BITS")) ? atoi(e) : (wr > 1 ? 10 : 11);
+        if (wb < 10)
+            wb = 10;
         if (wb > 13)
             wb = 13;
         /* smallest window with at most 64 windows over N columns */
@@ -180,6 +186,7 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
         if (fits && (sel == BML_HIP_ROW_ENGINE_WINDOW || est > wmin)) {
             engine = BMLHIP_ROW_WINDOW;
             out->wbits = wb;
+            out->wrows = wr;
             waves = 8;
         }
     }

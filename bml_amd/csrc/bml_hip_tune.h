@@ -12,6 +12,7 @@ typedef struct bmlh_spgemm_shape {
     int waves;  /* wavefronts per row (long rows; unordered engine) */
     int engine; /* BMLHIP_ROW_EXACT, _UNORDERED or _WINDOW */
     int wbits;  /* _WINDOW: log2 of the column window */
+    int wrows;  /* _WINDOW: output rows per block */
 } bmlh_spgemm_shape;
 
 /* Launch shape for a product whose left rows hold up to max_a entries and

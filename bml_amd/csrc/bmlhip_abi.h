@@ -32,6 +32,8 @@ typedef struct bmlhip_status {
     long long prof_out_nnz[5];  /* entries they wrote */
     long long prof_ovf_rows[5]; /* rows finished by the global-table pass */
     long long prof_touched[5];  /* distinct columns before thresholding */
+    long long prof_streamed[5]; /* entries of B's rows read (the products, unless
+                                   several output rows share each read) */
 } bmlhip_status;
 
 /* Per-row records: kernels store (kept, distinct, products, nnz of A's row)
@@ -44,6 +46,7 @@ typedef struct bmlhip_status {
 #define BMLHIP_RS_OVF_DIST 2 /* rows with distinct > MC overflow */
 #define BMLHIP_RS_STORED 4   /* max_nnz, total_nnz from min(kept, MC) */
 #define BMLHIP_RS_PROF 8     /* profiling sums of precision `prof` */
+#define BMLHIP_RS_OWN_STREAM 16 /* the kernel added its own streamed count */
 
 /* ---- runtime (hip/runtime.hip) ----------------------------------------- */
 int bmlhip_init(int device);
@@ -67,7 +70,7 @@ void *bmlhip_stream(void);
 /* Library-owned scratch, grown on demand; slot selects one of a few
  * independent buffers.  The pointer stays valid until the next call with
  * the same slot and a larger size. */
-#define BMLHIP_SCRATCH_SLOTS 16
+#define BMLHIP_SCRATCH_SLOTS 20
 void *bmlhip_scratch(int slot, size_t bytes);
 /* as bmlhip_scratch, zero-filled whenever the buffer is (re)allocated */
 void *bmlhip_scratch_zeroed(int slot, size_t bytes);
@@ -119,7 +122,8 @@ typedef struct bmlhip_spgemm_args {
                     wavefronts per row with LDS atomic adds, table order */
     int sym;     /* x2 of a symmetric / Hermitian X on the unordered or window
                     engine: rows form columns k >= i, the rest is mirrored */
-    int wbits;   /* BMLHIP_ROW_WINDOW: log2 of the column window (11..13) */
+    int wbits;   /* BMLHIP_ROW_WINDOW: log2 of the column window (10..13) */
+    int wrows;   /* BMLHIP_ROW_WINDOW: output rows per block (1, 2 or 4) */
     const int *d_skip; /* if not NULL and *d_skip != 0 the kernels do nothing */
     const bmlhip_batch *batch; /* not NULL: per-system traces into batch->d_trx/d_trx2 */
     void *d_ws;       /* not NULL: the product takes its scratch from these
@@ -138,6 +142,10 @@ typedef struct bmlhip_spgemm_args {
 #define BMLHIP_WIN_BOFF_SLOT 13
 #define BMLHIP_WIN_MASK_SLOT 14
 #define BMLHIP_WIN_NORM_SLOT 15
+/* row blocks: union columns, coefficients, union sizes */
+#define BMLHIP_WIN_UJ_SLOT 16
+#define BMLHIP_WIN_UC_SLOT 17
+#define BMLHIP_WIN_UN_SLOT 18
 /* window engine limits: at most 64 windows per row range */
 #define BMLHIP_WIN_MAX_WINDOWS 64
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);
@@ -197,8 +205,9 @@ int bmlhip_sp2_check(bmlhip_sp2_state *d_state, double tol, double *d_hist, int 
  * X^2 thresholded as bml_add_norm does, mirrors it and reduces ||X - X^2||^2
  * into scalars[2]; then bmlhip_sp2_check.  Nothing runs once d_state->done. */
 int bmlhip_sp2_window_step(int prec, int N, int M, void *d_Xv, int *d_Xi, int *d_Xn,
-                           double threshold, int wbits, int waves, bmlhip_sp2_state *d_state,
-                           double nocc, double tol, double *d_hist, int maxit);
+                           double threshold, int wbits, int waves, int wrows,
+                           bmlhip_sp2_state *d_state, double nocc, double tol, double *d_hist,
+                           int maxit);
 /* Capture the kernels queued on the library stream between begin and end
  * into an executable graph (no allocation, memset or host sync may happen
  * in between); launch replays them on the library stream. */

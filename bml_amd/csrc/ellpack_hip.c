@@ -438,6 +438,7 @@ static int run_product_ws(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t 
     a.waves = sh.waves;
     a.engine = sh.engine;
     a.wbits = sh.wbits;
+    a.wrows = sh.wrows;
     /* BML_HIP_ASSUME_SYMMETRIC=1: the caller asserts X is symmetric
      * (Hermitian) and x2 forms the upper triangle and mirrors it, as the SP2
      * drivers do for a symmetric H (a test hook for that path) */

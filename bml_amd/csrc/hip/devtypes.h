@@ -30,6 +30,7 @@ template <> struct Elem<float> {
     __device__ static float mul(float a, float b) { return a * b; }
     // |v| > t exactly as the reference's is_above_threshold: fabsf(x) > (float) t
     __device__ static bool above(float v, double t) { return fabsf(v) > (float) t; }
+    __device__ static bool nonzero(float v) { return v != 0.0f; }
     __device__ static double re(float v) { return (double) v; }
     __device__ static double mag(float v) { return (double) fabsf(v); }
     // |v|^2 as the oracle forms it: the magnitude squared in the element's
@@ -62,6 +63,7 @@ template <> struct Elem<double> {
     __device__ static double sub(double a, double b) { return a - b; }
     __device__ static double mul(double a, double b) { return a * b; }
     __device__ static bool above(double v, double t) { return fabs(v) > t; }
+    __device__ static bool nonzero(double v) { return v != 0.0; }
     __device__ static double re(double v) { return v; }
     __device__ static double mag(double v) { return fabs(v); }
     __device__ static double abs2(double v) { return v * v; }
@@ -85,6 +87,7 @@ template <> struct Elem<cf32> {
         return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
     }
     __device__ static bool above(cf32 v, double t) { return hypotf(v.x, v.y) > (float) t; }
+    __device__ static bool nonzero(cf32 v) { return v.x != 0.0f || v.y != 0.0f; }
     __device__ static double re(cf32 v) { return (double) v.x; }
     __device__ static double mag(cf32 v) { return (double) hypotf(v.x, v.y); }
     __device__ static double abs2(cf32 v) {
@@ -119,6 +122,7 @@ template <> struct Elem<cf64> {
         return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
     }
     __device__ static bool above(cf64 v, double t) { return hypot(v.x, v.y) > t; }
+    __device__ static bool nonzero(cf64 v) { return v.x != 0.0 || v.y != 0.0; }
     __device__ static double re(cf64 v) { return v.x; }
     __device__ static double mag(cf64 v) { return hypot(v.x, v.y); }
     __device__ static double abs2(cf64 v) {

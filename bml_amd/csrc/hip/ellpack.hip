@@ -1076,6 +1076,13 @@ struct WinParams {
     const int *skip;
     const int *row_sys, *sys_done;
     int remap;
+    // row blocks (k_spgemm_winb): per block of R rows the union of their
+    // columns and the R coefficients of each
+    const int *Uj;     // [nblk][ustride] union columns j
+    const void *Uc;    // [nblk][ustride][R] X[i0 + r, j] (T; zero where row r lacks j)
+    const int *Un;     // [nblk] union sizes
+    size_t ustride;
+    unsigned long long *streamed;  // prof_streamed of the precision
 };
 
 // Pack pass: one wavefront per row.  Row j's entries are grouped by window
@@ -1377,6 +1384,400 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_win(WinParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Window engine on row blocks.  On a lattice, neighbouring rows of the
+// density matrix reference mostly the same rows j (on the 32^3 iterate at
+// hopping -0.5, four consecutive rows reference ~1.3x the rows one does), and
+// the one-row kernel streams each row j of X once per output row that
+// references it.  Here a block takes R consecutive output rows i0 .. i0+R-1:
+// k_win_union lists the union U of their columns j with the R coefficients
+// X[i0 + r, j] (zero where row r lacks j), and k_spgemm_winb walks U once
+// per window, streaming each row j's segment once and adding c_r * X[j, k]
+// into the window of every row r with c_r != 0 (a wave-uniform test).  The
+// windows, epilogue, symmetric and fused cases are those of k_spgemm_win,
+// one row at a time.
+
+// Union of the columns of rows i0 .. i0+R-1 (one block per row block): an
+// LDS hash of TS slots (TS > R*M, so a probe always ends) gives each
+// distinct column a slot, a block-wide scan ranks the filled slots, and the
+// coefficients are written at the ranks.  Rows of finished systems, and
+// rows past N, count as empty.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_win_union(int N, int M, const T *Xv, const int *Xi,
+                                                   const int *Xn, int tbits, int *Uj, T *Uc,
+                                                   int *Un, size_t ustride, const int *skip,
+                                                   const int *row_sys, const int *sys_done) {
+    extern __shared__ __align__(16) int ukeys[];  // TS keys, then TS 16-bit ranks
+    __shared__ int part_s[256];
+    if (skip && *skip) return;  // uniform
+    const int TS = 1 << tbits;
+    uint16_t *rank = (uint16_t *) (ukeys + TS);
+    const int b = blockIdx.x, i0 = b * R, tid = threadIdx.x;
+    for (int t = tid; t < TS; t += 256) ukeys[t] = -1;
+    __syncthreads();
+    auto slot_of = [&](int j) -> int {
+        unsigned h = ((unsigned) j * 2654435761u) >> (32 - tbits);
+        while (true) {
+            const int k = ukeys[h];
+            if (k == j) return (int) h;
+            if (k < 0) {
+                const int old = atomicCAS(&ukeys[h], -1, j);
+                if (old < 0 || old == j) return (int) h;
+            }
+            h = (h + 1) & (unsigned) (TS - 1);
+        }
+    };
+    int nr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = i0 + r;
+        nr[r] = (i < N && !(row_sys && sys_done[row_sys[i]])) ? Xn[i] : 0;
+        for (int e = tid; e < nr[r]; e += 256) (void) slot_of(Xi[(size_t) i * M + e]);
+    }
+    __syncthreads();
+    // ranks of the filled slots in slot order: each thread counts a
+    // contiguous chunk, a block scan offsets the chunks
+    const int per = TS / 256;
+    int cnt = 0;
+    for (int t = tid * per; t < (tid + 1) * per; t++) cnt += ukeys[t] >= 0;
+    part_s[tid] = cnt;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const int v = tid >= d ? part_s[tid - d] : 0;
+        __syncthreads();
+        part_s[tid] += v;
+        __syncthreads();
+    }
+    int u = part_s[tid] - cnt;
+    const int total = part_s[255];
+    int *uj = Uj + (size_t) b * ustride;
+    T *uc = Uc + (size_t) b * ustride * R;
+    for (int t = tid * per; t < (tid + 1) * per; t++) {
+        const int k = ukeys[t];
+        if (k >= 0) {
+            rank[t] = (uint16_t) u;
+            uj[u] = k;
+#pragma unroll
+            for (int r = 0; r < R; r++) uc[(size_t) u * R + r] = Elem<T>::zero();
+            u++;
+        }
+    }
+    if (tid == 0) Un[b] = total;
+    __syncthreads();  // the zeroed coefficients land before they are set
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const size_t row = (size_t) (i0 + r) * M;
+        for (int e = tid; e < nr[r]; e += 256) {
+            const int h = slot_of(Xi[row + e]);
+            uc[(size_t) rank[h] * R + r] = Xv[row + e];
+        }
+    }
+}
+
+template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
+__global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
+    using E = Elem<T>;
+    using A = typename UAcc<T>::type;
+    constexpr int WC = 1 << WB;
+    constexpr int NT = 64 * W;
+    constexpr int PER = WC / W;  // epilogue columns per wavefront
+    static_assert(PER % 64 == 0, "window too small for the block");
+    static_assert(WC % R == 0, "a row block must lie in one window");
+    extern __shared__ __align__(16) unsigned char smem[];
+    A *win = (A *) smem;  // R windows of WC accumulators, then one dummy slot per lane
+    T *xwin = (T *) (smem + sizeof(A) * (R * WC + 64));  // FUSE: X's rows in the window
+    __shared__ unsigned long long mask_s;
+    __shared__ int cnt_s[W], dcnt_s[W], dflag_s[R];
+    __shared__ long long prods_s[W][R + 1];
+    __shared__ double nrm_s[W][R];
+    __shared__ T trx_s[R], d2_s[R];
+    if (p.skip && *p.skip) return;  // uniform
+    const int b = blockIdx.x, i0 = b * R;
+    if (i0 >= p.N) return;
+    const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+    const int dummy = R * WC + lane;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const unsigned *P = p.P;
+    const int nw1 = p.nwin + 1;
+    bool live[R];
+    int nA[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = i0 + r;
+        live[r] = i < p.N && !(p.row_sys && p.sys_done[p.row_sys[i]]);
+        nA[r] = live[r] ? p.boff[(size_t) i * nw1 + p.nwin] : 0;
+    }
+    const int nU = p.Un[b];
+    const int *Uj = p.Uj + (size_t) b * p.ustride;
+    const T *Uc = (const T *) p.Uc + (size_t) b * p.ustride * R;
+    if (tid == 0) {
+        mask_s = 0ull;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            trx_s[r] = E::zero();
+            d2_s[r] = E::zero();
+        }
+    }
+    __syncthreads();
+    // the windows the block touches: the union of its rows' rows' masks
+    unsigned long long m = 0ull;
+    for (int e = tid; e < nU; e += NT) {
+        const int j = Uj[e];
+        m |= p.wmask[j];
+        if constexpr (!FUSE) {
+            const int r = j - i0;  // a diagonal coefficient X[i0 + r, i0 + r]
+            if (r >= 0 && r < R) trx_s[r] = Uc[(size_t) e * R + r];
+        }
+    }
+    if constexpr (FUSE) {
+        // the update alpha X + beta X^2 also keeps X's own entries in windows
+        // no product reaches
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (live[r]) m |= p.wmask[i0 + r];
+    }
+    m = or64_wave(m);
+    if (lane == 0 && m) atomicOr(&mask_s, m);
+    __syncthreads();
+    m = mask_s;
+    if constexpr (SYM) m &= ~((1ull << (i0 >> WB)) - 1ull);
+    double alpha = 0.0, beta = 0.0;
+    if constexpr (FUSE) {
+        alpha = p.ab[0];
+        beta = p.ab[1];
+    }
+    const int MC = p.MC;
+    T *Cv = (T *) p.Cv;
+    int kept[R], distinct[R];  // block-uniform running counts
+    long long prods[R];        // this wavefront's products per row (wave-uniform)
+    long long streamed = 0;    // this wavefront's entries of B read (wave-uniform)
+    double nrm[R];             // this lane's norm partials
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        kept[r] = 0;
+        distinct[r] = 0;
+        prods[r] = 0;
+        nrm[r] = 0.0;
+    }
+    while (m) {
+        const int c = __ffsll((long long) m) - 1;
+        m &= m - 1ull;
+        const int cbase = c << WB;
+        for (int s = tid; s < R * WC; s += NT) {
+            win[s] = Elem<A>::zero();
+            if constexpr (FUSE) xwin[s] = E::zero();
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (tid == r) dflag_s[r] = SYM ? (nA[r] > 0) : 0;
+        bool dtouch[R];  // this lane added into column i0 + r (!SYM)
+#pragma unroll
+        for (int r = 0; r < R; r++) dtouch[r] = false;
+        __syncthreads();
+        // products: wavefront w takes groups g = w, w + W, ... of 64 union
+        // entries; lane q of a group holds entry q's column j, coefficients
+        // and the bounds of row j's segment in window c
+        for (int g0 = w * 64; g0 < nU; g0 += NT) {
+            const int e = g0 + lane;
+            int jq = 0, lo = 0, hi = 0;
+            T cq[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) cq[r] = E::zero();
+            if (e < nU) {
+                jq = Uj[e];
+                const int *bo = p.boff + (size_t) jq * nw1;
+                lo = bo[c];
+                hi = bo[c + 1];
+#pragma unroll
+                for (int r = 0; r < R; r++) cq[r] = Uc[(size_t) e * R + r];
+            }
+            uint64_t todo = ballot64(hi > lo);
+            while (todo) {
+                const int q = __ffsll((long long) todo) - 1;
+                todo &= todo - 1ull;
+                const int j = __builtin_amdgcn_readlane(jq, q);
+                const int l0 = __builtin_amdgcn_readlane(lo, q);
+                const int l1 = __builtin_amdgcn_readlane(hi, q);
+                T a[R];
+                unsigned rm = 0;  // rows with a nonzero coefficient (wave-uniform)
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    a[r] = bcast<64>(cq[r], q);
+                    if (E::nonzero(a[r])) rm |= 1u << r;
+                }
+                const size_t rb = (size_t) j * p.MP;
+                streamed += l1 - l0;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if (rm >> r & 1u) prods[r] += l1 - l0;
+                int k0 = l0 + lane;
+                // two entries per lane per round: both loads issue before
+                // the adds; (SYM) products of columns below row i0 + r go to
+                // the lane's dummy slot
+                for (; k0 + 64 < l1; k0 += 128) {
+                    T b0, b1;
+                    int c0, c1;
+                    pk_get<T>(P, rb + k0, b0, c0);
+                    pk_get<T>(P, rb + k0 + 64, b1, c1);
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        if (!(rm >> r & 1u)) continue;  // uniform
+                        const int s0 = (!SYM || c0 >= i0 + r) ? r * WC + c0 - cbase : dummy;
+                        const int s1 = (!SYM || c1 >= i0 + r) ? r * WC + c1 - cbase : dummy;
+                        lds_add(&win[s0], UAcc<T>::to(E::mul(a[r], b0)));
+                        lds_add(&win[s1], UAcc<T>::to(E::mul(a[r], b1)));
+                        if constexpr (!SYM) dtouch[r] = dtouch[r] | (c0 == i0 + r) | (c1 == i0 + r);
+                    }
+                }
+                if (k0 < l1) {
+                    T b0;
+                    int c0;
+                    pk_get<T>(P, rb + k0, b0, c0);
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        if (!(rm >> r & 1u)) continue;  // uniform
+                        const int s0 = (!SYM || c0 >= i0 + r) ? r * WC + c0 - cbase : dummy;
+                        lds_add(&win[s0], UAcc<T>::to(E::mul(a[r], b0)));
+                        if constexpr (!SYM) dtouch[r] = dtouch[r] | (c0 == i0 + r);
+                    }
+                }
+            }
+        }
+        if constexpr (!SYM) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (__any(dtouch[r]) && lane == 0) dflag_s[r] = 1;
+        }
+        if constexpr (FUSE) {
+            // X's own rows in this window
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if (!live[r]) continue;
+                const int i = i0 + r;
+                const int *bo = p.boff + (size_t) i * nw1;
+                for (int e = bo[c] + tid; e < bo[c + 1]; e += NT) {
+                    T v;
+                    int k;
+                    pk_get<T>(P, (size_t) i * p.MP + e, v, k);
+                    xwin[r * WC + k - cbase] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // epilogue, one row at a time: wavefront w walks columns
+        // [w PER, (w+1) PER) of row r's window twice, counting, then
+        // writing at its offset
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (!live[r]) continue;  // uniform
+            const int i = i0 + r;
+            const size_t rowC = (size_t) i * MC;
+            const bool dtouched = dflag_s[r] != 0;
+            auto entry = [&](int s, T &out, bool &keep, bool &nz, double &dn) {
+                const int k = cbase + s;
+                const bool valid = k < p.N && (!SYM || k >= i);
+                const T y = UAcc<T>::from(win[r * WC + s]);
+                nz = valid && (E::mag(y) != 0.0 || k == i);
+                const bool ykeep = valid && (k == i ? dtouched : E::above(y, p.thr));
+                dn = 0.0;
+                if constexpr (FUSE) {
+                    const T x = xwin[r * WC + s];
+                    const T yk = ykeep ? y : E::zero();
+                    if (valid) dn = E::abs2(E::sub(x, yk)) * ((SYM && k > i) ? 2.0 : 1.0);
+                    out = E::axpy(E::axpy(E::zero(), alpha, x), beta, yk);
+                    keep = valid && E::above(out, p.thr);
+                } else {
+                    out = y;
+                    keep = ykeep;
+                }
+            };
+            int cw = 0, dw = 0;
+            for (int s = w * PER + lane; s < (w + 1) * PER; s += 64) {
+                T v;
+                bool keep, nz;
+                double dn;
+                entry(s, v, keep, nz, dn);
+                nrm[r] += dn;
+                cw += popc64(ballot64(keep));
+                dw += popc64(ballot64(nz));
+                if (!FUSE && cbase + s == i) d2_s[r] = v;
+            }
+            if (lane == 0) {
+                cnt_s[w] = cw;
+                dcnt_s[w] = dw;
+            }
+            __syncthreads();
+            int pos0 = kept[r], tot = 0, dtot = 0;
+            for (int q = 0; q < W; q++) {
+                pos0 += q < w ? cnt_s[q] : 0;
+                tot += cnt_s[q];
+                dtot += dcnt_s[q];
+            }
+            for (int s = w * PER + lane; s < (w + 1) * PER; s += 64) {
+                T v;
+                bool keep, nz;
+                double dn;
+                entry(s, v, keep, nz, dn);
+                const uint64_t km = ballot64(keep);
+                const int pos = pos0 + popc64(km & ltmask);
+                if (keep && pos < MC) {
+                    p.Ci[rowC + pos] = cbase + s;
+                    Cv[rowC + pos] = v;
+                }
+                pos0 += popc64(km);
+            }
+            kept[r] += tot;
+            distinct[r] += dtot;
+            __syncthreads();  // cnt_s is reused by the next row
+        }
+    }
+    // row records, traces, norms
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < R; r++) prods_s[w][r] = prods[r];
+        prods_s[w][R] = streamed;
+    }
+    if constexpr (FUSE) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            double x = nrm[r];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+            if (lane == 0) nrm_s[w][r] = x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = i0 + r;
+        if (tid == r && i < p.N) {
+            if (!live[r]) {
+                p.rowstat[i] = make_int4(-1, 0, 0, 0);
+            } else {
+                long long pr = 0;
+                double nr = 0.0;
+                for (int q = 0; q < W; q++) {
+                    pr += prods_s[q][r];
+                    if constexpr (FUSE) nr += nrm_s[q][r];
+                }
+                p.Cn[i] = kept[r] < MC ? kept[r] : MC;
+                p.rowstat[i] = make_int4(kept[r], distinct[r], (int) (pr < INT_MAX ? pr : INT_MAX),
+                                         nA[r]);
+                if constexpr (FUSE) {
+                    p.rownorm[i] = nr;
+                } else {
+                    p.rowtr[i] = E::re(trx_s[r]);
+                    p.rowtr[p.N + i] = E::re(d2_s[r]);
+                }
+            }
+        }
+    }
+    if (tid == 0 && p.streamed) {
+        long long st = 0;
+        for (int q = 0; q < W; q++) st += prods_s[q][R];
+        atomicAdd(p.streamed, (unsigned long long) st);
+    }
+}
+
 // First pass: table in LDS, 64/GROUP rows per 64-thread block.
 template <typename T, int GROUP, int BK>
 __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
@@ -1505,6 +1906,8 @@ __global__ __launch_bounds__(256) void k_reduce_rowstat(const int4 *rs, int N, i
             atomicAdd((unsigned long long *) &st->prof_in_nnz[prof], innz);
             atomicAdd((unsigned long long *) &st->prof_out_nnz[prof], tot);
             atomicAdd((unsigned long long *) &st->prof_touched[prof], dist);
+            if (!(what & BMLHIP_RS_OWN_STREAM))
+                atomicAdd((unsigned long long *) &st->prof_streamed[prof], prods);
         }
     }
 }
@@ -2082,6 +2485,7 @@ template <typename T, bool SYM, bool FUSE>
 int launch_win_shape(const WinParams &wp, int wbits, int waves, hipStream_t s) {
     if (waves == 8) {
         switch (wbits) {
+        case 10: return launch_win<T, 8, 10, SYM, FUSE>(wp, s);
         case 11: return launch_win<T, 8, 11, SYM, FUSE>(wp, s);
         case 12: return launch_win<T, 8, 12, SYM, FUSE>(wp, s);
         case 13: return launch_win<T, 8, 13, SYM, FUSE>(wp, s);
@@ -2089,14 +2493,103 @@ int launch_win_shape(const WinParams &wp, int wbits, int waves, hipStream_t s) {
         }
     } else if (waves == 4) {
         switch (wbits) {
+        case 10: return launch_win<T, 4, 10, SYM, FUSE>(wp, s);
         case 11: return launch_win<T, 4, 11, SYM, FUSE>(wp, s);
         case 12: return launch_win<T, 4, 12, SYM, FUSE>(wp, s);
         case 13: return launch_win<T, 4, 13, SYM, FUSE>(wp, s);
         default: break;
         }
     }
-    return fail_msg(2, "window engine: window 2^%d with %d wavefronts not built (2^11..2^13, 4 or 8)",
+    return fail_msg(2, "window engine: window 2^%d with %d wavefronts not built (2^10..2^13, 4 or 8)",
                     wbits, waves);
+}
+
+template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
+int launch_winb(const WinParams &wp, hipStream_t s) {
+    using A = typename UAcc<T>::type;
+    const size_t WC = (size_t) 1 << WB;
+    const size_t lds = sizeof(A) * (R * WC + 64) + (FUSE ? sizeof(T) * R * WC : 0);
+    if (lds > 160 * 1024 - 2048)
+        return fail_msg(5, "window engine: %d rows of 2^%d columns need %zu bytes of LDS", R, WB,
+                        lds);
+    if (lds > 64 * 1024)
+        (void) hipFuncSetAttribute((const void *) k_spgemm_winb<T, W, WB, R, SYM, FUSE>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+    if (wp.N > 0)
+        hipLaunchKernelGGL((k_spgemm_winb<T, W, WB, R, SYM, FUSE>), dim3((wp.N + R - 1) / R),
+                           dim3(64 * W), lds, s, wp);
+    return report("k_spgemm_winb");
+}
+
+// The product on blocks of `rows` output rows (2 or 4, after win_union), or
+// one row per block (rows 1).
+template <typename T, bool SYM, bool FUSE>
+int launch_win_rows(const WinParams &wp, int rows, int wbits, int waves, hipStream_t s) {
+    if (rows == 1) return launch_win_shape<T, SYM, FUSE>(wp, wbits, waves, s);
+    if (waves == 8 && rows == 4) {
+        switch (wbits) {
+        case 10: return launch_winb<T, 8, 10, 4, SYM, FUSE>(wp, s);
+        case 11: return launch_winb<T, 8, 11, 4, SYM, FUSE>(wp, s);
+        default: break;
+        }
+    } else if (waves == 8 && rows == 2) {
+        switch (wbits) {
+        case 10: return launch_winb<T, 8, 10, 2, SYM, FUSE>(wp, s);
+        case 11: return launch_winb<T, 8, 11, 2, SYM, FUSE>(wp, s);
+        default: break;
+        }
+    }
+    return fail_msg(2, "window engine: blocks of %d rows with windows of 2^%d columns and %d "
+                       "wavefronts not built (2 or 4 rows, 2^10 or 2^11, 8 wavefronts)",
+                    rows, wbits, waves);
+}
+
+// The union lists of the row blocks (k_win_union) into the window scratch.
+// rows (4 or 2) is lowered until rows * M + 1 slots fit the union table's
+// 2^14 (96 KB of LDS); at 1 nothing is built and the one-row kernel runs,
+// as it does for the shapes the row blocks are not built for (windows
+// wider than 2^11 columns, other than 8 wavefronts).
+template <typename T>
+int win_union(int &rows, int N, int M, int wbits, int waves, const void *Xv, const int *Xi,
+              const int *Xn, const int *skip, const int *row_sys, const int *sys_done,
+              WinParams &wp, hipStream_t s) {
+    if (wbits > 11 || waves != 8) rows = 1;
+    while (rows > 1 && (long long) rows * M + 1 > (1ll << 14)) rows >>= 1;
+    if (rows != 2 && rows != 4) {
+        rows = 1;
+        return 0;
+    }
+    int tbits = 8;
+    while ((1ll << tbits) < (long long) rows * M + 1) tbits++;
+    const size_t lds = (size_t) 6 << tbits;  // 32-bit keys and 16-bit ranks
+    const int nblk = (N + rows - 1) / rows;
+    const size_t ustride = (size_t) rows * M < (size_t) N ? (size_t) rows * M : (size_t) N;
+    int *Uj = (int *) bmlhip_scratch(BMLHIP_WIN_UJ_SLOT, sizeof(int) * (size_t) nblk * ustride + 256);
+    T *Uc = (T *) bmlhip_scratch(BMLHIP_WIN_UC_SLOT,
+                                 sizeof(T) * (size_t) nblk * ustride * rows + 256);
+    int *Un = (int *) bmlhip_scratch(BMLHIP_WIN_UN_SLOT, sizeof(int) * ((size_t) nblk + 1));
+    if (!Uj || !Uc || !Un) return fail_msg(4, "scratch allocation failed");
+    wp.Uj = Uj;
+    wp.Uc = Uc;
+    wp.Un = Un;
+    wp.ustride = ustride;
+    if (N == 0) return 0;
+    if (rows == 4) {
+        if (lds > 64 * 1024)
+            (void) hipFuncSetAttribute((const void *) k_win_union<T, 4>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+        hipLaunchKernelGGL((k_win_union<T, 4>), dim3(nblk), dim3(256), lds, s, N, M,
+                           (const T *) Xv, Xi, Xn, tbits, Uj, Uc, Un, ustride, skip, row_sys,
+                           sys_done);
+    } else {
+        if (lds > 64 * 1024)
+            (void) hipFuncSetAttribute((const void *) k_win_union<T, 2>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+        hipLaunchKernelGGL((k_win_union<T, 2>), dim3(nblk), dim3(256), lds, s, N, M,
+                           (const T *) Xv, Xi, Xn, tbits, Uj, Uc, Un, ustride, skip, row_sys,
+                           sys_done);
+    }
+    return report("k_win_union");
 }
 
 // Window count of an N-column product, or -1 when it passes the limit.
@@ -2148,6 +2641,12 @@ int spgemm_win_typed(const bmlhip_spgemm_args *a) {
     if ((rc = win_pack<T>(a->N, a->MB, a->d_Bv, a->d_Bi, a->d_Bn, a->wbits, nullptr, a->d_skip,
                           row_sys, sys_done, wp, s)))
         return rc;
+    int rows = a->wrows;
+    if ((rc = win_union<T>(rows, a->N, a->MB, a->wbits, a->waves, a->d_Bv, a->d_Bi, a->d_Bn,
+                           a->d_skip, row_sys, sys_done, wp, s)))
+        return rc;
+    wp.streamed = rows > 1 ? (unsigned long long *) &bmlhip_status_dev()->prof_streamed[PrecOf<T>::value]
+                           : nullptr;
     // the profiling window spans the product and the mirror
     if (a->N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
     wp.MC = a->MC;
@@ -2168,9 +2667,9 @@ int spgemm_win_typed(const bmlhip_spgemm_args *a) {
     if (sym) {
         fill = (int *) bmlhip_scratch_zeroed(BMLHIP_SYMFILL_SLOT, sizeof(int) * (size_t) (a->N + 1));
         if (!fill) return fail_msg(4, "scratch allocation failed");
-        rc = launch_win_shape<T, true, false>(wp, a->wbits, a->waves, s);
+        rc = launch_win_rows<T, true, false>(wp, rows, a->wbits, a->waves, s);
     } else {
-        rc = launch_win_shape<T, false, false>(wp, a->wbits, a->waves, s);
+        rc = launch_win_rows<T, false, false>(wp, rows, a->wbits, a->waves, s);
     }
     if (rc) return rc;
     SpParams p;
@@ -2192,7 +2691,8 @@ int spgemm_win_typed(const bmlhip_spgemm_args *a) {
     if (a->N > 0) bmlhip_prof_mark_end();
     hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(a->N)), dim3(256), 0, s,
                        (const int4 *) wp.rowstat, a->N, a->MC,
-                       BMLHIP_RS_OVF_KEPT | BMLHIP_RS_STORED | BMLHIP_RS_PROF,
+                       BMLHIP_RS_OVF_KEPT | BMLHIP_RS_STORED | BMLHIP_RS_PROF |
+                           (rows > 1 ? BMLHIP_RS_OWN_STREAM : 0),
                        (int) PrecOf<T>::value, a->d_skip, bmlhip_status_dev());
     if ((rc = report("k_reduce_rowstat"))) return rc;
     bmlhip_status *st = bmlhip_status_dev();
@@ -2212,8 +2712,8 @@ int spgemm_win_typed(const bmlhip_spgemm_args *a) {
 // One fused SP2 iteration on the window engine (symmetric X, in place).
 template <typename T>
 int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int wbits, int waves,
-                     bmlhip_sp2_state *d_state, double nocc, double tol, double *d_hist,
-                     int maxit) {
+                     int wrows, bmlhip_sp2_state *d_state, double nocc, double tol,
+                     double *d_hist, int maxit) {
     hipStream_t s = (hipStream_t) bmlhip_stream();
     int rc = bmlhip_status_reset();
     if (rc) return rc;
@@ -2227,6 +2727,10 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
     WinParams wp;
     memset(&wp, 0, sizeof wp);
     if ((rc = win_pack<T>(N, M, Xv, Xi, Xn, wbits, rowtr, skip, nullptr, nullptr, wp, s))) return rc;
+    int rows = wrows;
+    if ((rc = win_union<T>(rows, N, M, wbits, waves, Xv, Xi, Xn, skip, nullptr, nullptr, wp, s)))
+        return rc;
+    wp.streamed = rows > 1 ? (unsigned long long *) &st->prof_streamed[PrecOf<T>::value] : nullptr;
     // the branch needs tr X and tr X^2 before the product
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rowtr, N, &st->scalars[0]);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rowtr + N, N, &st->scalars[1]);
@@ -2243,7 +2747,7 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
     wp.skip = skip;
     // the profiling window spans the product and the mirror
     if (N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
-    if ((rc = launch_win_shape<T, true, true>(wp, wbits, waves, s))) return rc;
+    if ((rc = launch_win_rows<T, true, true>(wp, rows, wbits, waves, s))) return rc;
     SpParams p;
     memset(&p, 0, sizeof p);
     p.N = N;
@@ -2261,7 +2765,8 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
     }
     hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(N)), dim3(256), 0, s,
                        (const int4 *) rowstat, N, M,
-                       BMLHIP_RS_OVF_KEPT | BMLHIP_RS_STORED | BMLHIP_RS_PROF,
+                       BMLHIP_RS_OVF_KEPT | BMLHIP_RS_STORED | BMLHIP_RS_PROF |
+                           (rows > 1 ? BMLHIP_RS_OWN_STREAM : 0),
                        (int) PrecOf<T>::value, skip, st);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rownorm, N, &st->scalars[2]);
     if ((rc = report("k_sum"))) return rc;
@@ -2671,10 +3176,10 @@ int bmlhip_rowstat_reduce(int N, int MC, int what, int prof, const int *skip) {
 }
 
 int bmlhip_sp2_window_step(int prec, int N, int M, void *Xv, int *Xi, int *Xn, double thr,
-                           int wbits, int waves, bmlhip_sp2_state *d_state, double nocc,
-                           double tol, double *d_hist, int maxit) {
-#define CALL(T) sp2_window_typed<T>(N, M, Xv, Xi, Xn, thr, wbits, waves, d_state, nocc, tol, \
-                                    d_hist, maxit)
+                           int wbits, int waves, int wrows, bmlhip_sp2_state *d_state,
+                           double nocc, double tol, double *d_hist, int maxit) {
+#define CALL(T) sp2_window_typed<T>(N, M, Xv, Xi, Xn, thr, wbits, waves, wrows, d_state, nocc, \
+                                    tol, d_hist, maxit)
     DISPATCH(prec, CALL)
 #undef CALL
 }

@@ -223,6 +223,7 @@ __global__ void k_prof_reset(bmlhip_status *st) {
         st->prof_out_nnz[threadIdx.x] = 0;
         st->prof_ovf_rows[threadIdx.x] = 0;
         st->prof_touched[threadIdx.x] = 0;
+        st->prof_streamed[threadIdx.x] = 0;
     }
 }
 

@@ -82,13 +82,15 @@ static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *
     a.waves = sh.waves;
     a.engine = sh.engine;
     a.wbits = sh.wbits;
+    a.wrows = sh.wrows;
     a.sym = sym;
     if (sh.engine == BMLHIP_ROW_WINDOW && sym)
         /* one fused pass: X^2's upper triangle, the update and the norm,
          * X rewritten in place (X2 is not used) */
         return bmlh_ok(bmlhip_sp2_window_step(X->matrix_precision, X->N, X->M, X->value,
                                               X->index, X->nnz, o->threshold, sh.wbits,
-                                              sh.waves, d_state, nocc, o->tol, d_hist, o->maxit),
+                                              sh.waves, sh.wrows, d_state, nocc, o->tol, d_hist,
+                                              o->maxit),
                        what);
     if (!*X2p && !(*X2p = bml_noinit_matrix_ellpack_hip(X->matrix_precision, X->N, X->M,
                                                         X->distribution_mode)))
@@ -341,6 +343,7 @@ static int batch_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t
     a.waves = sh->waves;
     a.engine = sh->engine;
     a.wbits = sh->wbits;
+    a.wrows = sh->wrows;
     a.sym = d->sym;
     a.batch = &d->b;
     return bmlh_ok(bmlhip_spgemm(&a), what) &&
@@ -485,7 +488,8 @@ void bml_hip_sp2_ellpack_batch(int nsys, bml_matrix_ellpack_hip_t *const *H,
              * every scratch buffer is sized before capture) */
             const int same = graph && sh.group == gshape.group && sh.table == gshape.table &&
                              sh.bucket == gshape.bucket && sh.remap == gshape.remap &&
-                             sh.waves == gshape.waves && sh.engine == gshape.engine;
+                             sh.waves == gshape.waves && sh.engine == gshape.engine &&
+                             sh.wbits == gshape.wbits && sh.wrows == gshape.wrows;
             if (!same) {
                 bmlhip_graph_destroy(graph);
                 graph = NULL;
@@ -783,6 +787,7 @@ void bml_hip_profile_end(bml_hip_profile *out)
             p.out_nnz[k] = st.prof_out_nnz[k];
             p.ovf_rows[k] = st.prof_ovf_rows[k];
             p.touched[k] = st.prof_touched[k];
+            p.streamed[k] = st.prof_streamed[k];
         }
     }
     if (out)

@@ -71,7 +71,8 @@ class Profile(C.Structure):
     _fields_ = [("launches", C.c_int * 5), ("kernel_ms", C.c_double * 5),
                 ("products", C.c_longlong * 5), ("rows", C.c_longlong * 5),
                 ("in_nnz", C.c_longlong * 5), ("out_nnz", C.c_longlong * 5),
-                ("ovf_rows", C.c_longlong * 5), ("touched", C.c_longlong * 5)]
+                ("ovf_rows", C.c_longlong * 5), ("touched", C.c_longlong * 5),
+                ("streamed", C.c_longlong * 5)]
 
 
 _lib = None

@@ -468,6 +468,8 @@ typedef struct bml_hip_profile {
     long long out_nnz[5];    /* entries written */
     long long ovf_rows[5];   /* rows whose LDS table overflowed (second pass) */
     long long touched[5];    /* distinct columns before thresholding */
+    long long streamed[5];   /* entries of B's rows read: the products, or fewer
+                                where several output rows share each read */
 } bml_hip_profile;
 void bml_hip_profile_begin(void);
 void bml_hip_profile_end(bml_hip_profile *out);

@@ -33,9 +33,17 @@ def window(hip):
     hip.set_row_engine(hip.ROW_ENGINE_EXACT)
 
 
+@pytest.fixture(params=[1, 2, 4])
+def rows(request, monkeypatch):
+    """Output rows per block: 1 (k_spgemm_win) or blocks of 2 and 4 rows
+    sharing each streamed row of X (k_win_union + k_spgemm_winb)."""
+    monkeypatch.setenv("BML_HIP_WIN_ROWS", str(request.param))
+    return request.param
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("shape", [(600, 160, 120), (2500, 128, 100), (257, 64, 40)])
-def test_x2_window_matches_oracle(window, dtype, shape):
+def test_x2_window_matches_oracle(window, rows, dtype, shape):
     hip = window
     N, M, fill = shape
     A = G.random_ell(N, M, fill, 700 + N, dtype)
@@ -51,14 +59,18 @@ def test_x2_window_matches_oracle(window, dtype, shape):
     assert rel_close(tr[0], rtr[0], vtol(dtype)) and rel_close(tr[1], rtr[1], vtol(dtype))
 
 
-@pytest.mark.parametrize("wbits", [11, 12, 13])
-@pytest.mark.parametrize("waves", [4, 8])
-def test_window_sizes(window, monkeypatch, wbits, waves):
-    """Windows of 2^11..2^13 columns over N = 9000 (2..5 windows), four or
-    eight wavefronts per row."""
+@pytest.mark.parametrize("rows,wbits,waves", [(1, 10, 8), (1, 11, 8), (1, 12, 8), (1, 13, 8),
+                                               (1, 10, 4), (1, 12, 4), (2, 10, 8), (2, 11, 8),
+                                               (4, 10, 8), (4, 11, 8), (4, 12, 8)])
+def test_window_sizes(window, monkeypatch, rows, wbits, waves):
+    """Windows of 2^10..2^13 columns over N = 9000 (2..9 windows), four or
+    eight wavefronts per block, blocks of 1, 2 or 4 rows (4 rows with
+    2^12-column windows run the one-row kernel: the row blocks are built
+    for windows up to 2^11)."""
     hip = window
     A = G.random_ell(9000, 96, 80, 31, np.float64)
     ref, rtr = orc.x2(A, 9000, 0.02)
+    monkeypatch.setenv("BML_HIP_WIN_ROWS", str(rows))
     monkeypatch.setenv("BML_HIP_WINDOW_BITS", str(wbits))
     monkeypatch.setenv("BML_HIP_WAVES", str(waves))
     X2 = hip.zero_matrix(A.precision, A.N, 4096)
@@ -67,7 +79,7 @@ def test_window_sizes(window, monkeypatch, wbits, waves):
     assert rel_close(tr[0], rtr[0], 1e-12) and rel_close(tr[1], rtr[1], 1e-12)
 
 
-def test_window_empty_rows_and_no_diagonal(window):
+def test_window_empty_rows_and_no_diagonal(window, rows):
     """Rows with no entries, rows whose product never reaches the diagonal
     (no diagonal entry is stored then, as the reference's first-touch list
     never holds it), N = 1."""
@@ -87,7 +99,7 @@ def test_window_empty_rows_and_no_diagonal(window):
     assert_ell_close(Y.get(), ref1, 1e-12, 0.0, "N=1")
 
 
-def test_window_m_overflow_raises(window):
+def test_window_m_overflow_raises(window, rows):
     """A product row with more kept entries than X2's M is the reference's
     fatal "Number of non-zeroes per row > M" (bml_multiply_ellpack_typed.c:
     325-330); the library stays usable afterwards."""
@@ -104,7 +116,7 @@ def test_window_m_overflow_raises(window):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
-def test_window_symmetric_x2(window, monkeypatch, dtype):
+def test_window_symmetric_x2(window, rows, monkeypatch, dtype):
     """The symmetric path the SP2 drivers take (upper triangle formed, the
     rest mirrored): a lattice SP2 iterate, squared."""
     hip = window
@@ -133,7 +145,7 @@ def test_window_symmetric_x2(window, monkeypatch, dtype):
     assert rel_close(tr[1], rtr[1], vtol(dtype))
 
 
-def test_sp2_window_matches_oracle_hop05(window):
+def test_sp2_window_matches_oracle_hop05(window, rows):
     """SP2 on the 10^3 lattice with the hopping of the 32^3 SP2 workload
     (-0.5 +- 0.05 disorder, onsite +-1, threshold 1e-5): the fused window
     step (X^2's upper triangle, the update, the norm, in place) against the
@@ -153,7 +165,7 @@ def test_sp2_window_matches_oracle_hop05(window):
     assert_ell_close(X.get(), ref, 1e-12, 1e-5, "SP2 density matrix")
 
 
-def test_sp2_window_two_windows(window, monkeypatch):
+def test_sp2_window_two_windows(window, rows, monkeypatch):
     """SP2 on the 16^3 lattice (hopping -0.1, threshold 1e-7: ~620 entries
     per row) with windows of 2^11 columns (two windows per row range), 30
     iterations, against the oracle: the same branches, X within
@@ -170,7 +182,7 @@ def test_sp2_window_two_windows(window, monkeypatch):
     assert_ell_close(X.get(), ref, 1e-9, 1e-7, "SP2 two windows")
 
 
-def test_sp2_window_mixed_against_oracle(window):
+def test_sp2_window_mixed_against_oracle(window, rows):
     """Mixed precision on the window engine (single-precision iterations,
     FP64 refinement) against the oracle's FP64 SP2 on the 10^3 lattice at
     hopping -0.5: the density matrix within 2e-5 of the FP64 one (the
