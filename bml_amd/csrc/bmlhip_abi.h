@@ -70,7 +70,7 @@ void *bmlhip_stream(void);
 /* Library-owned scratch, grown on demand; slot selects one of a few
  * independent buffers.  The pointer stays valid until the next call with
  * the same slot and a larger size. */
-#define BMLHIP_SCRATCH_SLOTS 20
+#define BMLHIP_SCRATCH_SLOTS 24
 void *bmlhip_scratch(int slot, size_t bytes);
 /* as bmlhip_scratch, zero-filled whenever the buffer is (re)allocated */
 void *bmlhip_scratch_zeroed(int slot, size_t bytes);
@@ -146,6 +146,11 @@ typedef struct bmlhip_spgemm_args {
 #define BMLHIP_WIN_UJ_SLOT 16
 #define BMLHIP_WIN_UC_SLOT 17
 #define BMLHIP_WIN_UN_SLOT 18
+/* ELLBLOCK bf16 engine: bfloat16 copies of A's tiles and of B's, transposed */
+#define BMLHIP_EB_A16_SLOT 19
+#define BMLHIP_EB_B16_SLOT 20
+/* window engine, single precision on row blocks: fixed-point bounds, scale, flag */
+#define BMLHIP_WIN_FX_SLOT 21
 /* window engine limits: at most 64 windows per row range */
 #define BMLHIP_WIN_MAX_WINDOWS 64
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);

@@ -85,6 +85,7 @@ struct EbParams {
     int remap;  // numeric pass: XCD-aware block-row order (xcd_block)
     int sym;    // x2 of an exactly symmetric X: each row forms the slots it owns (sym_owner)
     int *kflag; // [NB*maxs] sym: kept flag of every slot, set by its row or its mirror
+    const __bf16 *A16, *B16;  // bf16 engine: A's tiles row-major, B's transposed (k_eb_bf16)
 };
 
 // The symmetric X^2 forms each off-diagonal pair of blocks (i, k), (k, i)
@@ -351,41 +352,56 @@ struct MfmaAccF32 {
 // MFMA engine, single real inputs rounded to bfloat16, FP32 accumulation:
 // v_mfma_f32_32x32x16_bf16 takes 8 consecutive k per lane (lanes 0-31 one
 // group of 8, lanes 32-63 the next); instruction q covers k = 16q .. 16q+15.
-// Same output layout as the FP32 engine.
+// Same output layout as the FP32 engine.  The operands come from bfloat16
+// copies of the tiles made before the pass (k_eb_bf16): A's tile row-major,
+// B's transposed, so a lane's 8 consecutive k of its row of A, or of its
+// column of B, are one 16-byte load (2 KB per tile against the 4 KB of the
+// single-precision tile).
 struct MfmaAccBF16 : MfmaAccF32 {
     struct Frag {
         bf16x8 av[2], bv[2];
     };
-    __device__ static void load(const float *L, const float *R, Frag &f) {
+    __device__ static void load(const __bf16 *L, const __bf16 *Rt, Frag &f) {
         const int lane = lane_id();
         const int i = lane & 31, h = lane >> 5;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int k0 = 16 * q + 8 * h;
-            const float4 *l4 = reinterpret_cast<const float4 *>(L + i * 32 + k0);
-            const float4 x0 = l4[0], x1 = l4[1];
-            f.av[q][0] = (__bf16) x0.x;
-            f.av[q][1] = (__bf16) x0.y;
-            f.av[q][2] = (__bf16) x0.z;
-            f.av[q][3] = (__bf16) x0.w;
-            f.av[q][4] = (__bf16) x1.x;
-            f.av[q][5] = (__bf16) x1.y;
-            f.av[q][6] = (__bf16) x1.z;
-            f.av[q][7] = (__bf16) x1.w;
-#pragma unroll
-            for (int e = 0; e < 8; e++) f.bv[q][e] = (__bf16) R[(k0 + e) * 32 + i];
+            f.av[q] = *reinterpret_cast<const bf16x8 *>(L + i * 32 + k0);
+            f.bv[q] = *reinterpret_cast<const bf16x8 *>(Rt + i * 32 + k0);
         }
     }
     __device__ void mma(const Frag &f) {
 #pragma unroll
         for (int q = 0; q < 2; q++) a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.av[q], f.bv[q], a, 0, 0, 0);
     }
-    __device__ void mac(const float *L, const float *R, int, int, int) {
-        Frag f;
-        load(L, R, f);
-        mma(f);
-    }
 };
+
+// bfloat16 copies of the stored tiles of a single-precision matrix with
+// 32 x 32 tiles, for the bf16 engine: `rm` (if not null) row-major as
+// stored, `tr` (if not null) transposed.  One block per block row, its
+// stored slots in turn; each element rounded to nearest even, as a
+// conversion inside the product would round it.  Slots past nnzb are not
+// written (no pair reads them).
+__global__ __launch_bounds__(256) void k_eb_bf16(int M, const float *V, const int *nnzb,
+                                                 __bf16 *rm, __bf16 *tr, const int *skip) {
+    __shared__ float t[32][33];
+    if (skip && *skip) return;  // uniform
+    const int ib = blockIdx.x;
+    const int n = nnzb[ib];
+    for (int jp = 0; jp < n; jp++) {
+        const size_t base = ((size_t) ib * M + jp) * 1024;
+        for (int e = threadIdx.x; e < 1024; e += 256) {
+            const float v = V[base + e];
+            if (rm) rm[base + e] = (__bf16) v;
+            t[e >> 5][e & 31] = v;
+        }
+        __syncthreads();
+        if (tr)
+            for (int e = threadIdx.x; e < 1024; e += 256) tr[base + e] = (__bf16) t[e & 31][e >> 5];
+        __syncthreads();
+    }
+}
 
 // MFMA engine, double real, 32 x 32 tiles as 2 x 2 tiles of 16 x 16:
 // v_mfma_f64_16x16x4f64 takes a 16x4 slice of L (lane l: row l%16, k group
@@ -761,8 +777,16 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
             decode(n);
             return true;
         };
-        auto tile_a = [&](const Unit &u, int q) { return Av + (size_t) __builtin_amdgcn_readlane(u.sa, q) * B2; };
-        auto tile_b = [&](const Unit &u, int q) { return Bv + (size_t) __builtin_amdgcn_readlane(u.bt, q) * B2; };
+        auto tile_a = [&](const Unit &u, int q) {
+            const size_t t = (size_t) __builtin_amdgcn_readlane(u.sa, q) * B2;
+            if constexpr (ENGINE == 2) return p.A16 + t;
+            else return Av + t;
+        };
+        auto tile_b = [&](const Unit &u, int q) {
+            const size_t t = (size_t) __builtin_amdgcn_readlane(u.bt, q) * B2;
+            if constexpr (ENGINE == 2) return p.B16 + t;
+            else return Bv + t;
+        };
         Unit cu, nu;
         cu.np = 0;
         if (next_unit(cu, cu)) {
@@ -1311,6 +1335,27 @@ int eb_product_typed(const bmlhip_eb_args *a) {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
     hipLaunchKernelGGL(k_eb_symbolic, dim3(a->NB), dim3(64), lds, s, p, tbits);
     if ((rc = report("k_eb_symbolic"))) return rc;
+    if (std::is_same<T, float>::value && a->bt == 32 && a->engine == 2) {
+        // bf16 engine: the tiles in bfloat16 once per product (A row-major,
+        // B transposed; one pass when A is B)
+        __bf16 *a16 = (__bf16 *) bmlhip_scratch(BMLHIP_EB_A16_SLOT,
+                                                sizeof(__bf16) * 1024 * (NB * a->MA) + 256);
+        __bf16 *b16 = (__bf16 *) bmlhip_scratch(BMLHIP_EB_B16_SLOT,
+                                                sizeof(__bf16) * 1024 * (NB * a->MB) + 256);
+        if (!a16 || !b16) return fail_msg(4, "ellblock product: scratch allocation failed");
+        if (a->d_Av == a->d_Bv && a->MA == a->MB) {
+            hipLaunchKernelGGL(k_eb_bf16, dim3(a->NB), dim3(256), 0, s, a->MA,
+                               (const float *) a->d_Av, a->d_An, a16, b16, p.skip);
+        } else {
+            hipLaunchKernelGGL(k_eb_bf16, dim3(a->NB), dim3(256), 0, s, a->MA,
+                               (const float *) a->d_Av, a->d_An, a16, (__bf16 *) nullptr, p.skip);
+            hipLaunchKernelGGL(k_eb_bf16, dim3(a->NB), dim3(256), 0, s, a->MB,
+                               (const float *) a->d_Bv, a->d_Bn, (__bf16 *) nullptr, b16, p.skip);
+        }
+        if ((rc = report("k_eb_bf16"))) return rc;
+        p.A16 = a16;
+        p.B16 = b16;
+    }
     const int prec_mark = a->prec;
     bmlhip_prof_mark_begin(prec_mark);
     if (std::is_same<T, float>::value && a->bt == 32 && a->engine == 1) {
