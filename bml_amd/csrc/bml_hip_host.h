@@ -40,6 +40,8 @@ int bmlh_sync_mode(void);
 size_t bmlh_elem_size(int precision);
 /* block-product engine for products of A (BML_HIP_BLOCK_ENGINE_*, resolved) */
 int bmlh_block_engine_for(const bml_matrix_ellblock_hip_t *A);
+/* the engine set by bml_hip_set_block_engine (BML_HIP_BLOCK_ENGINE_*) */
+int bmlh_block_engine_setting(void);
 /* the real scalar s as one element of precision prec, into buf (16 bytes) */
 void bmlh_put_scalar(int prec, double s, void *buf);
 struct bml_hip_aux *bmlh_aux_new(void);

@@ -149,8 +149,6 @@ typedef struct bmlhip_spgemm_args {
 /* ELLBLOCK bf16 engine: bfloat16 copies of A's tiles and of B's, transposed */
 #define BMLHIP_EB_A16_SLOT 19
 #define BMLHIP_EB_B16_SLOT 20
-/* window engine, single precision on row blocks: fixed-point bounds, scale, flag */
-#define BMLHIP_WIN_FX_SLOT 21
 /* window engine limits: at most 64 windows per row range */
 #define BMLHIP_WIN_MAX_WINDOWS 64
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);

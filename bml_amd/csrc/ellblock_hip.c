@@ -90,6 +90,11 @@ void bml_hip_set_block_engine(int engine)
     s_engine = engine;
 }
 
+int bmlh_block_engine_setting(void)
+{
+    return s_engine;
+}
+
 /* ---- helpers ------------------------------------------------------------ */
 
 static int valid_eb(const bml_matrix_ellblock_hip_t *A, const char *what)

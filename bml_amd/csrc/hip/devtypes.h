@@ -152,13 +152,6 @@ __device__ __forceinline__ void lds_add(cf32 *p, cf32 v) {
     lds_add(&p->x, v.x);
     lds_add(&p->y, v.y);
 }
-__device__ __forceinline__ void lds_add(int *p, int v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void lds_add(int2 *p, int2 v) {
-    lds_add(&p->x, v.x);
-    lds_add(&p->y, v.y);
-}
 __device__ __forceinline__ void lds_add(cf64 *p, cf64 v) {
     lds_add(&p->x, v.x);
     lds_add(&p->y, v.y);

@@ -764,39 +764,6 @@ template <> struct UAcc<cf64> {
     __device__ static cf64 from(cf64 v) { return v; }
 };
 
-// Window accumulators of the row-block kernel.  FX (single precision):
-// 32-bit fixed point.  Each product is formed from a coefficient pre-scaled
-// by 2^s (exact), rounded to the nearest integer and added with ds_add_u32,
-// which ran 1.8x faster than ds_add_f64 on gfx950 (7.7e12 against 4.3e12
-// adds/s) at half the LDS; integer sums do not depend on the order of the
-// adds.  s is chosen per product from bounds the pack pass reduces: every
-// partial sum of row i is at most max|X| * max_i sum_j |X_ij| = B, and
-// B * 2^s <= 2^30.  Otherwise double (complex: two doubles) as UAcc.
-template <typename T, bool FX> struct WAcc {
-    using type = typename UAcc<T>::type;
-    __device__ static T scale(T a, int) { return a; }
-    __device__ static type to(T p) { return UAcc<T>::to(p); }
-    __device__ static T from(type a, int) { return UAcc<T>::from(a); }
-};
-template <> struct WAcc<float, true> {
-    using type = int;
-    __device__ static float scale(float a, int s) { return ldexpf(a, s); }
-    __device__ static int to(float p) { return (int) rintf(p); }
-    __device__ static float from(int a, int s) { return ldexpf((float) a, -s); }
-};
-template <> struct WAcc<cf32, true> {
-    using type = int2;
-    __device__ static cf32 scale(cf32 a, int s) { return {ldexpf(a.x, s), ldexpf(a.y, s)}; }
-    __device__ static int2 to(cf32 p) { return make_int2((int) rintf(p.x), (int) rintf(p.y)); }
-    __device__ static cf32 from(int2 a, int s) {
-        return {ldexpf((float) a.x, -s), ldexpf((float) a.y, -s)};
-    }
-};
-template <typename T> struct IsSingle { static constexpr bool value = false; };
-template <> struct IsSingle<float> { static constexpr bool value = true; };
-template <> struct IsSingle<cf32> { static constexpr bool value = true; };
-constexpr int FX_MIN_BITS = 24;  // fixed point resolves 2^-24 or better, else double
-
 // SYM (X symmetric or Hermitian, SP2 on such an H): row i forms only the
 // columns k >= i of X^2 (the other lanes of a chunk sit idle: no probe, no
 // add), the table holds about half the row's columns, and k_sym_mirror
@@ -1116,11 +1083,6 @@ struct WinParams {
     const int *Un;     // [nblk] union sizes
     size_t ustride;
     unsigned long long *streamed;  // prof_streamed of the precision
-    // single precision, fixed point: max|X| and max row sum of |X| (float
-    // bits, reduced by the pack pass), the scale exponent and the fallback
-    // flag (set when the scale would resolve less than 2^-FX_MIN_BITS)
-    unsigned *fxb;
-    int *fxs;
 };
 
 // Pack pass: one wavefront per row.  Row j's entries are grouped by window
@@ -1135,9 +1097,8 @@ __global__ __launch_bounds__(256) void k_win_pack(int N, int M, const T *Xv, con
                                                   unsigned *P, int *boff,
                                                   unsigned long long *wmask, double *rowtr,
                                                   const int *skip, const int *row_sys,
-                                                  const int *sys_done, unsigned *fxb) {
+                                                  const int *sys_done) {
     __shared__ int cnt[4][65];
-    __shared__ float fxm_s[4], fxr_s[4];
     if (skip && *skip) return;  // uniform
     const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
     const int j = blockIdx.x * 4 + w;
@@ -1192,51 +1153,6 @@ __global__ __launch_bounds__(256) void k_win_pack(int N, int M, const T *Xv, con
             rowtr[N + j] = sq;
         }
     }
-    if (fxb) {
-        // fixed-point bounds: the row's max |X_jk| and sum of |X_jk|, then
-        // the block's, folded into fxb as float bits (non-negative floats
-        // order as their bit patterns)
-        float am = 0.0f, rs = 0.0f;
-        for (int e = lane; e < n; e += 64) {
-            const float m = (float) Elem<T>::mag(Xv[rowX + e]);
-            am = fmaxf(am, m);
-            rs += m;
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            am = fmaxf(am, __shfl_down(am, off, 64));
-            rs += __shfl_down(rs, off, 64);
-        }
-        if (lane == 0) {
-            fxm_s[w] = am;
-            fxr_s[w] = rs * 1.0001f;  // room for the float sum's rounding
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float bm = 0.0f, br = 0.0f;
-            for (int q = 0; q < 4; q++) {
-                bm = fmaxf(bm, fxm_s[q]);
-                br = fmaxf(br, fxr_s[q]);
-            }
-            atomicMax(&fxb[0], __float_as_uint(bm));
-            atomicMax(&fxb[1], __float_as_uint(br));
-        }
-    }
-}
-
-// Fixed-point scale of the next product from the pack pass's bounds: s with
-// max|X| * max row sum * 2^s <= 2^30, and the fallback flag when s would
-// resolve less than 2^-FX_MIN_BITS; the bounds are cleared for the next
-// product.
-__global__ void k_fx_scale(unsigned *fxb, int *fxs, const int *skip) {
-    if (threadIdx.x != 0 || (skip && *skip)) return;
-    const double B = (double) __uint_as_float(fxb[0]) * (double) __uint_as_float(fxb[1]);
-    int e = 0;
-    if (B > 0.0) (void) frexp(B, &e);  // B <= 2^e
-    const int sx = B > 0.0 ? 30 - e : 30;
-    fxs[0] = sx;
-    fxs[1] = sx < FX_MIN_BITS;
-    fxb[0] = 0u;
-    fxb[1] = 0u;
 }
 
 // One block of W wavefronts per output row; LDS: the window's accumulators
@@ -1558,11 +1474,10 @@ __global__ __launch_bounds__(256) void k_win_union(int N, int M, const T *Xv, co
     }
 }
 
-template <typename T, int W, int WB, int R, bool SYM, bool FUSE, bool FX>
+template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
 __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
     using E = Elem<T>;
-    using WA = WAcc<T, FX>;
-    using A = typename WA::type;
+    using A = typename UAcc<T>::type;
     constexpr int WC = 1 << WB;
     constexpr int NT = 64 * W;
     constexpr int PER = WC / W;  // epilogue columns per wavefront
@@ -1577,15 +1492,6 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
     __shared__ double nrm_s[W][R];
     __shared__ T trx_s[R], d2_s[R];
     if (p.skip && *p.skip) return;  // uniform
-    // fixed point: the scale, unless the bounds forced the double kernel,
-    // which runs after this one and only then (uniform reads)
-    int sx = 0;
-    if constexpr (FX) {
-        if (p.fxs[1]) return;
-        sx = p.fxs[0];
-    } else {
-        if (p.fxs && !p.fxs[1]) return;
-    }
     const int b = blockIdx.x, i0 = b * R;
     if (i0 >= p.N) return;
     const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
@@ -1658,7 +1564,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
         m &= m - 1ull;
         const int cbase = c << WB;
         for (int s = tid; s < R * WC; s += NT) {
-            win[s] = A{};
+            win[s] = Elem<A>::zero();
             if constexpr (FUSE) xwin[s] = E::zero();
         }
 #pragma unroll
@@ -1683,7 +1589,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                 lo = bo[c];
                 hi = bo[c + 1];
 #pragma unroll
-                for (int r = 0; r < R; r++) cq[r] = WA::scale(Uc[(size_t) e * R + r], sx);
+                for (int r = 0; r < R; r++) cq[r] = Uc[(size_t) e * R + r];
             }
             uint64_t todo = ballot64(hi > lo);
             while (todo) {
@@ -1692,12 +1598,16 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                 const int j = __builtin_amdgcn_readlane(jq, q);
                 const int l0 = __builtin_amdgcn_readlane(lo, q);
                 const int l1 = __builtin_amdgcn_readlane(hi, q);
-                T a[R];
+                // products in the accumulator's precision (a single-precision
+                // entry is widened once for all R rows; the product of two
+                // floats is exact in double)
+                A a[R];
                 unsigned rm = 0;  // rows with a nonzero coefficient (wave-uniform)
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    a[r] = bcast<64>(cq[r], q);
-                    if (E::nonzero(a[r])) rm |= 1u << r;
+                    const T ar = bcast<64>(cq[r], q);
+                    a[r] = UAcc<T>::to(ar);
+                    if (E::nonzero(ar)) rm |= 1u << r;
                 }
                 const size_t rb = (size_t) j * p.MP;
                 streamed += l1 - l0;
@@ -1713,13 +1623,14 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                     int c0, c1;
                     pk_get<T>(P, rb + k0, b0, c0);
                     pk_get<T>(P, rb + k0 + 64, b1, c1);
+                    const A w0 = UAcc<T>::to(b0), w1 = UAcc<T>::to(b1);
 #pragma unroll
                     for (int r = 0; r < R; r++) {
                         if (!(rm >> r & 1u)) continue;  // uniform
                         const int s0 = (!SYM || c0 >= i0 + r) ? r * WC + c0 - cbase : dummy;
                         const int s1 = (!SYM || c1 >= i0 + r) ? r * WC + c1 - cbase : dummy;
-                        lds_add(&win[s0], WA::to(E::mul(a[r], b0)));
-                        lds_add(&win[s1], WA::to(E::mul(a[r], b1)));
+                        lds_add(&win[s0], Elem<A>::mul(a[r], w0));
+                        lds_add(&win[s1], Elem<A>::mul(a[r], w1));
                         if constexpr (!SYM) dtouch[r] = dtouch[r] | (c0 == i0 + r) | (c1 == i0 + r);
                     }
                 }
@@ -1727,11 +1638,12 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                     T b0;
                     int c0;
                     pk_get<T>(P, rb + k0, b0, c0);
+                    const A w0 = UAcc<T>::to(b0);
 #pragma unroll
                     for (int r = 0; r < R; r++) {
                         if (!(rm >> r & 1u)) continue;  // uniform
                         const int s0 = (!SYM || c0 >= i0 + r) ? r * WC + c0 - cbase : dummy;
-                        lds_add(&win[s0], WA::to(E::mul(a[r], b0)));
+                        lds_add(&win[s0], Elem<A>::mul(a[r], w0));
                         if constexpr (!SYM) dtouch[r] = dtouch[r] | (c0 == i0 + r);
                     }
                 }
@@ -1770,7 +1682,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
             auto entry = [&](int s, T &out, bool &keep, bool &nz, double &dn) {
                 const int k = cbase + s;
                 const bool valid = k < p.N && (!SYM || k >= i);
-                const T y = WA::from(win[r * WC + s], sx);
+                const T y = UAcc<T>::from(win[r * WC + s]);
                 nz = valid && (E::mag(y) != 0.0 || k == i);
                 const bool ykeep = valid && (k == i ? dtouched : E::above(y, p.thr));
                 dn = 0.0;
@@ -2598,32 +2510,21 @@ int launch_win_shape(const WinParams &wp, int wbits, int waves, hipStream_t s) {
                     wbits, waves);
 }
 
-template <typename T, int W, int WB, int R, bool SYM, bool FUSE, bool FX>
-int launch_winb1(const WinParams &wp, hipStream_t s) {
-    using A = typename WAcc<T, FX>::type;
+template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
+int launch_winb(const WinParams &wp, hipStream_t s) {
+    using A = typename UAcc<T>::type;
     const size_t WC = (size_t) 1 << WB;
     const size_t lds = sizeof(A) * (R * WC + 64) + (FUSE ? sizeof(T) * R * WC : 0);
     if (lds > 160 * 1024 - 2048)
         return fail_msg(5, "window engine: %d rows of 2^%d columns need %zu bytes of LDS", R, WB,
                         lds);
     if (lds > 64 * 1024)
-        (void) hipFuncSetAttribute((const void *) k_spgemm_winb<T, W, WB, R, SYM, FUSE, FX>,
+        (void) hipFuncSetAttribute((const void *) k_spgemm_winb<T, W, WB, R, SYM, FUSE>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
     if (wp.N > 0)
-        hipLaunchKernelGGL((k_spgemm_winb<T, W, WB, R, SYM, FUSE, FX>), dim3((wp.N + R - 1) / R),
+        hipLaunchKernelGGL((k_spgemm_winb<T, W, WB, R, SYM, FUSE>), dim3((wp.N + R - 1) / R),
                            dim3(64 * W), lds, s, wp);
     return report("k_spgemm_winb");
-}
-
-// Single precision: the fixed-point kernel, then the double-accumulator one,
-// which returns at once unless the scale fell short (wp.fxs[1]).
-template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
-int launch_winb(const WinParams &wp, hipStream_t s) {
-    if constexpr (IsSingle<T>::value) {
-        const int rc = launch_winb1<T, W, WB, R, SYM, FUSE, true>(wp, s);
-        if (rc) return rc;
-    }
-    return launch_winb1<T, W, WB, R, SYM, FUSE, false>(wp, s);
 }
 
 // The product on blocks of `rows` output rows (2 or 4, after win_union), or
@@ -2708,7 +2609,7 @@ inline int win_count(int N, int wbits) {
 template <typename T>
 int win_pack(int N, int M, const void *Xv, const int *Xi, const int *Xn, int wbits,
              double *rowtr, const int *skip, const int *row_sys, const int *sys_done,
-             WinParams &wp, hipStream_t s, unsigned *fxb = nullptr) {
+             WinParams &wp, hipStream_t s) {
     const int nwin = win_count(N, wbits);
     if (nwin < 0)
         return fail_msg(5, "window engine: N=%d needs more than %d windows of 2^%d columns", N,
@@ -2728,26 +2629,8 @@ int win_pack(int N, int M, const void *Xv, const int *Xi, const int *Xn, int wbi
     wp.wmask = wmask;
     if (N > 0)
         hipLaunchKernelGGL(k_win_pack<T>, dim3((N + 3) / 4), dim3(256), 0, s, N, M, (const T *) Xv,
-                           Xi, Xn, M, nwin, wbits, P, boff, wmask, rowtr, skip, row_sys, sys_done,
-                           fxb);
+                           Xi, Xn, M, nwin, wbits, P, boff, wmask, rowtr, skip, row_sys, sys_done);
     return report("k_win_pack");
-}
-
-// Single precision on row blocks: the fixed-point bounds (zeroed scratch,
-// left zeroed by k_fx_scale after each product) for the pack pass, and the
-// scale and flag words the product reads.  Null for other cases.
-template <typename T>
-unsigned *win_fx_words(int rows) {
-    if (!IsSingle<T>::value || rows < 2) return nullptr;
-    return (unsigned *) bmlhip_scratch_zeroed(BMLHIP_WIN_FX_SLOT, 4 * sizeof(unsigned));
-}
-
-inline int win_fx_scale(unsigned *fx, const int *skip, WinParams &wp, hipStream_t s) {
-    if (!fx) return 0;
-    wp.fxb = fx;
-    wp.fxs = (int *) (fx + 2);
-    hipLaunchKernelGGL(k_fx_scale, dim3(1), dim3(64), 0, s, fx, (int *) (fx + 2), skip);
-    return report("k_fx_scale");
 }
 
 // X^2 on the window engine (a->d_Av == a->d_Bv): pack, rows, mirror (sym),
@@ -2761,17 +2644,13 @@ int spgemm_win_typed(const bmlhip_spgemm_args *a) {
     memset(&wp, 0, sizeof wp);
     const int *row_sys = a->batch ? a->batch->d_row_sys : nullptr;
     const int *sys_done = a->batch ? a->batch->d_done : nullptr;
-    unsigned *fx = win_fx_words<T>(a->wrows);
-    if (IsSingle<T>::value && a->wrows > 1 && !fx) return fail_msg(4, "scratch allocation failed");
     if ((rc = win_pack<T>(a->N, a->MB, a->d_Bv, a->d_Bi, a->d_Bn, a->wbits, nullptr, a->d_skip,
-                          row_sys, sys_done, wp, s, fx)))
+                          row_sys, sys_done, wp, s)))
         return rc;
     int rows = a->wrows;
     if ((rc = win_union<T>(rows, a->N, a->MB, a->wbits, a->waves, a->d_Bv, a->d_Bi, a->d_Bn,
                            a->d_skip, row_sys, sys_done, wp, s)))
         return rc;
-    if ((rc = win_fx_scale(fx, a->d_skip, wp, s))) return rc;
-    if (rows == 1) wp.fxs = nullptr;
     wp.streamed = rows > 1 ? (unsigned long long *) &bmlhip_status_dev()->prof_streamed[PrecOf<T>::value]
                            : nullptr;
     // the profiling window spans the product and the mirror
@@ -2853,15 +2732,10 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
     if (!rowtr || !rownorm || !rowstat || !fill) return fail_msg(4, "scratch allocation failed");
     WinParams wp;
     memset(&wp, 0, sizeof wp);
-    unsigned *fx = win_fx_words<T>(wrows);
-    if (IsSingle<T>::value && wrows > 1 && !fx) return fail_msg(4, "scratch allocation failed");
-    if ((rc = win_pack<T>(N, M, Xv, Xi, Xn, wbits, rowtr, skip, nullptr, nullptr, wp, s, fx)))
-        return rc;
+    if ((rc = win_pack<T>(N, M, Xv, Xi, Xn, wbits, rowtr, skip, nullptr, nullptr, wp, s))) return rc;
     int rows = wrows;
     if ((rc = win_union<T>(rows, N, M, wbits, waves, Xv, Xi, Xn, skip, nullptr, nullptr, wp, s)))
         return rc;
-    if ((rc = win_fx_scale(fx, skip, wp, s))) return rc;
-    if (rows == 1) wp.fxs = nullptr;
     wp.streamed = rows > 1 ? (unsigned long long *) &st->prof_streamed[PrecOf<T>::value] : nullptr;
     // the branch needs tr X and tr X^2 before the product
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, rowtr, N, &st->scalars[0]);

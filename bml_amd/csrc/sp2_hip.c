@@ -672,6 +672,7 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
     }
     bml_hip_sp2_result r;
     memset(&r, 0, sizeof r);
+    const int engine0 = bmlh_block_engine_setting(); /* restored at the end */
     double *g = bml_gershgorin_ellblock_hip(H);
     if (!g)
         return;
@@ -718,6 +719,11 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
     bmlhip_sp2_state st;
     memset(&st, 0, sizeof st);
     stall_track stall = { 0, 0.0, 0.0 };
+    /* bf16 inputs: once the bf16 iterate stalls, the single-precision phase
+     * goes on with FP32 MFMA until that stalls too, then double (bf16's 8-bit
+     * mantissa stalls ~1e-3 from idempotency; FP32 iterations bring it far
+     * closer for less than the double-precision ones would cost) */
+    int bf16 = single && bmlh_block_engine_for(Xs) == BML_HIP_BLOCK_ENGINE_MFMA_BF16;
     for (int it = 0; it < o.maxit; it++) {
         bml_matrix_ellblock_hip_t *cur = single ? Xs : X, *cur2 = single ? X2s : X2;
         if (!eb_sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, sym, what))
@@ -730,8 +736,15 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
                 r.single_iterations = st.it;
             if (st.done)
                 break;
-            if (single && stalled(&o, &stall, st.norm))
-                last_single = 1;
+            if (single && stalled(&o, &stall, st.norm)) {
+                if (bf16) {
+                    bf16 = 0;
+                    bml_hip_set_block_engine(BML_HIP_BLOCK_ENGINE_MFMA);
+                    stall.seen = 0;
+                } else {
+                    last_single = 1;
+                }
+            }
         }
         if (last_single) {
             if (!eb_convert(Xs, X, what))
@@ -759,6 +772,8 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
     if (res)
         *res = r;
 out:
+    if (bmlh_block_engine_setting() != engine0)
+        bml_hip_set_block_engine(engine0);
     bmlhip_free(d_state);
     bmlhip_free(d_hist);
     bml_deallocate_ellblock_hip(X2);

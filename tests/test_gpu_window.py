@@ -200,36 +200,3 @@ def test_sp2_window_mixed_against_oracle(window, rows):
     D, R = X.get().to_dense(), ref.to_dense()
     assert np.abs(D - R).max() <= 5e-5
     assert abs(np.trace(D) - np.trace(R)) <= 1e-6 * 500
-
-
-@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
-def test_window_single_fixed_point(window, monkeypatch, dtype):
-    """Single precision on row blocks accumulates in 32-bit fixed point at
-    a scale set from max|X| * max row sum of |X| (integer sums: the same
-    bits on every run, whatever the order of the adds); a matrix whose
-    bound leaves fewer than 24 fraction bits runs the double-accumulator
-    kernel instead.  Both against the oracle to the single-precision
-    tolerance."""
-    hip = window
-    monkeypatch.setenv("BML_HIP_WIN_ROWS", "4")
-    A = G.random_ell(2500, 128, 100, 77, dtype)
-    ref, rtr = orc.x2(A, A.N, 0.05)
-    dA = hip.from_ell(A)
-    outs = []
-    for _ in range(2):
-        X2 = hip.zero_matrix(A.precision, A.N, A.N)
-        tr = hip.multiply_x2(dA, X2, 0.05)
-        outs.append(X2.get())
-        assert rel_close(tr[1], rtr[1], 2e-5)
-    assert_ell_close(outs[0], ref, 2e-5, 0.05, "fixed point")
-    for i in range(A.N):
-        n = int(outs[0].nnz[i])
-        assert outs[1].nnz[i] == n
-        assert np.array_equal(outs[0].value[i, :n].view(np.uint8), outs[1].value[i, :n].view(np.uint8))
-    # values 40x larger: the bound (~40 * 40 * 100) leaves < 24 fraction bits
-    B = A.copy()
-    B.value = (B.value * 40).astype(dtype)
-    refb, _ = orc.x2(B, B.N, 2.0)
-    X2 = hip.zero_matrix(B.precision, B.N, B.N)
-    hip.multiply_x2(hip.from_ell(B), X2, 2.0)
-    assert_ell_close(X2.get(), refb, 2e-5, 2.0, "double fallback")
