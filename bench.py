@@ -56,9 +56,13 @@ HOP_SMALL, DISORDER_SMALL = -0.1, 0.01
 
 def bytes_alg(prof, p, s):
     """Streamed-Gustavson bytes of the precision-p row-product launches in a
-    profile: A rows (s = element + index bytes per entry) and nnz, each
-    referenced B row plus its nnz entry, the C rows and nnz."""
-    return (s * prof.in_nnz[p] + 4 * prof.rows[p] + s * prof.products[p]
+    profile: A rows (s = element + index bytes per entry) and nnz, the
+    entries of B streamed (one referenced B row per A entry; the window
+    engine's row blocks stream each row referenced by any of a block's rows
+    once for the block, which the profile counts) plus one nnz read per A
+    entry, the C rows and nnz."""
+    streamed = prof.streamed[p] or prof.products[p]
+    return (s * prof.in_nnz[p] + 4 * prof.rows[p] + s * streamed
             + 4 * prof.in_nnz[p] + s * prof.out_nnz[p] + 4 * prof.rows[p])
 
 
@@ -74,6 +78,7 @@ def roofline(prof, p, s, name):
             "avg_launch_ms": round(avg_ms, 4), "launches": n,
             "bytes_alg_per_launch": int(per_launch),
             "products_per_launch": int(prof.products[p] / n),
+            "streamed_per_launch": int((prof.streamed[p] or prof.products[p]) / n),
             "touched_per_row": round(prof.touched[p] / max(prof.rows[p], 1), 1),
             "table_overflow_rows_per_launch": round(prof.ovf_rows[p] / n, 1),
             "gflops": round(2.0 * prof.products[p] / (prof.kernel_ms[p] * 1e-3) / 1e9, 2)}
@@ -472,10 +477,10 @@ def main():
     t_dbl, p_dbl, h_dbl, X64 = sp2_steps(hip, G, np.float64, args.steps, args.warmup)
 
     # dominant kernel of the timed steps: the fused window product (+ mirror)
-    roof = roofline(p_mix, 1, 8, "k_spgemm_win<float, 8, 11, true, true> + k_sym_mirror<float> "
-                                 "(fused FP32 SP2 step)")
-    roof64 = roofline(p_dbl, 2, 12, "k_spgemm_win<double, 8, 11, true, true> + "
-                                    "k_sym_mirror<double> (fused FP64 SP2 step)")
+    roof = roofline(p_mix, 1, 8, "k_spgemm_winb<float, 8, 10, 4, true, true> + "
+                                 "k_sym_mirror<float> (fused FP32 SP2 step, 4-row blocks)")
+    roof64 = roofline(p_dbl, 2, 12, "k_spgemm_winb<double, 8, 10, 4, true, true> + "
+                                    "k_sym_mirror<double> (fused FP64 SP2 step, 4-row blocks)")
     traffic = profiled_traffic()
     for r, p in ((roof, 1), (roof64, 2)):
         if r is not None:

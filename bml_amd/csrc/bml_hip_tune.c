@@ -29,8 +29,9 @@
 #include <stdlib.h>
 
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
-                      int hint_max_a, int square, bmlh_spgemm_shape *out)
+                      int hint_max_a, int flags, bmlh_spgemm_shape *out)
 {
+    const int square = flags & BMLH_PICK_SQUARE;
     int g = 8;
     while (g < 64 && 3 * g < max_b)
         g <<= 1;
@@ -101,7 +102,8 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
             tm -= 64;
         t = tm;
     }
-    const int sel = bml_hip_get_row_engine();
+    const int sel =
+        flags & BMLH_PICK_EXACT ? BML_HIP_ROW_ENGINE_EXACT : bml_hip_get_row_engine();
     int engine = (sel == BML_HIP_ROW_ENGINE_UNORDERED || sel == BML_HIP_ROW_ENGINE_WINDOW) &&
                          g == 64
                      ? BMLHIP_ROW_UNORDERED

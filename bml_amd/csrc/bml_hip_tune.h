@@ -18,10 +18,12 @@ typedef struct bmlh_spgemm_shape {
 /* Launch shape for a product whose left rows hold up to max_a entries and
  * right rows up to max_b; touched_hint is the largest distinct-column count
  * seen the last time (0 = unknown), when the left rows held up to
- * hint_max_a entries.  The engine follows bml_hip_set_row_engine; square is
- * 1 for a product X*X (x2, the SP2 iterations), the only one the window
- * engine takes. */
+ * hint_max_a entries.  The engine follows bml_hip_set_row_engine unless
+ * flags holds BMLH_PICK_EXACT; BMLH_PICK_SQUARE marks a product X*X (x2, the
+ * SP2 iterations), the only one the window engine takes. */
+#define BMLH_PICK_SQUARE 1
+#define BMLH_PICK_EXACT 2
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
-                      int hint_max_a, int square, bmlh_spgemm_shape *out);
+                      int hint_max_a, int flags, bmlh_spgemm_shape *out);
 
 #endif

@@ -130,6 +130,10 @@ typedef struct bmlhip_spgemm_args {
                          ws_bytes of device memory and allocates nothing
                          (exact and unordered engines, no batch, no sym) */
     size_t ws_bytes;
+    int fuse;         /* 1: C <- beta*C + alpha*(A*B), C merged into the
+                         product's epilogue (exact engine, kind 0 or 1, one
+                         system, C distinct from A and B) */
+    double alpha, beta;
 } bmlhip_spgemm_args;
 #define BMLHIP_ROW_EXACT 0
 #define BMLHIP_ROW_UNORDERED 1

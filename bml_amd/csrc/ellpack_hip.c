@@ -402,11 +402,13 @@ void bml_ellpack_hip_row_stats(const bml_matrix_ellpack_hip_t *A, int *max_nnz,
 /* ---- the multiply path ---------------------------------------------------- */
 
 /* C = A*B into C's storage.  kind 0 = x2 (traces, kept-count check), 1 = AB
- * (touched-count check), 2 = AB with the kept-count check (adjust). */
+ * (touched-count check), 2 = AB with the kept-count check (adjust).  With
+ * ab (alpha, beta): C <- beta*C + alpha*A*B on the exact engine, C merged in
+ * the product's epilogue. */
 static int run_product_ws(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
                           bml_matrix_ellpack_hip_t *C, double threshold, int kind,
                           struct bml_hip_aux *hint_owner, void *ws, size_t ws_bytes,
-                          const char *what)
+                          const double *ab, const char *what)
 {
     bmlhip_spgemm_args a;
     memset(&a, 0, sizeof a);
@@ -430,7 +432,8 @@ static int run_product_ws(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t 
     /* on a workspace the window engine (squares only) is not offered */
     bmlh_pick_spgemm(a.prec, A->N, max_nnz_of(A), max_nnz_of(B),
                      hint_owner ? hint_owner->touched_hint : 0,
-                     hint_owner ? hint_owner->hint_max_nnz : 0, A == B && !ws, &sh);
+                     hint_owner ? hint_owner->hint_max_nnz : 0,
+                     (A == B && !ws ? BMLH_PICK_SQUARE : 0) | (ab ? BMLH_PICK_EXACT : 0), &sh);
     a.group = sh.group;
     a.table = sh.table;
     a.bucket = sh.bucket;
@@ -443,9 +446,14 @@ static int run_product_ws(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t 
      * (Hermitian) and x2 forms the upper triangle and mirrors it, as the SP2
      * drivers do for a symmetric H (a test hook for that path) */
     const char *e = getenv("BML_HIP_ASSUME_SYMMETRIC");
-    a.sym = kind == 0 && !ws && e && atoi(e) != 0;
+    a.sym = kind == 0 && !ws && !ab && e && atoi(e) != 0;
     a.d_ws = ws;
     a.ws_bytes = ws_bytes;
+    if (ab) {
+        a.fuse = 1;
+        a.alpha = ab[0];
+        a.beta = ab[1];
+    }
     return bmlh_ok(bmlhip_spgemm(&a), what);
 }
 
@@ -453,7 +461,7 @@ static int run_product(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
                        bml_matrix_ellpack_hip_t *C, double threshold, int kind,
                        struct bml_hip_aux *hint_owner, const char *what)
 {
-    return run_product_ws(A, B, C, threshold, kind, hint_owner, NULL, 0, what);
+    return run_product_ws(A, B, C, threshold, kind, hint_owner, NULL, 0, NULL, what);
 }
 
 void *bml_multiply_x2_ellpack_hip(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *X2,
@@ -502,7 +510,7 @@ void bml_multiply_x2_ellpack_hip_noalloc(bml_matrix_ellpack_hip_t *X,
         bmlh_error(BML_HIP_ERR_DEVICE, "%s: the library is not initialised on this GPU", what);
         return;
     }
-    if (!run_product_ws(X, X, X2, threshold, 0, X->aux, ws, ws_bytes, what))
+    if (!run_product_ws(X, X, X2, threshold, 0, X->aux, ws, ws_bytes, NULL, what))
         return;
     bmlhip_status st;
     if (!bmlh_finish_sync(what, &st))
@@ -619,7 +627,20 @@ void bml_multiply_ellpack_hip(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hi
         free(bml_multiply_x2_ellpack_hip(A, C, threshold));
         return;
     }
-    /* product into a temporary of C's shape, then C <- beta*C + alpha*T */
+    if (C != A && C != B) {
+        /* C <- beta*C + alpha*A*B in one pass: the product's epilogue merges
+         * C's row as bml_add does, no temporary */
+        const double ab[2] = {alpha, beta};
+        if (run_product_ws(A, B, C, threshold, A == B ? 0 : 1, A == B ? A->aux : NULL, NULL, 0,
+                           ab, what))
+            finish_into(what, C);
+        else
+            bmlh_set_stats(C->aux, NULL);
+        C->aux->touched_hint = 0;
+        return;
+    }
+    /* C is A or B: the product into a temporary of C's shape, then
+     * C <- beta*C + alpha*T */
     bml_matrix_ellpack_hip_t *T =
         bml_noinit_matrix_ellpack_hip(A->matrix_precision, A->N, C->M, A->distribution_mode);
     if (!T)

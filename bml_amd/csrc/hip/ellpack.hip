@@ -85,6 +85,9 @@ struct SpParams {
     // batched systems: row i belongs to system row_sys[i]; rows of systems
     // with sys_done set are skipped (both NULL: one system)
     const int *row_sys, *sys_done;
+    // fuse: C <- beta*C + alpha*(A*B), merged in the epilogue
+    int fuse;
+    double alpha, beta;
 };
 
 // Broadcast lane src's value to the caller's lanes.  With one group per
@@ -174,6 +177,114 @@ __device__ __forceinline__ void bucket_scan(const Bucket<BK> &b, int col, unsign
     }
     hit_at = e_hit;
     filled = BK - empties;
+}
+
+// Slot of col in a finished table, or -1 when absent (the probe sequence of
+// the build: an empty slot in a bucket proves the column absent).
+template <int BK, bool BIG>
+__device__ __forceinline__ int find_slot(const int *keys, int col, unsigned nbk, unsigned tsize) {
+    unsigned h = home_bucket<BIG>(col, nbk) * BK;
+    while (true) {
+        const Bucket<BK> kb = load_bucket<BK>(keys, h);
+        unsigned eh, filled;
+        bucket_scan<BK>(kb, col, eh, filled);
+        if (eh < (unsigned) BK) return (int) (h + eh);
+        if (filled < (unsigned) BK) return -1;
+        h += BK;
+        if (h >= tsize) h -= tsize;
+    }
+}
+
+// A product slot whose column C's row already took (the key keeps its sign
+// bit clear, so probes still see the slot filled; columns are below 2^24).
+constexpr int TAKEN = 1 << 30;
+
+// Epilogue of C <- beta*C + alpha*P for one row, P the row's product in the
+// table (count columns, first-touch order in ord).  It is the reference's
+// bml_multiply: the product thresholded as bml_multiply_x2 / _AB store it
+// (bml_multiply_ellpack_typed.c:86-101; the diagonal always kept, :306,527), then
+// bml_add(C, P, beta, alpha) (bml_add_ellpack_typed.c:162-196): C's columns
+// in C's order, each 0 + beta*c (when beta != 0) plus alpha*p where P keeps
+// that column, then P's other kept columns in first-touch order as
+// 0 + alpha*p (when alpha != 0), every sum kept iff |v| > threshold.  C's
+// row is read and compacted in place (an entry is only ever written at or
+// before its own position).  Returns the kept count, or MC + 1 when the
+// product alone overflows C (x2: kept entries, AB: distinct columns); d2 is
+// the product's diagonal.  Cleans the table.
+template <typename T, int GROUP, int BK, bool BIG, typename ORD>
+__device__ int fused_epilogue(const SpParams &p, int i, bool rowact, int *keys, T *vals,
+                              const ORD *ord, unsigned tsize, int count, uint64_t gmask,
+                              T &d2) {
+    using E = Elem<T>;
+    const int lane = lane_id(), gl = lane % GROUP;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const unsigned nbk = tsize / BK;
+    const int MC = p.MC;
+    const size_t rowC = (size_t) (rowact ? i : 0) * MC;
+    T *Cv = (T *) p.Cv;
+    const bool usec = p.beta > 0.0 || p.beta < 0.0;
+    const bool usep = p.alpha > 0.0 || p.alpha < 0.0;
+    auto pkeeps = [&](int col, T pv) { return col == i || E::above(pv, p.thr); };
+    const int nC = rowact && usec ? p.Cn[i] : 0;
+    int kept = 0;
+    for (int r0 = 0; __any(r0 < nC); r0 += GROUP) {
+        const int r = r0 + gl;
+        const bool act = r < nC;
+        int col = 0;
+        T v = E::zero();
+        if (act) {
+            col = p.Ci[rowC + r];
+            v = E::axpy(E::zero(), p.beta, Cv[rowC + r]);
+            const int s = find_slot<BK, BIG>(keys, col, nbk, tsize);
+            if (s >= 0) {
+                const T pv = vals[s];
+                if (usep && pkeeps(col, pv)) {
+                    v = E::axpy(v, p.alpha, pv);
+                    keys[s] = col | TAKEN;
+                }
+            }
+        }
+        const bool keep = act && E::above(v, p.thr);
+        const uint64_t km = ballot64(keep) & gmask;
+        const int pos = kept + popc64(km & ltmask);
+        if (keep && pos < MC) {
+            p.Ci[rowC + pos] = col;
+            Cv[rowC + pos] = v;
+        }
+        kept += popc64(km);
+    }
+    int keptp = 0;
+    for (int r0 = 0; __any(rowact && r0 < count); r0 += GROUP) {
+        const int r = r0 + gl;
+        const bool act = rowact && r < count;
+        int s = 0, key = 0;
+        T pv = E::zero();
+        if (act) {
+            s = (int) ord[r];
+            key = keys[s];
+            pv = vals[s];
+        }
+        const int col = key & ~TAKEN;
+        const bool pk = act && pkeeps(col, pv);
+        keptp += popc64(ballot64(pk) & gmask);
+        const uint64_t dm = ballot64(act && col == i) & gmask;
+        if (dm) d2 = shfl_any(pv, __ffsll((unsigned long long) dm) - 1);
+        const T v = E::axpy(E::zero(), p.alpha, pv);
+        const bool keep = usep && pk && !(key & TAKEN) && E::above(v, p.thr);
+        const uint64_t km = ballot64(keep) & gmask;
+        const int pos = kept + popc64(km & ltmask);
+        if (keep && pos < MC) {
+            p.Ci[rowC + pos] = col;
+            Cv[rowC + pos] = v;
+        }
+        if (act) {
+            keys[s] = EMPTY;
+            vals[s] = E::zero();
+        }
+        kept += popc64(km);
+    }
+    const bool povf = p.kind == 0 ? keptp > MC : count > MC;
+    return povf && kept <= MC ? MC + 1 : kept;
 }
 
 // One row per group.  keys/vals/ord point at the group's table (LDS or
@@ -360,7 +471,10 @@ __device__ void spgemm_row(const SpParams &p, int i, int *keys, T *vals, ORD *or
     T *Cv = (T *) p.Cv;
     int kept = 0;
     T d2 = E::zero();
-    for (int r0 = 0; __any(rowact && r0 < count); r0 += GROUP) {
+    if (p.fuse)
+        kept = fused_epilogue<T, GROUP, BK, BIG, ORD>(p, i, rowact, keys, vals, ord, tsize, count,
+                                                      gmask, d2);
+    else for (int r0 = 0; __any(rowact && r0 < count); r0 += GROUP) {
         const int r = r0 + gl;
         const bool act = rowact && r < count;
         int s = 0, col = 0;
@@ -609,7 +723,10 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_mw(SpParams p, unsigned tsize
     T *Cv = (T *) p.Cv;
     int kept = 0;
     T d2 = E::zero();
-    if (!tovf) {
+    if (!tovf && p.fuse) {
+        kept = fused_epilogue<T, 64, BK, false, uint16_t>(p, i, true, keys, vals, ord, tsize, count,
+                                                          ~0ull, d2);
+    } else if (!tovf) {
         for (int r0 = 0; r0 < count; r0 += 64) {
             const int r = r0 + lane;
             const bool act = r < count;
@@ -2678,7 +2795,7 @@ int spgemm_win_typed(const bmlhip_spgemm_args *a) {
         rc = launch_win_rows<T, false, false>(wp, rows, a->wbits, a->waves, s);
     }
     if (rc) return rc;
-    SpParams p;
+    SpParams p{};
     memset(&p, 0, sizeof p);
     p.N = a->N;
     p.MC = a->MC;
@@ -2754,7 +2871,7 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
     // the profiling window spans the product and the mirror
     if (N > 0) bmlhip_prof_mark_begin(PrecOf<T>::value);
     if ((rc = launch_win_rows<T, true, true>(wp, rows, wbits, waves, s))) return rc;
-    SpParams p;
+    SpParams p{};
     memset(&p, 0, sizeof p);
     p.N = N;
     p.MC = M;
@@ -2781,7 +2898,10 @@ int sp2_window_typed(int N, int M, void *Xv, int *Xi, int *Xn, double thr, int w
 
 template <typename T>
 int spgemm_typed(const bmlhip_spgemm_args *a) {
-    if (!a->d_ws && a->engine == BMLHIP_ROW_WINDOW && a->kind == 0 && a->d_Av == a->d_Bv &&
+    if (a->fuse && (a->engine != BMLHIP_ROW_EXACT || a->batch || a->sym || a->d_ws ||
+                    a->d_Cv == a->d_Av || a->d_Cv == a->d_Bv || a->kind == 2))
+        return fail_msg(2, "fused multiply: exact engine, one system, C distinct from A and B");
+    if (!a->d_ws && !a->fuse && a->engine == BMLHIP_ROW_WINDOW && a->kind == 0 && a->d_Av == a->d_Bv &&
         a->d_Ai == a->d_Bi && a->d_An == a->d_Bn && a->MA == a->MB)
         return spgemm_win_typed<T>(a);
     hipStream_t s = (hipStream_t) bmlhip_stream();
@@ -2794,7 +2914,7 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     // a row's product count is kept in 32 bits
     if ((size_t) a->MA * (size_t) a->MB >= (1ull << 32))
         return fail_msg(5, "row product: MA*MB = %d*%d exceeds 2^32", a->MA, a->MB);
-    SpParams p;
+    SpParams p{};
     p.N = a->N;
     p.MA = a->MA;
     p.MB = a->MB;
@@ -2816,6 +2936,9 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     p.sys_done = a->batch ? a->batch->d_done : nullptr;
     p.remap = a->remap;
     p.sym = 0;
+    p.fuse = a->fuse;
+    p.alpha = a->alpha;
+    p.beta = a->beta;
     // global-table pass: one table per block, >= 2x the row bound
     long long bound = (long long) a->MA * a->MB;
     if (bound > a->N) bound = a->N;
@@ -2917,9 +3040,10 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
         if ((rc = report("k_sym_mirror"))) return rc;
     }
     // kind 1 (AB) checks the distinct count against MC, as the reference does
+    // (the fused epilogue folds that check into the kept count it records)
     hipLaunchKernelGGL(k_reduce_rowstat, dim3(rowstat_blocks(a->N)), dim3(256), 0, s,
                        (const int4 *) p.rowstat, a->N, a->MC,
-                       (a->kind == 1 ? BMLHIP_RS_OVF_DIST : BMLHIP_RS_OVF_KEPT) |
+                       (a->kind == 1 && !a->fuse ? BMLHIP_RS_OVF_DIST : BMLHIP_RS_OVF_KEPT) |
                            BMLHIP_RS_STORED | BMLHIP_RS_PROF,
                        (int) PrecOf<T>::value, p.skip, p.st);
     if ((rc = report("k_reduce_rowstat"))) return rc;
