@@ -26,7 +26,10 @@
 #include "bml_hip_tune.h"
 #include "bml_hip_host.h"
 
+#include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
                       int hint_max_a, int flags, bmlh_spgemm_shape *out)
@@ -203,4 +206,257 @@ void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hi
     /* contiguous rows per XCD cut the HBM fetch of the lattice X^2 5x but
      * ran 1.3x slower than the hardware's round-robin order */
     out->remap = (e = getenv("BML_HIP_XCD_REMAP")) ? atoi(e) : 0;
+}
+
+/* ---- measured shapes ------------------------------------------------------
+ *
+ * With autotuning on, the first product of each kind of operand (a key of
+ * precision, flags, the analytic engine and the sizes, the sizes in steps of
+ * 2^(1/4)) times candidate shapes around the analytic one on the operands
+ * themselves and keeps the fastest, in memory and, when a cache file is
+ * named, on disk (one line per key, read back at the next start).  The
+ * search is coordinate-wise: for each parameter in turn (exact engine:
+ * bucket, lanes per row, table size, wavefronts; unordered: wavefronts,
+ * bucket, table; window: window width, rows per block) every value is
+ * timed with the others at the best so far.  Each candidate runs twice and
+ * the faster time counts; a candidate that fails to launch is skipped.
+ * Shapes set through the environment (BML_HIP_GROUP and the like) are left
+ * alone. */
+typedef struct {
+    char key[128];
+    bmlh_spgemm_shape sh;
+    double ms;
+} tune_entry;
+
+static int g_tune = -1; /* -1: not read from the environment yet */
+static char g_tune_path[1024];
+static tune_entry *g_tab;
+static int g_ntab, g_captab, g_loaded;
+
+static const char *const g_override_env[] = {
+    "BML_HIP_GROUP", "BML_HIP_TABLE", "BML_HIP_BUCKET", "BML_HIP_WAVES", "BML_HIP_WIN_ROWS",
+    "BML_HIP_WINDOW_BITS", "BML_HIP_LDS_BUDGET", "BML_HIP_XCD_REMAP"};
+
+static void tune_env(void)
+{
+    if (g_tune >= 0)
+        return;
+    const char *e = getenv("BML_HIP_AUTOTUNE");
+    g_tune = e && atoi(e) != 0;
+    const char *p = getenv("BML_HIP_TUNE_CACHE");
+    if (p)
+        snprintf(g_tune_path, sizeof g_tune_path, "%s", p);
+}
+
+static int tune_add(const char *key, const bmlh_spgemm_shape *sh, double ms)
+{
+    for (int k = 0; k < g_ntab; k++)
+        if (!strcmp(g_tab[k].key, key)) {
+            g_tab[k].sh = *sh;
+            g_tab[k].ms = ms;
+            return 1;
+        }
+    if (g_ntab == g_captab) {
+        const int cap = g_captab ? 2 * g_captab : 64;
+        tune_entry *t = realloc(g_tab, sizeof(tune_entry) * (size_t) cap);
+        if (!t)
+            return 0;
+        g_tab = t;
+        g_captab = cap;
+    }
+    tune_entry *t = &g_tab[g_ntab++];
+    snprintf(t->key, sizeof t->key, "%s", key);
+    t->sh = *sh;
+    t->ms = ms;
+    return 1;
+}
+
+static void tune_load(void)
+{
+    if (g_loaded || !g_tune_path[0])
+        return;
+    g_loaded = 1;
+    FILE *f = fopen(g_tune_path, "r");
+    if (!f)
+        return;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        char key[128];
+        bmlh_spgemm_shape sh;
+        double ms;
+        if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %lf", key, &sh.group, &sh.table,
+                   &sh.bucket, &sh.remap, &sh.waves, &sh.engine, &sh.wbits, &sh.wrows, &ms) == 10)
+            tune_add(key, &sh, ms);
+    }
+    fclose(f);
+}
+
+static void tune_save(const char *key, const bmlh_spgemm_shape *sh, double ms)
+{
+    if (!g_tune_path[0])
+        return;
+    FILE *f = fopen(g_tune_path, "a");
+    if (!f)
+        return;
+    fprintf(f, "%s %d %d %d %d %d %d %d %d %.6f\n", key, sh->group, sh->table, sh->bucket,
+            sh->remap, sh->waves, sh->engine, sh->wbits, sh->wrows, ms);
+    fclose(f);
+}
+
+void bml_hip_set_autotune(int on, const char *cache_path)
+{
+    tune_env();
+    g_tune = on != 0;
+    if (cache_path) {
+        snprintf(g_tune_path, sizeof g_tune_path, "%s", cache_path);
+        g_loaded = 0;
+    }
+}
+
+int bml_hip_get_autotune(void)
+{
+    tune_env();
+    return g_tune;
+}
+
+int bml_hip_autotune_entries(void)
+{
+    tune_env();
+    tune_load();
+    return g_ntab;
+}
+
+void bml_hip_autotune_clear(void)
+{
+    free(g_tab);
+    g_tab = NULL;
+    g_ntab = g_captab = 0;
+    g_loaded = 1; /* the file is not read again until a path is set */
+}
+
+static int qbucket(long long x)
+{
+    return x <= 1 ? 0 : (int) ceil(4.0 * log2((double) x));
+}
+
+/* time one shape: the faster of two runs, < 0 if it does not launch */
+static double tune_time(const bmlh_spgemm_shape *sh, bmlh_measure_fn fn, void *ctx)
+{
+    double best = -1.0;
+    for (int r = 0; r < 2; r++) {
+        double ms = 0.0;
+        if (fn(sh, ctx, &ms))
+            return -1.0;
+        if (best < 0.0 || ms < best)
+            best = ms;
+    }
+    return best;
+}
+
+static void try_shape(const bmlh_spgemm_shape *c, bmlh_spgemm_shape *best, double *best_ms,
+                      bmlh_measure_fn fn, void *ctx)
+{
+    const double ms = tune_time(c, fn, ctx);
+    if (ms >= 0.0 && (*best_ms < 0.0 || ms < *best_ms)) {
+        *best = *c;
+        *best_ms = ms;
+    }
+}
+
+void bmlh_tuned_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
+                       int hint_max_a, int flags, bmlh_spgemm_shape *out, bmlh_measure_fn fn,
+                       void *ctx)
+{
+    bmlh_pick_spgemm(precision, N, max_a, max_b, touched_hint, hint_max_a, flags, out);
+    tune_env();
+    if (!g_tune || !fn || N <= 0)
+        return;
+    for (size_t k = 0; k < sizeof g_override_env / sizeof g_override_env[0]; k++)
+        if (getenv(g_override_env[k]))
+            return;
+    tune_load();
+    char key[128];
+    snprintf(key, sizeof key, "cu%d:p%d:f%d:e%d:n%d:a%d:b%d:t%d", bmlhip_device_cus(), precision,
+             flags, out->engine, qbucket(N), qbucket(max_a), qbucket(max_b),
+             qbucket(touched_hint));
+    for (int k = 0; k < g_ntab; k++)
+        if (!strcmp(g_tab[k].key, key)) {
+            *out = g_tab[k].sh;
+            return;
+        }
+    bmlh_spgemm_shape best = *out;
+    double best_ms = tune_time(out, fn, ctx);
+    if (best_ms < 0.0)
+        return; /* the analytic shape itself fails: leave it to report */
+    bmlh_spgemm_shape c;
+    if (out->engine == BMLHIP_ROW_WINDOW) {
+        static const int wb[] = {10, 11};
+        static const int wr[] = {1, 2, 4};
+        for (int k = 0; k < 2; k++) {
+            c = best;
+            c.wbits = wb[k];
+            if (c.wbits != best.wbits)
+                try_shape(&c, &best, &best_ms, fn, ctx);
+        }
+        for (int k = 0; k < 3; k++) {
+            c = best;
+            c.wrows = wr[k];
+            if (c.wrows != best.wrows)
+                try_shape(&c, &best, &best_ms, fn, ctx);
+        }
+    } else if (out->engine == BMLHIP_ROW_UNORDERED) {
+        static const int wv[] = {4, 8};
+        for (int k = 0; k < 2; k++) {
+            c = best;
+            c.waves = wv[k];
+            if (c.waves != best.waves)
+                try_shape(&c, &best, &best_ms, fn, ctx);
+        }
+        c = best;
+        c.bucket = best.bucket == 2 ? 4 : 2;
+        try_shape(&c, &best, &best_ms, fn, ctx);
+        const int t0 = best.table;
+        for (int k = 0; k < 2; k++) {
+            c = best;
+            c.table = (k ? t0 * 3 / 2 : t0 * 3 / 4) / 64 * 64;
+            if (c.table >= 256)
+                try_shape(&c, &best, &best_ms, fn, ctx);
+        }
+    } else {
+        static const int bk[] = {2, 4, 8};
+        for (int k = 0; k < 3; k++) {
+            c = best;
+            c.bucket = bk[k];
+            if (c.bucket != best.bucket)
+                try_shape(&c, &best, &best_ms, fn, ctx);
+        }
+        const int g0 = best.group;
+        for (int g = g0 / 2; g <= 2 * g0 && g <= 64; g *= 2) {
+            if (g < 8 || g == g0)
+                continue;
+            c = best;
+            c.group = g;
+            c.waves = 1;
+            try_shape(&c, &best, &best_ms, fn, ctx);
+        }
+        const int t0 = best.table;
+        for (int k = 0; k < 2; k++) {
+            c = best;
+            c.table = (k ? t0 * 3 / 2 : t0 * 3 / 4) / 64 * 64;
+            if (c.table >= 64)
+                try_shape(&c, &best, &best_ms, fn, ctx);
+        }
+        if (best.group == 64) {
+            static const int wv[] = {1, 2, 3, 4};
+            for (int k = 0; k < 4; k++) {
+                c = best;
+                c.waves = wv[k];
+                if (c.waves != best.waves)
+                    try_shape(&c, &best, &best_ms, fn, ctx);
+            }
+        }
+    }
+    *out = best;
+    if (tune_add(key, &best, best_ms))
+        tune_save(key, &best, best_ms);
 }

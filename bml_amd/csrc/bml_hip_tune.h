@@ -26,4 +26,16 @@ typedef struct bmlh_spgemm_shape {
 void bmlh_pick_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
                       int hint_max_a, int flags, bmlh_spgemm_shape *out);
 
+/* Runs the product with shape sh on the caller's operands (its result
+ * overwrites the output, so the caller runs the chosen shape last) and
+ * gives its duration in ms; nonzero if the shape does not launch. */
+typedef int (*bmlh_measure_fn)(const bmlh_spgemm_shape *sh, void *ctx, double *ms);
+
+/* bmlh_pick_spgemm, then, with autotuning on (bml_hip_set_autotune or
+ * BML_HIP_AUTOTUNE=1) and no shape set through the environment, the
+ * measured shape for this key: from the cache, or searched with fn. */
+void bmlh_tuned_spgemm(int precision, int N, int max_a, int max_b, int touched_hint,
+                       int hint_max_a, int flags, bmlh_spgemm_shape *out, bmlh_measure_fn fn,
+                       void *ctx);
+
 #endif

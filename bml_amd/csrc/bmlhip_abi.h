@@ -50,6 +50,11 @@ typedef struct bmlhip_status {
 
 /* ---- runtime (hip/runtime.hip) ----------------------------------------- */
 int bmlhip_init(int device);
+/* compute units of the current GPU (0 before init) */
+int bmlhip_device_cus(void);
+/* a timed region on the library stream: end waits for it and gives ms */
+int bmlhip_time_begin(void);
+int bmlhip_time_end(double *ms);
 void bmlhip_shutdown(void);
 const char *bmlhip_errmsg(void);
 /* The fixed workspace of the entry points that allocate nothing: a
@@ -100,6 +105,13 @@ typedef struct bmlhip_batch {
     double *d_trx, *d_trx2, *d_norm2; /* [nsys] tr X, tr X^2, ||X - X^2||_F^2 */
     double *d_rowtr;       /* [2 N] per-row trace partials */
     double *d_rownorm;     /* [N] per-row norm partials */
+    /* work queue (all three set, or NULL: one block per row, rows of
+     * finished systems return at once): the rows of the unfinished systems
+     * in order, rebuilt on the device before each product; the row kernels
+     * then run persistent, taking rows from it with an atomic cursor */
+    int *d_list;           /* [N] */
+    int *d_qinfo;          /* [4]: rows listed, product cursor, add cursor, systems left */
+    int *d_act_off;        /* [nsys + 1] first list position of each system */
 } bmlhip_batch;
 
 typedef struct bmlhip_spgemm_args {
@@ -156,6 +168,9 @@ typedef struct bmlhip_spgemm_args {
 /* window engine limits: at most 64 windows per row range */
 #define BMLHIP_WIN_MAX_WINDOWS 64
 int bmlhip_spgemm(const bmlhip_spgemm_args *a);
+/* rebuild b's work queue (d_list, d_qinfo) from the done flags: queued
+ * before each batched iteration's product (nothing when b has no queue) */
+int bmlhip_batch_queue(const bmlhip_batch *b, int N);
 
 /* A <- alpha*A + beta*B (mode 0), or that plus the norm over the union of
  * (A - B): mode 1 Re((A-B)^2) as bml_add_norm, mode 2 |A - B|^2; the norm

@@ -405,6 +405,30 @@ void bml_ellpack_hip_row_stats(const bml_matrix_ellpack_hip_t *A, int *max_nnz,
  * (touched-count check), 2 = AB with the kept-count check (adjust).  With
  * ab (alpha, beta): C <- beta*C + alpha*A*B on the exact engine, C merged in
  * the product's epilogue. */
+static void apply_shape(bmlhip_spgemm_args *a, const bmlh_spgemm_shape *sh)
+{
+    a->group = sh->group;
+    a->table = sh->table;
+    a->bucket = sh->bucket;
+    a->remap = sh->remap;
+    a->waves = sh->waves;
+    a->engine = sh->engine;
+    a->wbits = sh->wbits;
+    a->wrows = sh->wrows;
+}
+
+/* the autotuner's measurement: the product with shape sh, timed */
+static int measure_product(const bmlh_spgemm_shape *sh, void *ctx, double *ms)
+{
+    bmlhip_spgemm_args a = *(const bmlhip_spgemm_args *) ctx;
+    apply_shape(&a, sh);
+    if (bmlhip_time_begin())
+        return 1;
+    const int rc = bmlhip_spgemm(&a);
+    const int rt = bmlhip_time_end(ms);
+    return rc || rt;
+}
+
 static int run_product_ws(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t *B,
                           bml_matrix_ellpack_hip_t *C, double threshold, int kind,
                           struct bml_hip_aux *hint_owner, void *ws, size_t ws_bytes,
@@ -428,20 +452,6 @@ static int run_product_ws(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t 
     a.d_Ci = C->index;
     a.d_Cn = C->nnz;
     a.threshold = threshold;
-    bmlh_spgemm_shape sh;
-    /* on a workspace the window engine (squares only) is not offered */
-    bmlh_pick_spgemm(a.prec, A->N, max_nnz_of(A), max_nnz_of(B),
-                     hint_owner ? hint_owner->touched_hint : 0,
-                     hint_owner ? hint_owner->hint_max_nnz : 0,
-                     (A == B && !ws ? BMLH_PICK_SQUARE : 0) | (ab ? BMLH_PICK_EXACT : 0), &sh);
-    a.group = sh.group;
-    a.table = sh.table;
-    a.bucket = sh.bucket;
-    a.remap = sh.remap;
-    a.waves = sh.waves;
-    a.engine = sh.engine;
-    a.wbits = sh.wbits;
-    a.wrows = sh.wrows;
     /* BML_HIP_ASSUME_SYMMETRIC=1: the caller asserts X is symmetric
      * (Hermitian) and x2 forms the upper triangle and mirrors it, as the SP2
      * drivers do for a symmetric H (a test hook for that path) */
@@ -454,6 +464,17 @@ static int run_product_ws(bml_matrix_ellpack_hip_t *A, bml_matrix_ellpack_hip_t 
         a.alpha = ab[0];
         a.beta = ab[1];
     }
+    bmlh_spgemm_shape sh;
+    /* on a workspace the window engine (squares only) is not offered; the
+     * autotuner measures x2 and AB, whose runs overwrite C (not the fused
+     * multiply, which reads C, nor the workspace and adjust forms) */
+    const int tune = !ab && !ws && kind != 2;
+    bmlh_tuned_spgemm(a.prec, A->N, max_nnz_of(A), max_nnz_of(B),
+                      hint_owner ? hint_owner->touched_hint : 0,
+                      hint_owner ? hint_owner->hint_max_nnz : 0,
+                      (A == B && !ws ? BMLH_PICK_SQUARE : 0) | (ab ? BMLH_PICK_EXACT : 0), &sh,
+                      tune ? measure_product : NULL, &a);
+    apply_shape(&a, &sh);
     return bmlh_ok(bmlhip_spgemm(&a), what);
 }
 

@@ -88,6 +88,10 @@ struct SpParams {
     // fuse: C <- beta*C + alpha*(A*B), merged in the epilogue
     int fuse;
     double alpha, beta;
+    // batched work queue (NULL: one block per row): the rows to form are
+    // qlist[0 .. qinfo[0]), taken with the cursor qinfo[1]
+    const int *qlist;
+    int *qinfo;
 };
 
 // Broadcast lane src's value to the caller's lanes.  With one group per
@@ -881,38 +885,14 @@ template <> struct UAcc<cf64> {
     __device__ static cf64 from(cf64 v) { return v; }
 };
 
-// SYM (X symmetric or Hermitian, SP2 on such an H): row i forms only the
-// columns k >= i of X^2 (the other lanes of a chunk sit idle: no probe, no
-// add), the table holds about half the row's columns, and k_sym_mirror
-// copies each kept (i, k), k > i, to (k, i) conjugated.
+// One row i of the unordered engine on the block's table (keys, vals:
+// cleared, res_s and ovf_s zero); ends with wavefront 0's epilogue, after
+// a barrier every wavefront reaches.
 template <typename T, int W, int BK, bool SYM>
-__global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned tsize) {
+__device__ void unord_row(const SpParams &p, unsigned tsize, int i, int *keys,
+                          typename UAcc<T>::type *vals, int &res_s, int &ovf_s,
+                          long long (&prods_s)[W], T (&trx_s)[W]) {
     using E = Elem<T>;
-    using A = typename UAcc<T>::type;
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int res_s, ovf_s;
-    __shared__ long long prods_s[W];
-    __shared__ T trx_s[W];
-    if (p.skip && *p.skip) return;
-    const unsigned stride = tsize + table_pad<64>();
-    A *vals = (A *) smem;
-    int *keys = (int *) (smem + (size_t) stride * sizeof(A));
-    for (unsigned s = threadIdx.x; s < stride; s += 64 * W) {
-        keys[s] = EMPTY;
-        vals[s] = Elem<A>::zero();
-    }
-    if (threadIdx.x == 0) {
-        res_s = 0;
-        ovf_s = 0;
-    }
-    __syncthreads();
-    const int row = p.remap ? xcd_block(blockIdx.x, gridDim.x) : (int) blockIdx.x;
-    if (row >= p.N) return;
-    if (p.row_sys && p.sys_done[p.row_sys[row]]) {  // uniform
-        if (threadIdx.x == 0) p.rowstat[row] = make_int4(-1, 0, 0, 0);
-        return;
-    }
-    const int i = row;
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const uint64_t ltmask = (1ull << lane) - 1ull;
     const T *Av = (const T *) p.Av;
@@ -1059,6 +1039,59 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
             p.rowtr[i] = E::re(tx);
             p.rowtr[p.N + i] = E::re(d2);
         }
+    }
+}
+
+// SYM (X symmetric or Hermitian, SP2 on such an H): row i forms only the
+// columns k >= i of X^2 (the other lanes of a chunk sit idle: no probe, no
+// add), the table holds about half the row's columns, and k_sym_mirror
+// copies each kept (i, k), k > i, to (k, i) conjugated.
+template <typename T, int W, int BK, bool SYM>
+__global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned tsize) {
+    using A = typename UAcc<T>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int res_s, ovf_s, item_s;
+    __shared__ long long prods_s[W];
+    __shared__ T trx_s[W];
+    if (p.skip && *p.skip) return;
+    const unsigned stride = tsize + table_pad<64>();
+    A *vals = (A *) smem;
+    int *keys = (int *) (smem + (size_t) stride * sizeof(A));
+    auto clear = [&]() {
+        for (unsigned s = threadIdx.x; s < stride; s += 64 * W) {
+            keys[s] = EMPTY;
+            vals[s] = Elem<A>::zero();
+        }
+        if (threadIdx.x == 0) {
+            res_s = 0;
+            ovf_s = 0;
+        }
+    };
+    clear();
+    __syncthreads();
+    if (!p.qlist) {
+        const int row = p.remap ? xcd_block(blockIdx.x, gridDim.x) : (int) blockIdx.x;
+        if (row >= p.N) return;
+        if (p.row_sys && p.sys_done[p.row_sys[row]]) {  // uniform
+            if (threadIdx.x == 0) p.rowstat[row] = make_int4(-1, 0, 0, 0);
+            return;
+        }
+        unord_row<T, W, BK, SYM>(p, tsize, row, keys, vals, res_s, ovf_s, prods_s, trx_s);
+        return;
+    }
+    // persistent over the work queue, one listed row per take; the cursor
+    // is read by every thread between two barriers, and written again only
+    // after the next two
+    const int n = p.qinfo[0];
+    while (true) {
+        if (threadIdx.x == 0) item_s = atomicAdd(&p.qinfo[1], 1);
+        __syncthreads();
+        const int q = item_s;
+        if (q >= n) return;
+        unord_row<T, W, BK, SYM>(p, tsize, p.qlist[q], keys, vals, res_s, ovf_s, prods_s, trx_s);
+        __syncthreads();  // the epilogue's table reads are done
+        clear();
+        __syncthreads();
     }
 }
 
@@ -1916,13 +1949,30 @@ __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
         vals[s] = Elem<T>::zero();
     }
     __syncthreads();
+    const int g = threadIdx.x / GROUP;
+    const int cap = (int) tsize - 8;
+    if (p.qlist) {
+        // persistent over the work queue, R listed rows per take (each
+        // row's epilogue leaves its table clean)
+        const int n = p.qinfo[0];
+        while (true) {
+            int q = 0;
+            if (threadIdx.x == 0) q = atomicAdd(&p.qinfo[1], 1);
+            q = __shfl(q, 0, 64);
+            if ((long long) q * R >= n) break;
+            const int v = q * R + g;
+            spgemm_row<T, GROUP, BK, false, uint16_t>(p, v < n ? p.qlist[v] : -1, keys + g * stride,
+                                                      vals + (size_t) g * stride, ord + g * stride,
+                                                      tsize, cap);
+            __syncthreads();
+        }
+        return;
+    }
     // with p.remap each XCD takes a contiguous range of rows
     const int vb = p.remap ? xcd_block(blockIdx.x, gridDim.x) : (int) blockIdx.x;
-    const int g = threadIdx.x / GROUP;
     const int row = vb * R + g;
     const int i = (row < p.N && !(p.row_sys && p.sys_done[p.row_sys[row]])) ? row : -1;
     if (row < p.N && i < 0 && threadIdx.x % GROUP == 0) p.rowstat[row] = make_int4(-1, 0, 0, 0);
-    const int cap = (int) tsize - 8;
     spgemm_row<T, GROUP, BK, false, uint16_t>(p, i, keys + g * stride, vals + (size_t) g * stride,
                                               ord + g * stride, tsize, cap);
 }
@@ -2083,6 +2133,62 @@ __global__ __launch_bounds__(1024) void k_minmax(const double *x, const double *
 
 // Per-system sums of x over row ranges [off[s], off[s+1]) (fixed partition
 // and tree: deterministic); systems with done[s] set are left alone.
+// Work queue of a batched iteration.  k_batch_scan (one block): act_off[s]
+// = rows of the unfinished systems before s, qinfo = (rows listed, 0, 0,
+// systems left).  k_batch_fill (one block per system): an unfinished
+// system's rows go to list[act_off[s] ..] in order; a finished system's
+// row records are marked skipped (kept = -1), as the row kernels would
+// have marked them.
+__global__ __launch_bounds__(1024) void k_batch_scan(int nsys, const int *row_off, const int *done,
+                                                     int *act_off, int *qinfo) {
+    __shared__ int sums[1024];
+    __shared__ int carry_s, left_s;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        carry_s = 0;
+        left_s = 0;
+    }
+    __syncthreads();
+    for (int base = 0; base < nsys; base += 1024) {
+        const int s = base + t;
+        const bool act = s < nsys && !done[s];
+        const int n = act ? row_off[s + 1] - row_off[s] : 0;
+        sums[t] = n;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int v = t >= off ? sums[t - off] : 0;
+            __syncthreads();
+            sums[t] += v;
+            __syncthreads();
+        }
+        const int carry = carry_s;
+        if (s < nsys) act_off[s] = carry + sums[t] - n;
+        if (act) atomicAdd(&left_s, 1);
+        __syncthreads();
+        if (t == 1023) carry_s = carry + sums[1023];
+        __syncthreads();
+    }
+    if (t == 0) {
+        act_off[nsys] = carry_s;
+        qinfo[0] = carry_s;
+        qinfo[1] = 0;
+        qinfo[2] = 0;
+        qinfo[3] = left_s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_fill(const int *row_off, const int *done,
+                                                    const int *act_off, int *list, int4 *rowstat) {
+    const int s = blockIdx.x;
+    const int r0 = row_off[s], n = row_off[s + 1] - r0;
+    if (done[s]) {
+        for (int k = threadIdx.x; k < n; k += 256) rowstat[r0 + k] = make_int4(-1, 0, 0, 0);
+    } else {
+        const int o = act_off[s];
+        for (int k = threadIdx.x; k < n; k += 256) list[o + k] = r0 + k;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_seg_sum(const double *x, const int *off, const int *done,
                                                  double *out) {
     __shared__ double part[256];
@@ -2149,6 +2255,10 @@ struct AddParams {
     const int *row_sys, *sys_done;
     const double *ab_sys;
     int4 *rowstat;  // [N]: (kept, 0, 0, 0), kept = -1 for a skipped row
+    // batched work queue (NULL: one block per row): rows qlist[0 ..
+    // qinfo[0]), taken with the cursor qinfo[2]
+    const int *qlist;
+    int *qinfo;
 };
 
 // tsize: any table size; a column's home slot is its multiplicative hash
@@ -2255,6 +2365,19 @@ __global__ __launch_bounds__(64) void k_add_lds(AddParams p, unsigned tsize) {
         ys[s] = Elem<T>::zero();
     }
     __syncthreads();
+    if (p.qlist) {
+        // persistent over the work queue (each row leaves the table clean)
+        const int n = p.qinfo[0];
+        while (true) {
+            int q = 0;
+            if (threadIdx.x == 0) q = atomicAdd(&p.qinfo[2], 1);
+            q = __shfl(q, 0, 64);
+            if (q >= n) break;
+            add_row<T, uint16_t>(p, p.qlist[q], keys, xs, ys, ord, tsize);
+            __syncthreads();
+        }
+        return;
+    }
     const int i = blockIdx.x;
     if (p.row_sys && p.sys_done[p.row_sys[i]]) {
         if (threadIdx.x == 0) p.rowstat[i] = make_int4(-1, 0, 0, 0);
@@ -2510,6 +2633,45 @@ int report(const char *what) {
     return fail_msg(3, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Grid of a persistent kernel: as many blocks as the GPU holds at once
+// (blocks per CU for this kernel, block size and LDS, times the CUs), at
+// most `items`.
+int persistent_blocks(const void *fn, int threads, size_t lds, long long items) {
+    static int cus[64];
+    int dev = 0;
+    (void) hipGetDevice(&dev);
+    const int d = dev >= 0 && dev < 64 ? dev : 0;
+    if (!cus[d]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        cus[d] = n;
+    }
+    // the occupancy of a (kernel, block, LDS) triple is looked up once, so a
+    // graph capture (whose eager pass came first) makes no query
+    struct Occ {
+        const void *fn;
+        int threads;
+        size_t lds;
+        int per;
+    };
+    static Occ occ[64];
+    static int nocc = 0;
+    int per = 0;
+    for (int k = 0; k < nocc && !per; k++)
+        if (occ[k].fn == fn && occ[k].threads == threads && occ[k].lds == lds) per = occ[k].per;
+    if (!per) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess ||
+            per < 1)
+            per = 1;
+        if (nocc < 64) occ[nocc++] = {fn, threads, lds, per};
+    }
+    long long b = (long long) per * cus[d];
+    if (b > items) b = items;
+    return b < 1 ? 1 : (int) b;
+}
+
 template <typename T, int GROUP, int BK>
 int launch_spgemm_lds(const SpParams &p, unsigned tsize, hipStream_t s) {
     constexpr int R = 64 / GROUP;
@@ -2520,7 +2682,9 @@ int launch_spgemm_lds(const SpParams &p, unsigned tsize, hipStream_t s) {
     if (lds > 64 * 1024)
         (void) hipFuncSetAttribute((const void *) k_spgemm_lds<T, GROUP, BK>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-    const int blocks = (p.N + R - 1) / R;
+    const int blocks = !p.qlist || p.N == 0 ? (p.N + R - 1) / R
+                       : persistent_blocks((const void *) k_spgemm_lds<T, GROUP, BK>, 64, lds,
+                                           (p.N + R - 1) / R);
     if (blocks > 0) {
         bmlhip_prof_mark_begin(PrecOf<T>::value);
         hipLaunchKernelGGL((k_spgemm_lds<T, GROUP, BK>), dim3(blocks), dim3(64), lds, s, p, tsize);
@@ -2559,8 +2723,12 @@ int launch_spgemm_unord(const SpParams &p, unsigned tsize, hipStream_t s) {
         (void) hipFuncSetAttribute((const void *) k_spgemm_unord<T, W, BK, SYM>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
     if (p.N > 0) {
+        const int blocks = !p.qlist ? p.N
+                           : persistent_blocks((const void *) k_spgemm_unord<T, W, BK, SYM>,
+                                               64 * W, lds, p.N);
         if (!SYM) bmlhip_prof_mark_begin(PrecOf<T>::value);
-        hipLaunchKernelGGL((k_spgemm_unord<T, W, BK, SYM>), dim3(p.N), dim3(64 * W), lds, s, p, tsize);
+        hipLaunchKernelGGL((k_spgemm_unord<T, W, BK, SYM>), dim3(blocks), dim3(64 * W), lds, s, p,
+                           tsize);
         if (!SYM) bmlhip_prof_mark_end();
     }
     return report("k_spgemm_unord");
@@ -2994,6 +3162,13 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     if (rc) return rc;
     const unsigned ts = (unsigned) tsize;
     const bool sym = a->sym && a->kind == 0 && a->engine == BMLHIP_ROW_UNORDERED && a->group == 64;
+    // batched: the unordered and one-wavefront kernels run persistent over
+    // the work queue (bmlhip_batch_queue built it for this iteration)
+    if (a->batch && a->batch->d_list &&
+        ((a->engine == BMLHIP_ROW_UNORDERED && a->group == 64) || !(a->group == 64 && a->waves > 1))) {
+        p.qlist = a->batch->d_list;
+        p.qinfo = a->batch->d_qinfo;
+    }
     int *fill = nullptr;
     if (sym) {
         fill = (int *) bmlhip_scratch_zeroed(BMLHIP_SYMFILL_SLOT, sizeof(int) * (size_t) (a->N + 1));
@@ -3069,7 +3244,7 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
               const double *abdev = nullptr, const int *skip = nullptr,
               const bmlhip_batch *batch = nullptr, double *rownorm = nullptr) {
     hipStream_t s = (hipStream_t) bmlhip_stream();
-    AddParams p;
+    AddParams p{};
     p.N = N;
     p.MA = MA;
     p.MB = MB;
@@ -3088,6 +3263,8 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
     p.row_sys = batch ? batch->d_row_sys : nullptr;
     p.sys_done = batch ? batch->d_done : nullptr;
     p.ab_sys = batch ? batch->d_ab : nullptr;
+    p.qlist = batch ? batch->d_list : nullptr;
+    p.qinfo = batch ? batch->d_qinfo : nullptr;
     p.st = bmlhip_status_dev();
     p.rownorm = rownorm ? rownorm : (double *) bmlhip_scratch(0, sizeof(double) * (size_t) (N + 1));
     p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (N + 1));
@@ -3102,7 +3279,9 @@ int add_typed(int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,
     const size_t lds_l = (size_t) tl * (2 * sizeof(T) + sizeof(int) + sizeof(uint16_t));
     if (N > 0) {
         if (lds_l <= 64 * 1024 && tl < 65536) {
-            hipLaunchKernelGGL(k_add_lds<T>, dim3(N), dim3(64), lds_l, s, p, tl);
+            const int blocks = !p.qlist ? N
+                               : persistent_blocks((const void *) k_add_lds<T>, 64, lds_l, N);
+            hipLaunchKernelGGL(k_add_lds<T>, dim3(blocks), dim3(64), lds_l, s, p, tl);
             if ((rc = report("k_add_lds"))) return rc;
         } else {
             const int blocks = 256;
@@ -3365,6 +3544,19 @@ int bmlhip_identity(int prec, int N, int M, void *Av, int *Ai, int *An) {
 #define CALL(T) identity_typed<T>(N, M, Av, Ai, An)
     DISPATCH(prec, CALL)
 #undef CALL
+}
+
+int bmlhip_batch_queue(const bmlhip_batch *b, int N) {
+    if (!b->d_list || !b->d_qinfo || !b->d_act_off) return 0;
+    int4 *rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (size_t) (N + 1));
+    if (!rowstat) return fail_msg(4, "scratch allocation failed");
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, s, b->nsys, b->d_row_off, b->d_done,
+                       b->d_act_off, b->d_qinfo);
+    if (b->nsys > 0)
+        hipLaunchKernelGGL(k_batch_fill, dim3(b->nsys), dim3(256), 0, s, b->d_row_off, b->d_done,
+                           b->d_act_off, b->d_list, rowstat);
+    return report("k_batch_fill");
 }
 
 int bmlhip_add_batch(int prec, int N, int MA, void *Av, int *Ai, int *An, int MB, const void *Bv,

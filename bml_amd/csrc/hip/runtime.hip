@@ -112,6 +112,37 @@ void bmlhip_shutdown(void) {
 
 void *bmlhip_stream(void) { return (void *) g.stream; }
 
+int bmlhip_device_cus(void) {
+    int n = 0;
+    if (g.device < 0 ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, g.device) != hipSuccess)
+        return 0;
+    return n;
+}
+
+// A timed region on the library stream (the autotuner's measurements):
+// begin records an event, end records a second one, waits for it and gives
+// the milliseconds between them.
+static hipEvent_t g_tev[2];
+
+int bmlhip_time_begin(void) {
+    if (!g_tev[0]) {
+        CHECK(hipEventCreate(&g_tev[0]));
+        CHECK(hipEventCreate(&g_tev[1]));
+    }
+    CHECK(hipEventRecord(g_tev[0], g.stream));
+    return 0;
+}
+
+int bmlhip_time_end(double *ms) {
+    float f = 0.0f;
+    CHECK(hipEventRecord(g_tev[1], g.stream));
+    CHECK(hipEventSynchronize(g_tev[1]));
+    CHECK(hipEventElapsedTime(&f, g_tev[0], g_tev[1]));
+    *ms = (double) f;
+    return 0;
+}
+
 void *bmlhip_noalloc_workspace(size_t *bytes) {
     if (bytes) *bytes = BMLHIP_NOALLOC_BYTES;
     return g.device >= 0 && g.device < 64 ? g.noalloc_ws[g.device] : nullptr;

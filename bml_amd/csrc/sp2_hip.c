@@ -57,10 +57,75 @@ static int stalled(const bml_hip_sp2_options *o, stall_track *t, double norm)
 /* one SP2 iteration on (X, *X2p), all queued on the library stream; *X2p
  * (the product's storage, of X's shape) is allocated on first use: the
  * fused window step updates X in place and needs none */
+typedef struct {
+    bml_matrix_ellpack_hip_t *X, **X2p;
+    double threshold;
+    int sym;
+} sp2_tune_ctx;
+
+/* the autotuner's measurement for an SP2 iteration: X^2 (the symmetric
+ * form when the solve uses it) into X2, timed; X is left as it is */
+static int measure_sp2(const bmlh_spgemm_shape *sh, void *ctx, double *ms)
+{
+    sp2_tune_ctx *c = ctx;
+    bml_matrix_ellpack_hip_t *X = c->X;
+    if (!*c->X2p && !(*c->X2p = bml_noinit_matrix_ellpack_hip(X->matrix_precision, X->N, X->M,
+                                                              X->distribution_mode)))
+        return 1;
+    bml_matrix_ellpack_hip_t *X2 = *c->X2p;
+    bmlhip_spgemm_args a;
+    memset(&a, 0, sizeof a);
+    a.prec = X->matrix_precision;
+    a.N = X->N;
+    a.MA = a.MB = X->M;
+    a.MC = X2->M;
+    a.d_Av = a.d_Bv = X->value;
+    a.d_Ai = a.d_Bi = X->index;
+    a.d_An = a.d_Bn = X->nnz;
+    a.d_Cv = X2->value;
+    a.d_Ci = X2->index;
+    a.d_Cn = X2->nnz;
+    a.threshold = c->threshold;
+    a.group = sh->group;
+    a.table = sh->table;
+    a.bucket = sh->bucket;
+    a.remap = sh->remap;
+    a.waves = sh->waves;
+    a.engine = sh->engine;
+    a.wbits = sh->wbits;
+    a.wrows = sh->wrows;
+    a.sym = c->sym;
+    if (bmlhip_time_begin())
+        return 1;
+    const int rc = bmlhip_spgemm(&a);
+    const int rt = bmlhip_time_end(ms);
+    return rc || rt;
+}
+
+/* launch shape of an SP2 iteration on X (the tuner reads the row counts
+ * the last poll recorded; with autotuning on, measured) */
+static void sp2_shape(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t **X2p,
+                      double threshold, int sym, bmlh_spgemm_shape *sh)
+{
+    const int mx = X->aux->stats_valid ? X->aux->max_nnz : X->M;
+    sp2_tune_ctx c = { X, X2p, threshold, sym };
+    bmlh_tuned_spgemm(X->matrix_precision, X->N, mx, mx, X->aux->touched_hint,
+                      X->aux->hint_max_nnz, BMLH_PICK_SQUARE, sh, measure_sp2, &c);
+}
+
+static int same_shape(const bmlh_spgemm_shape *a, const bmlh_spgemm_shape *b)
+{
+    return a->group == b->group && a->table == b->table && a->bucket == b->bucket &&
+           a->remap == b->remap && a->waves == b->waves && a->engine == b->engine &&
+           a->wbits == b->wbits && a->wrows == b->wrows;
+}
+
 static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t **X2p,
                          bmlhip_sp2_state *d_state, double *d_hist,
-                         const bml_hip_sp2_options *o, double nocc, int sym, const char *what)
+                         const bml_hip_sp2_options *o, double nocc, int sym,
+                         const bmlh_spgemm_shape *shp, const char *what)
 {
+    const bmlh_spgemm_shape sh = *shp;
     bmlhip_spgemm_args a;
     memset(&a, 0, sizeof a);
     a.prec = X->matrix_precision;
@@ -72,9 +137,6 @@ static int sp2_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t *
     a.d_An = a.d_Bn = X->nnz;
     a.threshold = o->threshold;
     a.d_skip = &d_state->done;
-    int mx = X->aux->stats_valid ? X->aux->max_nnz : X->M;
-    bmlh_spgemm_shape sh;
-    bmlh_pick_spgemm(a.prec, X->N, mx, mx, X->aux->touched_hint, X->aux->hint_max_nnz, 1, &sh);
     a.group = sh.group;
     a.table = sh.table;
     a.bucket = sh.bucket;
@@ -168,6 +230,10 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     bml_matrix_ellpack_hip_t *X2 = NULL, *Xs = NULL, *X2s = NULL;
     bmlhip_sp2_state *d_state = NULL;
     double *d_hist = NULL;
+    void *graph = NULL;
+    const bml_matrix_ellpack_hip_t *gcur = NULL;
+    bmlh_spgemm_shape gshape;
+    memset(&gshape, 0, sizeof gshape);
     if (!bmlh_ok(bmlhip_malloc((void **) &d_state, sizeof(bmlhip_sp2_state)), what) ||
         !bmlh_ok(bmlhip_malloc((void **) &d_hist, sizeof(double) * 4 * (size_t) o.maxit), what))
         goto out;
@@ -192,8 +258,38 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     stall_track stall = { 0, 0.0, 0.0 };
     for (int it = 0; it < o.maxit; it++) {
         bml_matrix_ellpack_hip_t *cur = single ? Xs : X;
-        if (!sp2_iteration(cur, single ? &X2s : &X2, d_state, d_hist, &o, nocc, sym, what))
+        bmlh_spgemm_shape sh;
+        sp2_shape(cur, single ? &X2s : &X2, o.threshold, sym, &sh);
+        if (o.graph && o.poll > 1) {
+            /* between two polls every iteration is the same launches: replay
+             * them as a graph, captured anew when the iterate or its launch
+             * shape changes (the first iteration of each runs eagerly, which
+             * sizes every scratch buffer and allocates X2; the captured
+             * kernels are not run by the capture) */
+            if (!graph || gcur != cur || !same_shape(&sh, &gshape)) {
+                bmlhip_graph_destroy(graph);
+                graph = NULL;
+                if (!sp2_iteration(cur, single ? &X2s : &X2, d_state, d_hist, &o, nocc, sym, &sh,
+                                   what))
+                    goto out;
+                if (!bmlh_ok(bmlhip_graph_begin(), what))
+                    goto out;
+                const int okc = sp2_iteration(cur, single ? &X2s : &X2, d_state, d_hist, &o, nocc,
+                                              sym, &sh, what);
+                void *ge = NULL;
+                const int oke = bmlh_ok(bmlhip_graph_end(&ge), what);
+                if (!okc || !oke)
+                    goto out;
+                graph = ge;
+                gshape = sh;
+                gcur = cur;
+            } else if (!bmlh_ok(bmlhip_graph_launch(graph), what)) {
+                goto out;
+            }
+        } else if (!sp2_iteration(cur, single ? &X2s : &X2, d_state, d_hist, &o, nocc, sym, &sh,
+                                  what)) {
             goto out;
+        }
         int last_single = single && it + 1 >= o.maxit - o.refine_iters;
         if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {
             if (!bmlh_ok(bmlhip_d2h(&st, d_state, sizeof st), what))
@@ -246,6 +342,7 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
     if (res)
         *res = r;
 out:
+    bmlhip_graph_destroy(graph);
     bmlhip_free(d_state);
     bmlhip_free(d_hist);
     bml_deallocate_ellpack_hip(X2);
@@ -259,6 +356,7 @@ typedef struct {
     bmlhip_batch b;
     int *row_sys, *row_off, *done, *branch, *it, *alldone, *stats;
     double *ab, *trx, *trx2, *norm2, *norm, *rowtr, *rownorm, *nocc, *emin, *emax;
+    int *list, *qinfo, *act_off; /* the row work queue */
     int sym; /* every H symmetric (Hermitian): symmetric x2 on the unordered engine */
 } batch_dev;
 
@@ -266,7 +364,7 @@ static void batch_free(batch_dev *d)
 {
     void *p[] = { d->row_sys, d->row_off, d->done, d->branch, d->it, d->alldone, d->stats,
                   d->ab, d->trx, d->trx2, d->norm2, d->norm, d->rowtr, d->rownorm, d->nocc,
-                  d->emin, d->emax };
+                  d->emin, d->emax, d->list, d->qinfo, d->act_off };
     for (size_t k = 0; k < sizeof p / sizeof p[0]; k++)
         bmlhip_free(p[k]);
 }
@@ -291,7 +389,10 @@ static int batch_alloc(batch_dev *d, int nsys, int N, const char *what)
              bmlh_ok(bmlhip_malloc((void **) &d->rownorm, R * sizeof(double) + 8), what) &&
              bmlh_ok(bmlhip_malloc((void **) &d->nocc, S * sizeof(double)), what) &&
              bmlh_ok(bmlhip_malloc((void **) &d->emin, S * sizeof(double)), what) &&
-             bmlh_ok(bmlhip_malloc((void **) &d->emax, S * sizeof(double)), what);
+             bmlh_ok(bmlhip_malloc((void **) &d->emax, S * sizeof(double)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->list, R * sizeof(int) + 8), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->qinfo, 4 * sizeof(int)), what) &&
+             bmlh_ok(bmlhip_malloc((void **) &d->act_off, (S + 1) * sizeof(int)), what);
     if (!ok)
         return 0;
     bmlhip_zero(d->done, S * sizeof(int));
@@ -314,6 +415,13 @@ static int batch_alloc(batch_dev *d, int nsys, int N, const char *what)
     d->b.d_norm2 = d->norm2;
     d->b.d_rowtr = d->rowtr;
     d->b.d_rownorm = d->rownorm;
+    /* BML_HIP_BATCH_QUEUE=0: one block per row instead of the work queue */
+    const char *e = getenv("BML_HIP_BATCH_QUEUE");
+    if (!e || atoi(e) != 0) {
+        d->b.d_list = d->list;
+        d->b.d_qinfo = d->qinfo;
+        d->b.d_act_off = d->act_off;
+    }
     return 1;
 }
 
@@ -346,7 +454,7 @@ static int batch_iteration(bml_matrix_ellpack_hip_t *X, bml_matrix_ellpack_hip_t
     a.wrows = sh->wrows;
     a.sym = d->sym;
     a.batch = &d->b;
-    return bmlh_ok(bmlhip_spgemm(&a), what) &&
+    return bmlh_ok(bmlhip_batch_queue(&d->b, X->N), what) && bmlh_ok(bmlhip_spgemm(&a), what) &&
            bmlh_ok(bmlhip_sp2_decide_batch(&d->b, d->nocc, d->branch, d->stats), what) &&
            bmlh_ok(bmlhip_add_batch(X->matrix_precision, X->N, X->M, X->value, X->index, X->nnz,
                                     X2->M, X2->value, X2->index, X2->nnz, o->threshold, 2,
@@ -481,16 +589,13 @@ void bml_hip_sp2_ellpack_batch(int nsys, bml_matrix_ellpack_hip_t *const *H,
         bml_matrix_ellpack_hip_t *cur = single ? Xs : Xc, *cur2 = single ? X2s : X2c;
         const int mx = cur->aux->stats_valid ? cur->aux->max_nnz : cur->M;
         bmlh_spgemm_shape sh;
-        bmlh_pick_spgemm(cur->matrix_precision, N, mx, mx, touched, hint_max, 1, &sh);
+        bmlh_pick_spgemm(cur->matrix_precision, N, mx, mx, touched, hint_max, BMLH_PICK_SQUARE,
+                         &sh);
         if (o.graph) {
             /* replay the iteration as a graph; capture anew when the launch
              * shape or the precision changes (the first pass runs eagerly so
              * every scratch buffer is sized before capture) */
-            const int same = graph && sh.group == gshape.group && sh.table == gshape.table &&
-                             sh.bucket == gshape.bucket && sh.remap == gshape.remap &&
-                             sh.waves == gshape.waves && sh.engine == gshape.engine &&
-                             sh.wbits == gshape.wbits && sh.wrows == gshape.wrows;
-            if (!same) {
+            if (!graph || !same_shape(&sh, &gshape)) {
                 bmlhip_graph_destroy(graph);
                 graph = NULL;
                 if (!batch_iteration(cur, cur2, &d, &o, &sh, what))

@@ -90,6 +90,10 @@ _SIGS = {
     "bml_hip_synchronize": (None, []),
     "bml_hip_stream": (_P, []),
     "bml_hip_version": (C.c_char_p, []),
+    "bml_hip_set_autotune": (None, [_I, C.c_char_p]),
+    "bml_hip_get_autotune": (_I, []),
+    "bml_hip_autotune_entries": (_I, []),
+    "bml_hip_autotune_clear": (None, []),
     "bml_zero_matrix_ellpack_hip": (_P, [_I, _I, _I, _I]),
     "bml_noinit_matrix_ellpack_hip": (_P, [_I, _I, _I, _I]),
     "bml_identity_matrix_ellpack_hip": (_P, [_I, _I, _I, _I]),
@@ -449,13 +453,14 @@ def normalize(A: EllpackHip, mineval: float, maxeval: float) -> None:
 
 def sp2(H: EllpackHip, X: EllpackHip, nocc: float, threshold: float = 1e-5,
         tol: float = 1e-8, maxit: int = 60, mixed: bool = False, refine_iters: int = 4,
-        poll: int = 1, mixed_switch: bool = True, resume: bool = False):
+        poll: int = 1, mixed_switch: bool = True, resume: bool = False, graph: bool = True):
     """X <- SP2 density matrix of H; returns (Sp2Result, history[it, 4]).
-    resume: continue from the iterate X holds."""
+    resume: continue from the iterate X holds; graph (with poll > 1): the
+    iterations between two polls replay a captured graph."""
     o = Sp2Options()
     lib().bml_hip_sp2_defaults(C.byref(o))
     o.threshold, o.tol, o.maxit = threshold, tol, maxit
-    o.mixed, o.refine_iters, o.poll = int(mixed), refine_iters, poll
+    o.mixed, o.refine_iters, o.poll, o.graph = int(mixed), refine_iters, poll, int(graph)
     o.mixed_switch, o.resume = int(mixed_switch), int(resume)
     r = Sp2Result()
     hist = np.zeros((maxit, 4))
@@ -579,6 +584,23 @@ def set_row_engine(engine: int) -> None:
 
 def get_row_engine() -> int:
     return int(lib().bml_hip_get_row_engine())
+
+
+def set_autotune(on: bool, cache_path: str | None = None) -> None:
+    """bml_hip_set_autotune: measured launch shapes, cached in cache_path."""
+    lib().bml_hip_set_autotune(int(on), cache_path.encode() if cache_path else None)
+
+
+def get_autotune() -> bool:
+    return bool(lib().bml_hip_get_autotune())
+
+
+def autotune_entries() -> int:
+    return int(lib().bml_hip_autotune_entries())
+
+
+def autotune_clear() -> None:
+    lib().bml_hip_autotune_clear()
 
 
 def block_matrix(precision: int, bsize, MB: int, M: int = 0) -> EllblockHip:

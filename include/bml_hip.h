@@ -156,6 +156,22 @@ const char *bml_hip_version(void);
 void bml_hip_set_row_engine(int engine);
 int bml_hip_get_row_engine(void);
 
+/* Measured launch shapes.  With autotuning on (also BML_HIP_AUTOTUNE=1),
+ * the first ELLPACK x2 or AB product (and SP2 iteration) of each kind of
+ * operand -- precision, engine and the sizes in steps of 2^(1/4) -- times
+ * candidate launch shapes on its own operands (lanes per row, table size,
+ * bucket width and wavefronts; window width and rows per block on the
+ * window engine) and keeps the fastest.  cache_path (also
+ * BML_HIP_TUNE_CACHE) names a text file the shapes are read from and
+ * appended to, so later processes reuse them; NULL keeps the current one.
+ * Shapes set through the environment (BML_HIP_GROUP and the like) are not
+ * tuned.  entries counts the shapes held; clear forgets them (the file is
+ * left alone).  No bml counterpart. */
+void bml_hip_set_autotune(int on, const char *cache_path);
+int bml_hip_get_autotune(void);
+int bml_hip_autotune_entries(void);
+void bml_hip_autotune_clear(void);
+
 /* ---- ELLPACK-R: allocation and conversion ------------------------------- */
 
 /* bml_zero_matrix_ellpack (ellpack/bml_allocate_ellpack.c, typed
@@ -408,8 +424,9 @@ typedef struct bml_hip_sp2_options {
                           refine_iters iterations in double */
     int refine_iters;  /* double-precision iterations at the end (mixed) */
     int poll;          /* read the convergence flag every poll iterations */
-    int graph;         /* batched solves: replay each iteration's kernels as a
-                          captured hipGraph (1) or launch them one by one (0) */
+    int graph;         /* replay each iteration's kernels as a captured
+                          hipGraph (1) or launch them one by one (0): batched
+                          solves, and single ELLPACK solves with poll > 1 */
     int mixed_switch;  /* mixed: also move to double as soon as the single-
                           precision iterate stops improving (1, needs poll 1),
                           or only for the last refine_iters iterations (0) */

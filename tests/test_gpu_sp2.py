@@ -124,6 +124,75 @@ def test_batch_matches_single_solves(hip):
         assert_ell_identical(a, b, "graph vs eager")
 
 
+@pytest.mark.parametrize("engine", ["EXACT", "UNORDERED", "WINDOW"])
+def test_single_solve_graph_between_polls(hip, engine):
+    """poll = 4: the iterations between two polls replay a captured graph
+    (captured again when the launch shape changes at a poll); the solve
+    runs past convergence on the done flag.  Against one-by-one launches
+    with poll = 1: exact engine the same bits, the other engines to
+    rounding (their sums follow the atomics' order)."""
+    H = G.lattice_hamiltonian(8, 8, 8, M=16, hopping=-0.1, disorder=0.01)
+    dH = hip.from_ell(H)
+    hip.set_row_engine(getattr(hip, "ROW_ENGINE_" + engine))
+    try:
+        runs = []
+        for poll, graph in ((1, False), (4, True), (4, True)):
+            X = hip.zero_matrix(H.precision, H.N, 512)
+            r, h = hip.sp2(dH, X, H.N / 2, threshold=1e-9, tol=1e-9, maxit=50, poll=poll,
+                           graph=graph)
+            runs.append((r, h, X.get()))
+    finally:
+        hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+    (r0, h0, X0) = runs[0]
+    assert r0.converged and r0.iterations == 13  # the oracle's count; then 3 queued replays
+    for r, h, X in runs[1:]:
+        assert r.converged == r0.converged
+        if engine == "EXACT":
+            assert r.iterations == r0.iterations
+            np.testing.assert_array_equal(h, h0)
+            assert_ell_identical(X, X0, "graph vs eager")
+        else:
+            assert abs(r.iterations - r0.iterations) <= 1
+            assert np.abs(X.to_dense() - X0.to_dense()).max() <= 1e-9
+
+
+@pytest.mark.parametrize("engine", ["EXACT", "UNORDERED"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_work_queue_matches_block_per_row(hip, monkeypatch, engine, graph):
+    """The batched row kernels run persistent over a work queue (the rows of
+    the unfinished systems, rebuilt on the device before each product and
+    taken with an atomic cursor) unless BML_HIP_BATCH_QUEUE=0 gives one
+    block per row.  The systems converge after different iteration counts,
+    so the queue shrinks as the solve goes.  Exact engine: the same bits;
+    unordered engine: the same to rounding (its sums follow the atomics'
+    order; the 8^3 system runs its 50 iterations at the threshold's
+    idempotency floor, ~2e-7, where entries within rounding of the
+    threshold are kept or dropped differently: X within 10 x threshold)."""
+    Hs = _batch_systems()
+    dHs = [hip.from_ell(H) for H in Hs]
+    nocc = [H.N / 2 for H in Hs]
+    outs = {}
+    hip.set_row_engine(getattr(hip, "ROW_ENGINE_" + engine))
+    try:
+        for queue in ("0", "1"):
+            monkeypatch.setenv("BML_HIP_BATCH_QUEUE", queue)
+            Xs = [hip.zero_matrix(H.precision, H.N, 512) for H in Hs]
+            res = hip.sp2_batch(dHs, Xs, nocc, threshold=1e-7, tol=1e-9, maxit=50, graph=graph)
+            outs[queue] = (res, [X.get() for X in Xs])
+    finally:
+        hip.set_row_engine(hip.ROW_ENGINE_EXACT)
+    its = sorted({r.iterations for r in outs["1"][0]})
+    assert len(its) > 1, its  # systems leave the queue at different iterations
+    for a, b, Xa, Xb in zip(outs["0"][0], outs["1"][0], outs["0"][1], outs["1"][1]):
+        if engine == "EXACT":
+            assert a.iterations == b.iterations and a.converged == b.converged
+            assert a.trace_x == b.trace_x and a.idempotency == b.idempotency
+            assert_ell_identical(Xa, Xb, "queue vs block per row")
+        else:
+            assert abs(a.iterations - b.iterations) <= 1
+            assert np.abs(Xa.to_dense() - Xb.to_dense()).max() <= 1e-6
+
+
 def test_batch_mixed_precision(hip):
     Hs = _batch_systems()
     dHs = [hip.from_ell(H) for H in Hs]
