@@ -26,6 +26,7 @@
 #include "devtypes.h"
 
 #include <limits.h>
+#include <type_traits>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -1079,19 +1080,23 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_unord(SpParams p, unsigned ts
         unord_row<T, W, BK, SYM>(p, tsize, row, keys, vals, res_s, ovf_s, prods_s, trx_s);
         return;
     }
-    // persistent over the work queue, one listed row per take; the cursor
-    // is read by every thread between two barriers, and written again only
-    // after the next two
+    // persistent over the work queue, one listed row per take; the next
+    // row is taken while the current one runs (its atomic's latency hides
+    // behind the row's loads).  item_s is read by every thread after a
+    // barrier and written again only after the next one.
     const int n = p.qinfo[0];
-    while (true) {
-        if (threadIdx.x == 0) item_s = atomicAdd(&p.qinfo[1], 1);
-        __syncthreads();
-        const int q = item_s;
-        if (q >= n) return;
+    if (threadIdx.x == 0) item_s = atomicAdd(&p.qinfo[1], 1);
+    __syncthreads();
+    int q = item_s;
+    while (q < n) {
+        int nxt = 0;
+        if (threadIdx.x == 0) nxt = atomicAdd(&p.qinfo[1], 1);
         unord_row<T, W, BK, SYM>(p, tsize, p.qlist[q], keys, vals, res_s, ovf_s, prods_s, trx_s);
         __syncthreads();  // the epilogue's table reads are done
         clear();
+        if (threadIdx.x == 0) item_s = nxt;
         __syncthreads();
+        q = item_s;
     }
 }
 
@@ -1183,6 +1188,23 @@ __device__ __forceinline__ void pk_get(const unsigned *P, size_t e, T &v, int &c
     v = cv.t;
     c = (int) x.w[NW - 1];
 }
+// Entry k of a packed row starting at Pj: a scalar base and a 32-bit
+// per-lane byte offset (one global load with a scalar address; rows hold
+// fewer than 2^24 entries).
+template <typename T>
+__device__ __forceinline__ void pk_get_at(const unsigned *Pj, unsigned k, T &v, int &c) {
+    constexpr int NW = PkW<T>::n;
+    const unsigned off = __umul24(k, 4u * NW);
+    const Words<NW> x = *reinterpret_cast<const Words<NW> *>((const char *) Pj + off);
+    union {
+        T t;
+        unsigned u[NW - 1];
+    } cv;
+#pragma unroll
+    for (int q = 0; q < NW - 1; q++) cv.u[q] = x.w[q];
+    v = cv.t;
+    c = (int) x.w[NW - 1];
+}
 template <typename T>
 __device__ __forceinline__ int pk_col(const unsigned *P, size_t e) {
     return (int) P[e * PkW<T>::n + PkW<T>::n - 1];
@@ -1216,6 +1238,8 @@ struct WinParams {
     const unsigned *P;               // packed rows: N x MP entries
     const int *boff;                 // [N][nwin + 1] window offsets within each packed row
     const unsigned long long *wmask; // [N] windows each row touches
+    const double *rowmax;            // [N] largest |X_jk| of each row (row blocks' scale)
+    int fx_extra;                    // row blocks: scale exponent lowered by this (wide rows)
     void *Cv;
     int *Ci, *Cn;
     double thr;
@@ -1246,8 +1270,8 @@ __global__ __launch_bounds__(256) void k_win_pack(int N, int M, const T *Xv, con
                                                   const int *Xn, int MP, int nwin, int wb,
                                                   unsigned *P, int *boff,
                                                   unsigned long long *wmask, double *rowtr,
-                                                  const int *skip, const int *row_sys,
-                                                  const int *sys_done) {
+                                                  double *rowmax, const int *skip,
+                                                  const int *row_sys, const int *sys_done) {
     __shared__ int cnt[4][65];
     if (skip && *skip) return;  // uniform
     const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
@@ -1257,15 +1281,20 @@ __global__ __launch_bounds__(256) void k_win_pack(int N, int M, const T *Xv, con
     const size_t rowX = live ? (size_t) j * M : 0;
     cnt[w][lane] = 0;
     __syncthreads();
-    double trx = 0.0, sq = 0.0;
+    double trx = 0.0, sq = 0.0, mx = 0.0;
     for (int e = lane; e < n; e += 64) {
         const int c = Xi[rowX + e];
         atomicAdd(&cnt[w][c >> wb], 1);
-        if (rowtr) {
+        if (rowtr || rowmax) {
             const T v = Xv[rowX + e];
             if (c == j) trx += Elem<T>::re(v);
             sq += Elem<T>::abs2(v);
+            mx = fmax(mx, Elem<T>::mag(v));
         }
+    }
+    if (rowmax && live) {
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+        if (lane == 0) rowmax[j] = mx;
     }
     __syncthreads();
     const int x = lane < nwin ? cnt[w][lane] : 0;
@@ -1624,19 +1653,95 @@ __global__ __launch_bounds__(256) void k_win_union(int N, int M, const T *Xv, co
     }
 }
 
+// Fixed-point window sums (row blocks).  A row block sums its products as
+// 64-bit integers in units of 2^-E: integer adds are associative, so the sums
+// do not depend on the order the wavefronts' atomics arrive in, and an LDS
+// add of 64-bit integers (ds_add_u64) runs about 1.8x as many random-address
+// adds per second as one of doubles (ds_add_f64) on gfx950.  E is the
+// block's own: with ma the largest |X_ik| of its rows and mb the largest of
+// the rows j it streams, 2^(e-1) <= ma * mb < 2^e, E = 50 - e - fx_extra, so
+// every product is below 2^50 units and a slot's sum of at most M products
+// below 2^62.  A product a * b is rounded to units by one fused multiply-add
+// t = (a 2^E) * b + 1.5 * 2^52: t lies in [2^52, 2^53), where consecutive
+// doubles are one unit apart, so t's bits minus those of 1.5 * 2^52 are the
+// rounded product in two's complement (the constant's low word is zero).
+// The rounding error is at most 2^-(E+1) per product, about 2^-52 ma mb:
+// below the tolerance of the double-precision unordered sums.  Entries must
+// be finite.
+template <typename T> struct FxW;
+template <> struct FxW<float> {
+    static constexpr int NC = 1;  // 64-bit slots per column
+    using V = double;             // an entry widened (products of floats are exact)
+    __device__ static double widen(float v) { return (double) v; }
+    __device__ static float out(const unsigned long long *w, int s, double inv) {
+        return (float) ((double) (long long) w[s] * inv);
+    }
+};
+template <> struct FxW<double> {
+    static constexpr int NC = 1;
+    using V = double;
+    __device__ static double widen(double v) { return v; }
+    __device__ static double out(const unsigned long long *w, int s, double inv) {
+        return (double) (long long) w[s] * inv;
+    }
+};
+template <> struct FxW<cf32> {
+    static constexpr int NC = 2;
+    using V = cf64;
+    __device__ static cf64 widen(cf32 v) { return {(double) v.x, (double) v.y}; }
+    __device__ static cf32 out(const unsigned long long *w, int s, double inv) {
+        return {(float) ((double) (long long) w[2 * s] * inv),
+                (float) ((double) (long long) w[2 * s + 1] * inv)};
+    }
+};
+template <> struct FxW<cf64> {
+    static constexpr int NC = 2;
+    using V = cf64;
+    __device__ static cf64 widen(cf64 v) { return v; }
+    __device__ static cf64 out(const unsigned long long *w, int s, double inv) {
+        return {(double) (long long) w[2 * s] * inv, (double) (long long) w[2 * s + 1] * inv};
+    }
+};
+__device__ __forceinline__ void lds_add_u64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// (t's bits) - (bits of 1.5 * 2^52): round(x) for t = x + 1.5 * 2^52, |x| < 2^51
+__device__ __forceinline__ unsigned long long fx_units(double t) {
+    return (unsigned long long) __double_as_longlong(t) - 0x4338000000000000ull;
+}
+constexpr double FX_MAGIC = 6755399441055744.0;  // 1.5 * 2^52
+// a (scaled by 2^E) times b, rounded to units, added into slot s
+__device__ __forceinline__ void fx_add(unsigned long long *win, int s, double a, double b) {
+    lds_add_u64(&win[s], fx_units(__builtin_fma(a, b, FX_MAGIC)));
+}
+__device__ __forceinline__ void fx_add(unsigned long long *win, int s, cf64 a, cf64 b) {
+    // re = ar br - ai bi, im = ar bi + ai br; the inner sum rounds to units
+    // first, so each component takes two roundings
+    const double re = __builtin_fma(a.x, b.x, __builtin_fma(-a.y, b.y, FX_MAGIC));
+    const double im = __builtin_fma(a.x, b.y, __builtin_fma(a.y, b.x, FX_MAGIC));
+    lds_add_u64(&win[2 * s], fx_units(re));
+    lds_add_u64(&win[2 * s + 1], fx_units(im));
+}
+__device__ __forceinline__ double fx_scale(double v, double sc) { return v * sc; }
+__device__ __forceinline__ cf64 fx_scale(cf64 v, double sc) { return {v.x * sc, v.y * sc}; }
+
 template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
 __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
     using E = Elem<T>;
-    using A = typename UAcc<T>::type;
+    using F = FxW<T>;
+    using V = typename F::V;
     constexpr int WC = 1 << WB;
     constexpr int NT = 64 * W;
+    constexpr int NC = F::NC;
     constexpr int PER = WC / W;  // epilogue columns per wavefront
     static_assert(PER % 64 == 0, "window too small for the block");
     static_assert(WC % R == 0, "a row block must lie in one window");
     extern __shared__ __align__(16) unsigned char smem[];
-    A *win = (A *) smem;  // R windows of WC accumulators, then one dummy slot per lane
-    T *xwin = (T *) (smem + sizeof(A) * (R * WC + 64));  // FUSE: X's rows in the window
-    __shared__ unsigned long long mask_s;
+    // R windows of WC slots, then one dummy slot per lane; a slot is NC
+    // 64-bit integers
+    unsigned long long *win = (unsigned long long *) smem;
+    T *xwin = (T *) (smem + sizeof(unsigned long long) * NC * (R * WC + 64));  // FUSE: X's rows
+    __shared__ unsigned long long mask_s, mb_s;
     __shared__ int cnt_s[W], dcnt_s[W], dflag_s[R];
     __shared__ long long prods_s[W][R + 1];
     __shared__ double nrm_s[W][R];
@@ -1651,17 +1756,20 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
     const int nw1 = p.nwin + 1;
     bool live[R];
     int nA[R];
+    double ma = 0.0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int i = i0 + r;
         live[r] = i < p.N && !(p.row_sys && p.sys_done[p.row_sys[i]]);
         nA[r] = live[r] ? p.boff[(size_t) i * nw1 + p.nwin] : 0;
+        if (live[r]) ma = fmax(ma, p.rowmax[i]);
     }
     const int nU = p.Un[b];
     const int *Uj = p.Uj + (size_t) b * p.ustride;
     const T *Uc = (const T *) p.Uc + (size_t) b * p.ustride * R;
     if (tid == 0) {
         mask_s = 0ull;
+        mb_s = 0ull;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             trx_s[r] = E::zero();
@@ -1669,11 +1777,14 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
         }
     }
     __syncthreads();
-    // the windows the block touches: the union of its rows' rows' masks
+    // the windows the block touches (the union of its rows' rows' masks) and
+    // the largest magnitude in the rows it streams
     unsigned long long m = 0ull;
+    double mb = 0.0;
     for (int e = tid; e < nU; e += NT) {
         const int j = Uj[e];
         m |= p.wmask[j];
+        mb = fmax(mb, p.rowmax[j]);
         if constexpr (!FUSE) {
             const int r = j - i0;  // a diagonal coefficient X[i0 + r, i0 + r]
             if (r >= 0 && r < R) trx_s[r] = Uc[(size_t) e * R + r];
@@ -1687,10 +1798,27 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
             if (live[r]) m |= p.wmask[i0 + r];
     }
     m = or64_wave(m);
-    if (lane == 0 && m) atomicOr(&mask_s, m);
+    for (int off = 32; off > 0; off >>= 1) mb = fmax(mb, __shfl_xor(mb, off, 64));
+    if (lane == 0) {
+        if (m) atomicOr(&mask_s, m);
+        // non-negative doubles order as their bits do
+        atomicMax(&mb_s, (unsigned long long) __double_as_longlong(mb));
+    }
     __syncthreads();
     m = mask_s;
     if constexpr (SYM) m &= ~((1ull << (i0 >> WB)) - 1ull);
+    // the block's scale 2^E (block-uniform)
+    int ex = 0;
+    {
+        const double pm = ma * __longlong_as_double((long long) mb_s);
+        if (pm > 0.0 && pm <= 1.7976931348623157e308) {
+            int e;
+            (void) frexp(pm, &e);  // pm < 2^e
+            ex = 50 - e - p.fx_extra;
+            ex = ex < -1000 ? -1000 : (ex > 1000 ? 1000 : ex);
+        }
+    }
+    const double sc = ldexp(1.0, ex), inv = ldexp(1.0, -ex);
     double alpha = 0.0, beta = 0.0;
     if constexpr (FUSE) {
         alpha = p.ab[0];
@@ -1709,13 +1837,14 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
         prods[r] = 0;
         nrm[r] = 0.0;
     }
+    const int dwin = i0 >> WB;  // the window holding the block's diagonal
     while (m) {
         const int c = __ffsll((long long) m) - 1;
         m &= m - 1ull;
         const int cbase = c << WB;
-        for (int s = tid; s < R * WC; s += NT) {
-            win[s] = Elem<A>::zero();
-            if constexpr (FUSE) xwin[s] = E::zero();
+        for (int s = tid; s < NC * R * WC; s += NT) win[s] = 0ull;
+        if constexpr (FUSE) {
+            for (int s = tid; s < R * WC; s += NT) xwin[s] = E::zero();
         }
 #pragma unroll
         for (int r = 0; r < R; r++)
@@ -1726,79 +1855,120 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
         __syncthreads();
         // products: wavefront w takes groups g = w, w + W, ... of 64 union
         // entries; lane q of a group holds entry q's column j, coefficients
-        // and the bounds of row j's segment in window c
-        for (int g0 = w * 64; g0 < nU; g0 += NT) {
-            const int e = g0 + lane;
-            int jq = 0, lo = 0, hi = 0;
-            T cq[R];
+        // and the bounds of row j's segment in window c.  Only the window
+        // holding the diagonal (DIAG) sends SYM's columns below row i0 + r
+        // to the lane's dummy slot and notes (!SYM) the diagonal's touch.
+        auto products = [&](auto diag) {
+            constexpr bool DIAG = decltype(diag)::value;
+            // an LDS base with the window's first column at 0 (slot of
+            // column k in row r: r * WC + k - cbase)
+            unsigned long long *wc = win - (size_t) NC * cbase;
+            for (int g0 = w * 64; g0 < nU; g0 += NT) {
+                const int e = g0 + lane;
+                int jq = 0, lo = 0, hi = 0;
+                unsigned rq = 0;  // rows with a nonzero coefficient
+                V cs[R];          // the coefficients, widened and scaled by 2^E
 #pragma unroll
-            for (int r = 0; r < R; r++) cq[r] = E::zero();
-            if (e < nU) {
-                jq = Uj[e];
-                const int *bo = p.boff + (size_t) jq * nw1;
-                lo = bo[c];
-                hi = bo[c + 1];
-#pragma unroll
-                for (int r = 0; r < R; r++) cq[r] = Uc[(size_t) e * R + r];
-            }
-            uint64_t todo = ballot64(hi > lo);
-            while (todo) {
-                const int q = __ffsll((long long) todo) - 1;
-                todo &= todo - 1ull;
-                const int j = __builtin_amdgcn_readlane(jq, q);
-                const int l0 = __builtin_amdgcn_readlane(lo, q);
-                const int l1 = __builtin_amdgcn_readlane(hi, q);
-                // products in the accumulator's precision (a single-precision
-                // entry is widened once for all R rows; the product of two
-                // floats is exact in double)
-                A a[R];
-                unsigned rm = 0;  // rows with a nonzero coefficient (wave-uniform)
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const T ar = bcast<64>(cq[r], q);
-                    a[r] = UAcc<T>::to(ar);
-                    if (E::nonzero(ar)) rm |= 1u << r;
-                }
-                const size_t rb = (size_t) j * p.MP;
-                streamed += l1 - l0;
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    if (rm >> r & 1u) prods[r] += l1 - l0;
-                int k0 = l0 + lane;
-                // two entries per lane per round: both loads issue before
-                // the adds; (SYM) products of columns below row i0 + r go to
-                // the lane's dummy slot
-                for (; k0 + 64 < l1; k0 += 128) {
-                    T b0, b1;
-                    int c0, c1;
-                    pk_get<T>(P, rb + k0, b0, c0);
-                    pk_get<T>(P, rb + k0 + 64, b1, c1);
-                    const A w0 = UAcc<T>::to(b0), w1 = UAcc<T>::to(b1);
+                for (int r = 0; r < R; r++) cs[r] = fx_scale(F::widen(E::zero()), 0.0);
+                if (e < nU) {
+                    jq = Uj[e];
+                    const int *bo = p.boff + (size_t) jq * nw1;
+                    lo = bo[c];
+                    hi = bo[c + 1];
 #pragma unroll
                     for (int r = 0; r < R; r++) {
-                        if (!(rm >> r & 1u)) continue;  // uniform
-                        const int s0 = (!SYM || c0 >= i0 + r) ? r * WC + c0 - cbase : dummy;
-                        const int s1 = (!SYM || c1 >= i0 + r) ? r * WC + c1 - cbase : dummy;
-                        lds_add(&win[s0], Elem<A>::mul(a[r], w0));
-                        lds_add(&win[s1], Elem<A>::mul(a[r], w1));
-                        if constexpr (!SYM) dtouch[r] = dtouch[r] | (c0 == i0 + r) | (c1 == i0 + r);
+                        const T t = Uc[(size_t) e * R + r];
+                        cs[r] = fx_scale(F::widen(t), sc);
+                        if (E::nonzero(t)) rq |= 1u << r;
                     }
                 }
-                if (k0 < l1) {
-                    T b0;
-                    int c0;
-                    pk_get<T>(P, rb + k0, b0, c0);
-                    const A w0 = UAcc<T>::to(b0);
+                uint64_t todo = ballot64(hi > lo);
+                if (!todo) continue;
+                // The group's segments as a sequence of rounds of 128
+                // entries (two per lane), software-pipelined: the next
+                // round's loads are issued before the current round's adds.
+                // Segment state is wave-uniform (scalar registers).
+                struct Seg {
+                    const unsigned *Pj;  // row j's packed entries
+                    int l1;              // segment end
+                    unsigned rm;
+                    V a[R];
+                };
+                auto fetch = [&](Seg &sg, int &kb) {
+                    const int q = __ffsll((long long) todo) - 1;
+                    todo &= todo - 1ull;
+                    const int j = __builtin_amdgcn_readlane(jq, q);
+                    kb = __builtin_amdgcn_readlane(lo, q);
+                    sg.l1 = __builtin_amdgcn_readlane(hi, q);
+                    sg.rm = (unsigned) __builtin_amdgcn_readlane((int) rq, q);
+                    sg.Pj = P + (size_t) j * p.MP * PkW<T>::n;
+#pragma unroll
+                    for (int r = 0; r < R; r++) sg.a[r] = bcast<64>(cs[r], q);
+                    const int len = sg.l1 - kb;
+                    streamed += len;
+#pragma unroll
+                    for (int r = 0; r < R; r++)
+                        if (sg.rm >> r & 1u) prods[r] += len;
+                };
+                // Two loads per lane, always issued (straight-line code, so
+                // that the adds wait only for their own round's loads):
+                // lanes past the segment's end reload its last entry and
+                // mark it column -1.
+                auto load = [&](const Seg &sg, int kb, T &b0, int &c0, T &b1, int &c1) {
+                    const int k0 = kb + lane, kl = sg.l1 - 1;
+                    pk_get_at<T>(sg.Pj, (unsigned) (k0 < kl ? k0 : kl), b0, c0);
+                    pk_get_at<T>(sg.Pj, (unsigned) (k0 + 64 < kl ? k0 + 64 : kl), b1, c1);
+                    if (k0 > kl) c0 = -1;
+                    if (k0 + 64 > kl) c1 = -1;
+                };
+                auto add = [&](const Seg &sg, T bv, int col) {
+                    if (col < 0) return;
+                    const V wv = F::widen(bv);
 #pragma unroll
                     for (int r = 0; r < R; r++) {
-                        if (!(rm >> r & 1u)) continue;  // uniform
-                        const int s0 = (!SYM || c0 >= i0 + r) ? r * WC + c0 - cbase : dummy;
-                        lds_add(&win[s0], Elem<A>::mul(a[r], w0));
-                        if constexpr (!SYM) dtouch[r] = dtouch[r] | (c0 == i0 + r);
+                        if (!(sg.rm >> r & 1u)) continue;  // uniform
+                        int sl = r * WC + col;
+                        if constexpr (DIAG && SYM) sl = col >= i0 + r ? sl : dummy + cbase;
+                        fx_add(wc, sl, sg.a[r], wv);
+                        if constexpr (DIAG && !SYM) dtouch[r] = dtouch[r] | (col == i0 + r);
                     }
+                };
+                Seg cur, nxt;
+                int kc, kn;
+                T b0 = E::zero(), b1 = E::zero(), n0 = E::zero(), n1 = E::zero();
+                int c0, c1, d0, d1;
+                fetch(cur, kc);
+                load(cur, kc, b0, c0, b1, c1);
+                while (true) {
+                    // the round after (kc): the rest of this segment, or the next
+                    const bool same = kc + 128 < cur.l1;
+                    const bool more = same || todo != 0ull;
+                    if (same) {
+                        nxt = cur;
+                        kn = kc + 128;
+                    } else if (more) {
+                        fetch(nxt, kn);
+                    } else {
+                        nxt = cur;  // a round that is loaded and never added
+                        kn = kc;
+                    }
+                    load(nxt, kn, n0, d0, n1, d1);
+                    add(cur, b0, c0);
+                    add(cur, b1, c1);
+                    if (!more) break;
+                    cur = nxt;
+                    kc = kn;
+                    b0 = n0;
+                    c0 = d0;
+                    b1 = n1;
+                    c1 = d1;
                 }
             }
-        }
+        };
+        if (c == dwin)
+            products(std::true_type{});
+        else
+            products(std::false_type{});
         if constexpr (!SYM) {
 #pragma unroll
             for (int r = 0; r < R; r++)
@@ -1832,7 +2002,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
             auto entry = [&](int s, T &out, bool &keep, bool &nz, double &dn) {
                 const int k = cbase + s;
                 const bool valid = k < p.N && (!SYM || k >= i);
-                const T y = UAcc<T>::from(win[r * WC + s]);
+                const T y = F::out(win, r * WC + s, inv);
                 nz = valid && (E::mag(y) != 0.0 || k == i);
                 const bool ykeep = valid && (k == i ? dtouched : E::above(y, p.thr));
                 dn = 0.0;
@@ -1954,17 +2124,20 @@ __global__ __launch_bounds__(64) void k_spgemm_lds(SpParams p, unsigned tsize) {
     if (p.qlist) {
         // persistent over the work queue, R listed rows per take (each
         // row's epilogue leaves its table clean)
+        // (the next take is issued while the current rows run)
         const int n = p.qinfo[0];
-        while (true) {
-            int q = 0;
-            if (threadIdx.x == 0) q = atomicAdd(&p.qinfo[1], 1);
-            q = __shfl(q, 0, 64);
-            if ((long long) q * R >= n) break;
+        int q = 0;
+        if (threadIdx.x == 0) q = atomicAdd(&p.qinfo[1], 1);
+        q = __shfl(q, 0, 64);
+        while ((long long) q * R < n) {
+            int nxt = 0;
+            if (threadIdx.x == 0) nxt = atomicAdd(&p.qinfo[1], 1);
             const int v = q * R + g;
             spgemm_row<T, GROUP, BK, false, uint16_t>(p, v < n ? p.qlist[v] : -1, keys + g * stride,
                                                       vals + (size_t) g * stride, ord + g * stride,
                                                       tsize, cap);
             __syncthreads();
+            q = __shfl(nxt, 0, 64);
         }
         return;
     }
@@ -2366,15 +2539,18 @@ __global__ __launch_bounds__(64) void k_add_lds(AddParams p, unsigned tsize) {
     }
     __syncthreads();
     if (p.qlist) {
-        // persistent over the work queue (each row leaves the table clean)
+        // persistent over the work queue (each row leaves the table clean;
+        // the next take is issued while the current row runs)
         const int n = p.qinfo[0];
-        while (true) {
-            int q = 0;
-            if (threadIdx.x == 0) q = atomicAdd(&p.qinfo[2], 1);
-            q = __shfl(q, 0, 64);
-            if (q >= n) break;
+        int q = 0;
+        if (threadIdx.x == 0) q = atomicAdd(&p.qinfo[2], 1);
+        q = __shfl(q, 0, 64);
+        while (q < n) {
+            int nxt = 0;
+            if (threadIdx.x == 0) nxt = atomicAdd(&p.qinfo[2], 1);
             add_row<T, uint16_t>(p, p.qlist[q], keys, xs, ys, ord, tsize);
             __syncthreads();
+            q = __shfl(nxt, 0, 64);
         }
         return;
     }
@@ -2797,9 +2973,9 @@ int launch_win_shape(const WinParams &wp, int wbits, int waves, hipStream_t s) {
 
 template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
 int launch_winb(const WinParams &wp, hipStream_t s) {
-    using A = typename UAcc<T>::type;
     const size_t WC = (size_t) 1 << WB;
-    const size_t lds = sizeof(A) * (R * WC + 64) + (FUSE ? sizeof(T) * R * WC : 0);
+    const size_t lds = sizeof(unsigned long long) * FxW<T>::NC * (R * WC + 64) +
+                       (FUSE ? sizeof(T) * R * WC : 0);
     if (lds > 160 * 1024 - 2048)
         return fail_msg(5, "window engine: %d rows of 2^%d columns need %zu bytes of LDS", R, WB,
                         lds);
@@ -2899,22 +3075,32 @@ int win_pack(int N, int M, const void *Xv, const int *Xi, const int *Xn, int wbi
     if (nwin < 0)
         return fail_msg(5, "window engine: N=%d needs more than %d windows of 2^%d columns", N,
                         BMLHIP_WIN_MAX_WINDOWS, wbits);
+    if (M >= (1 << 24))  // packed rows are addressed with 24-bit entry offsets
+        return fail_msg(5, "window engine: rows of M=%d entries (at most 2^24 - 1)", M);
     unsigned *P = (unsigned *) bmlhip_scratch(BMLHIP_WIN_PACK_SLOT,
                                               sizeof(unsigned) * PkW<T>::n * ((size_t) N * M + 1));
     int *boff = (int *) bmlhip_scratch(BMLHIP_WIN_BOFF_SLOT,
                                        sizeof(int) * ((size_t) N * (nwin + 1) + 1));
+    // the row masks, then the row maxima
     unsigned long long *wmask = (unsigned long long *) bmlhip_scratch(
-        BMLHIP_WIN_MASK_SLOT, sizeof(unsigned long long) * ((size_t) N + 1));
+        BMLHIP_WIN_MASK_SLOT, sizeof(unsigned long long) * 2 * ((size_t) N + 1));
     if (!P || !boff || !wmask) return fail_msg(4, "scratch allocation failed");
+    double *rowmax = (double *) (wmask + N + 1);
     wp.N = N;
     wp.MP = M;
     wp.nwin = nwin;
     wp.P = P;
     wp.boff = boff;
     wp.wmask = wmask;
+    wp.rowmax = rowmax;
+    // a window slot sums at most M products; each is below 2^(50 - fx_extra)
+    // units, so the sum stays below 2^62 while M <= 2^(12 + fx_extra)
+    wp.fx_extra = 0;
+    while ((long long) M > (1ll << (12 + wp.fx_extra))) wp.fx_extra++;
     if (N > 0)
         hipLaunchKernelGGL(k_win_pack<T>, dim3((N + 3) / 4), dim3(256), 0, s, N, M, (const T *) Xv,
-                           Xi, Xn, M, nwin, wbits, P, boff, wmask, rowtr, skip, row_sys, sys_done);
+                           Xi, Xn, M, nwin, wbits, P, boff, wmask, rowtr, rowmax, skip, row_sys,
+                           sys_done);
     return report("k_win_pack");
 }
 
