@@ -222,24 +222,36 @@ def lib_digest():
     return h.hexdigest()[:16]
 
 
+def kernel_digest():
+    """Digest of what the window kernel is compiled from: its source file,
+    the device header and the build flags (host-side changes leave it)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("hip/ellpack.hip", "hip/devtypes.h", "Makefile"):
+        with open(os.path.join(ROOT, "bml_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def profiled_traffic():
     """HBM bytes per launch of the timed steps' window kernel, measured with
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this same command and scaled
     by the copy calibration (summary committed under profiles/).  Taken only
-    when the summary was measured on this very library build (its digest),
-    so a changed kernel reports null rather than a stale figure."""
+    when the summary was measured on this very kernel source and build flags
+    (their digest), so a changed kernel reports null rather than a stale
+    figure."""
     f = os.path.join(ROOT, "profiles", "sp2_lattice32_window_traffic.md")
     out = {}
     if os.path.exists(f):
         lib = None
         for line in open(f):
-            if line.startswith("library_sha256_16"):
+            if line.startswith("kernel_source_sha256_16"):
                 lib = line.split()[1]
             if line.startswith("traffic_bytes_per_launch"):
                 for kv in line.split()[1:]:
                     k, v = kv.split("=")
                     out[int(k[1:])] = int(v)
-        if lib != lib_digest():
+        if lib != kernel_digest():
             return {}
     return out
 
@@ -546,6 +558,7 @@ def main():
         "row_engine": "unordered setting (window engine for these rows)",
         "copy_calibration": calib,
         "library_sha256_16": lib_digest(),
+        "kernel_source_sha256_16": kernel_digest(),
         "x2_config1": None if args.only_headline else x2_config1(hip, G, cpu=not args.no_cpu),
         "x2_config5": None if args.only_headline else x2_config5(hip, G),
         "ellblock_sp2": None if args.only_headline else ellblock_sp2(hip, G),

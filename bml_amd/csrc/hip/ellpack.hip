@@ -1725,8 +1725,17 @@ __device__ __forceinline__ void fx_add(unsigned long long *win, int s, cf64 a, c
 __device__ __forceinline__ double fx_scale(double v, double sc) { return v * sc; }
 __device__ __forceinline__ cf64 fx_scale(cf64 v, double sc) { return {v.x * sc, v.y * sc}; }
 
+// Waves per SIMD the compiler is asked to fit (registers).  The fused step's
+// 4-row blocks of 2^10-column windows need 37 KB (float) and 42 KB (double)
+// of LDS, so 3 blocks of 8 wavefronts fit a CU; real types hold their
+// registers to 6 waves per SIMD (some spill to scratch), which ran the
+// 32^3 SP2 step's kernel 1.2x (float) and 1.15x (double) faster than the
+// 4-wave default.  Complex blocks are LDS-bound at 1-2 per CU.
+template <typename T> struct WinbOcc { static constexpr int value = 1; };
+template <> struct WinbOcc<float> { static constexpr int value = 6; };
+template <> struct WinbOcc<double> { static constexpr int value = 6; };
 template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
-__global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
+__global__ __launch_bounds__(64 * W, WinbOcc<T>::value) void k_spgemm_winb(WinParams p) {
     using E = Elem<T>;
     using F = FxW<T>;
     using V = typename F::V;
@@ -1740,8 +1749,8 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
     // R windows of WC slots, then one dummy slot per lane; a slot is NC
     // 64-bit integers
     unsigned long long *win = (unsigned long long *) smem;
-    T *xwin = (T *) (smem + sizeof(unsigned long long) * NC * (R * WC + 64));  // FUSE: X's rows
     __shared__ unsigned long long mask_s, mb_s;
+    __shared__ int grp_s;
     __shared__ int cnt_s[W], dcnt_s[W], dflag_s[R];
     __shared__ long long prods_s[W][R + 1];
     __shared__ double nrm_s[W][R];
@@ -1751,6 +1760,9 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
     if (i0 >= p.N) return;
     const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
     const int dummy = R * WC + lane;
+    // FUSE: each wavefront's PER columns of X's row, staged per row in the
+    // epilogue
+    T *xbuf = (T *) (smem + sizeof(unsigned long long) * NC * (R * WC + 64)) + w * PER;
     const uint64_t ltmask = (1ull << lane) - 1ull;
     const unsigned *P = p.P;
     const int nw1 = p.nwin + 1;
@@ -1829,13 +1841,12 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
     int kept[R], distinct[R];  // block-uniform running counts
     long long prods[R];        // this wavefront's products per row (wave-uniform)
     long long streamed = 0;    // this wavefront's entries of B read (wave-uniform)
-    double nrm[R];             // this lane's norm partials
 #pragma unroll
     for (int r = 0; r < R; r++) {
         kept[r] = 0;
         distinct[r] = 0;
         prods[r] = 0;
-        nrm[r] = 0.0;
+        if (lane == 0) nrm_s[w][r] = 0.0;  // FUSE: this wavefront's norm partials
     }
     const int dwin = i0 >> WB;  // the window holding the block's diagonal
     while (m) {
@@ -1843,12 +1854,10 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
         m &= m - 1ull;
         const int cbase = c << WB;
         for (int s = tid; s < NC * R * WC; s += NT) win[s] = 0ull;
-        if constexpr (FUSE) {
-            for (int s = tid; s < R * WC; s += NT) xwin[s] = E::zero();
-        }
 #pragma unroll
         for (int r = 0; r < R; r++)
             if (tid == r) dflag_s[r] = SYM ? (nA[r] > 0) : 0;
+        if (tid == 0) grp_s = 0;
         bool dtouch[R];  // this lane added into column i0 + r (!SYM)
 #pragma unroll
         for (int r = 0; r < R; r++) dtouch[r] = false;
@@ -1863,7 +1872,14 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
             // an LDS base with the window's first column at 0 (slot of
             // column k in row r: r * WC + k - cbase)
             unsigned long long *wc = win - (size_t) NC * cbase;
-            for (int g0 = w * 64; g0 < nU; g0 += NT) {
+            // groups of 64 union entries are taken from a block counter (the
+            // groups' work varies with their segments' lengths)
+            auto next_group = [&]() {
+                int g = 0;
+                if (lane == 0) g = 64 * atomicAdd(&grp_s, 1);
+                return __builtin_amdgcn_readfirstlane(g);
+            };
+            for (int g0 = next_group(); g0 < nU; g0 = next_group()) {
                 const int e = g0 + lane;
                 int jq = 0, lo = 0, hi = 0;
                 unsigned rq = 0;  // rows with a nonzero coefficient
@@ -1884,8 +1900,8 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                 }
                 uint64_t todo = ballot64(hi > lo);
                 if (!todo) continue;
-                // The group's segments as a sequence of rounds of 128
-                // entries (two per lane), software-pipelined: the next
+                // The group's segments as a sequence of rounds of 256
+                // entries (four per lane), software-pipelined: the next
                 // round's loads are issued before the current round's adds.
                 // Segment state is wave-uniform (scalar registers).
                 struct Seg {
@@ -1910,16 +1926,21 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                     for (int r = 0; r < R; r++)
                         if (sg.rm >> r & 1u) prods[r] += len;
                 };
-                // Two loads per lane, always issued (straight-line code, so
+                // EPL loads per lane, always issued (straight-line code, so
                 // that the adds wait only for their own round's loads):
                 // lanes past the segment's end reload its last entry and
                 // mark it column -1.
-                auto load = [&](const Seg &sg, int kb, T &b0, int &c0, T &b1, int &c1) {
+                constexpr int EPL = 4;  // 256 entries per round
+                auto load = [&](const Seg &sg, int kb, T (&bv)[EPL], int (&cv)[EPL]) {
                     const int k0 = kb + lane, kl = sg.l1 - 1;
-                    pk_get_at<T>(sg.Pj, (unsigned) (k0 < kl ? k0 : kl), b0, c0);
-                    pk_get_at<T>(sg.Pj, (unsigned) (k0 + 64 < kl ? k0 + 64 : kl), b1, c1);
-                    if (k0 > kl) c0 = -1;
-                    if (k0 + 64 > kl) c1 = -1;
+#pragma unroll
+                    for (int u = 0; u < EPL; u++) {
+                        const int k = k0 + 64 * u;
+                        pk_get_at<T>(sg.Pj, (unsigned) (k < kl ? k : kl), bv[u], cv[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < EPL; u++)
+                        if (k0 + 64 * u > kl) cv[u] = -1;
                 };
                 auto add = [&](const Seg &sg, T bv, int col) {
                     if (col < 0) return;
@@ -1935,33 +1956,34 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                 };
                 Seg cur, nxt;
                 int kc, kn;
-                T b0 = E::zero(), b1 = E::zero(), n0 = E::zero(), n1 = E::zero();
-                int c0, c1, d0, d1;
+                T bv[EPL], nv[EPL];
+                int cv[EPL], dv[EPL];
                 fetch(cur, kc);
-                load(cur, kc, b0, c0, b1, c1);
+                load(cur, kc, bv, cv);
                 while (true) {
                     // the round after (kc): the rest of this segment, or the next
-                    const bool same = kc + 128 < cur.l1;
+                    const bool same = kc + 64 * EPL < cur.l1;
                     const bool more = same || todo != 0ull;
                     if (same) {
                         nxt = cur;
-                        kn = kc + 128;
+                        kn = kc + 64 * EPL;
                     } else if (more) {
                         fetch(nxt, kn);
                     } else {
                         nxt = cur;  // a round that is loaded and never added
                         kn = kc;
                     }
-                    load(nxt, kn, n0, d0, n1, d1);
-                    add(cur, b0, c0);
-                    add(cur, b1, c1);
+                    load(nxt, kn, nv, dv);
+#pragma unroll
+                    for (int u = 0; u < EPL; u++) add(cur, bv[u], cv[u]);
                     if (!more) break;
                     cur = nxt;
                     kc = kn;
-                    b0 = n0;
-                    c0 = d0;
-                    b1 = n1;
-                    c1 = d1;
+#pragma unroll
+                    for (int u = 0; u < EPL; u++) {
+                        bv[u] = nv[u];
+                        cv[u] = dv[u];
+                    }
                 }
             }
         };
@@ -1974,21 +1996,6 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
             for (int r = 0; r < R; r++)
                 if (__any(dtouch[r]) && lane == 0) dflag_s[r] = 1;
         }
-        if constexpr (FUSE) {
-            // X's own rows in this window
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                if (!live[r]) continue;
-                const int i = i0 + r;
-                const int *bo = p.boff + (size_t) i * nw1;
-                for (int e = bo[c] + tid; e < bo[c + 1]; e += NT) {
-                    T v;
-                    int k;
-                    pk_get<T>(P, (size_t) i * p.MP + e, v, k);
-                    xwin[r * WC + k - cbase] = v;
-                }
-            }
-        }
         __syncthreads();
         // epilogue, one row at a time: wavefront w walks columns
         // [w PER, (w+1) PER) of row r's window twice, counting, then
@@ -1999,6 +2006,24 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
             const int i = i0 + r;
             const size_t rowC = (size_t) i * MC;
             const bool dtouched = dflag_s[r] != 0;
+            if constexpr (FUSE) {
+                // X's row i in this wavefront's columns of the window
+                for (int s = lane; s < PER; s += 64) xbuf[s] = E::zero();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int *bo = p.boff + (size_t) i * nw1;
+                const int kw = cbase + w * PER;
+                for (int e = bo[c] + lane; e < bo[c + 1]; e += 64) {
+                    T v;
+                    int k;
+                    pk_get<T>(P, (size_t) i * p.MP + e, v, k);
+                    if (k >= kw && k < kw + PER) xbuf[k - kw] = v;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
             auto entry = [&](int s, T &out, bool &keep, bool &nz, double &dn) {
                 const int k = cbase + s;
                 const bool valid = k < p.N && (!SYM || k >= i);
@@ -2007,7 +2032,7 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                 const bool ykeep = valid && (k == i ? dtouched : E::above(y, p.thr));
                 dn = 0.0;
                 if constexpr (FUSE) {
-                    const T x = xwin[r * WC + s];
+                    const T x = xbuf[s - w * PER];
                     const T yk = ykeep ? y : E::zero();
                     if (valid) dn = E::abs2(E::sub(x, yk)) * ((SYM && k > i) ? 2.0 : 1.0);
                     out = E::axpy(E::axpy(E::zero(), alpha, x), beta, yk);
@@ -2018,19 +2043,24 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
                 }
             };
             int cw = 0, dw = 0;
+            double nw = 0.0;
             for (int s = w * PER + lane; s < (w + 1) * PER; s += 64) {
                 T v;
                 bool keep, nz;
                 double dn;
                 entry(s, v, keep, nz, dn);
-                nrm[r] += dn;
+                nw += dn;
                 cw += popc64(ballot64(keep));
                 dw += popc64(ballot64(nz));
                 if (!FUSE && cbase + s == i) d2_s[r] = v;
             }
+            if constexpr (FUSE) {
+                for (int off = 32; off > 0; off >>= 1) nw += __shfl_down(nw, off, 64);
+            }
             if (lane == 0) {
                 cnt_s[w] = cw;
                 dcnt_s[w] = dw;
+                if constexpr (FUSE) nrm_s[w][r] += nw;
             }
             __syncthreads();
             int pos0 = kept[r], tot = 0, dtot = 0;
@@ -2062,14 +2092,6 @@ __global__ __launch_bounds__(64 * W) void k_spgemm_winb(WinParams p) {
 #pragma unroll
         for (int r = 0; r < R; r++) prods_s[w][r] = prods[r];
         prods_s[w][R] = streamed;
-    }
-    if constexpr (FUSE) {
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            double x = nrm[r];
-            for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-            if (lane == 0) nrm_s[w][r] = x;
-        }
     }
     __syncthreads();
 #pragma unroll
@@ -2975,7 +2997,7 @@ template <typename T, int W, int WB, int R, bool SYM, bool FUSE>
 int launch_winb(const WinParams &wp, hipStream_t s) {
     const size_t WC = (size_t) 1 << WB;
     const size_t lds = sizeof(unsigned long long) * FxW<T>::NC * (R * WC + 64) +
-                       (FUSE ? sizeof(T) * R * WC : 0);
+                       (FUSE ? sizeof(T) * WC : 0);
     if (lds > 160 * 1024 - 2048)
         return fail_msg(5, "window engine: %d rows of 2^%d columns need %zu bytes of LDS", R, WB,
                         lds);
@@ -3348,10 +3370,13 @@ int spgemm_typed(const bmlhip_spgemm_args *a) {
     if (rc) return rc;
     const unsigned ts = (unsigned) tsize;
     const bool sym = a->sym && a->kind == 0 && a->engine == BMLHIP_ROW_UNORDERED && a->group == 64;
-    // batched: the unordered and one-wavefront kernels run persistent over
-    // the work queue (bmlhip_batch_queue built it for this iteration)
-    if (a->batch && a->batch->d_list &&
-        ((a->engine == BMLHIP_ROW_UNORDERED && a->group == 64) || !(a->group == 64 && a->waves > 1))) {
+    // batched: the one-wavefront kernels run persistent over the work queue
+    // (bmlhip_batch_queue built it for this iteration).  The unordered
+    // engine's multi-wavefront blocks keep one block per row (rows of
+    // finished systems return at once): on the 512-system batch at hopping
+    // -0.1 they ran 147 solves/s so, against 129 persistent (each take waits
+    // for the listed row's index before the row's loads can start).
+    if (a->batch && a->batch->d_list && !(a->group == 64 && a->waves > 1)) {
         p.qlist = a->batch->d_list;
         p.qinfo = a->batch->d_qinfo;
     }

@@ -415,9 +415,13 @@ static int batch_alloc(batch_dev *d, int nsys, int N, const char *what)
     d->b.d_norm2 = d->norm2;
     d->b.d_rowtr = d->rowtr;
     d->b.d_rownorm = d->rownorm;
-    /* BML_HIP_BATCH_QUEUE=0: one block per row instead of the work queue */
+    /* The work queue is on for the exact engine and off for the others
+     * (their batched iterations ran faster with one block per row: 147
+     * against 129 solves/s on the 512-system batch); BML_HIP_BATCH_QUEUE=1
+     * or 0 sets it either way. */
     const char *e = getenv("BML_HIP_BATCH_QUEUE");
-    if (!e || atoi(e) != 0) {
+    const int queue = e ? atoi(e) != 0 : bml_hip_get_row_engine() == BML_HIP_ROW_ENGINE_EXACT;
+    if (queue) {
         d->b.d_list = d->list;
         d->b.d_qinfo = d->qinfo;
         d->b.d_act_off = d->act_off;
