@@ -2,9 +2,11 @@
 setting's choice for squares whose rows touch more columns than an LDS hash
 table holds).
 
-Each output row adds its products into a dense LDS window of 2^11..2^13
-columns per window it touches, so sums are formed in arrival order and rows
-come out sorted by column.  Results are compared with the oracle (the
+Each output row adds its products into a dense LDS window of 2^10..2^13
+columns per window it touches, and rows come out sorted by column.  The
+one-row kernel sums doubles in arrival order; the row blocks (2 or 4 rows)
+sum 64-bit fixed-point units, so their results do not depend on the order
+the adds arrive in.  Results are compared with the oracle (the
 reference's sums, bml_multiply_ellpack_typed.c:130-349 and the SP2 steps a
 bml client runs) as sets per row: values within 1e-12 x max|ref| in double
 precision (2e-5 in single), patterns equal except entries that close to the
@@ -57,6 +59,31 @@ def test_x2_window_matches_oracle(window, rows, dtype, shape):
         cols = got.index[i, : got.nnz[i]]
         assert np.all(np.diff(cols) > 0), f"row {i} not sorted"
     assert rel_close(tr[0], rtr[0], vtol(dtype)) and rel_close(tr[1], rtr[1], vtol(dtype))
+
+
+@pytest.mark.parametrize("rows_per_block", [2, 4])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_row_block_sums_are_order_independent(window, monkeypatch, rows_per_block, dtype):
+    """Row blocks sum in 64-bit fixed point: integer adds are associative, so
+    the same product repeated gives the same bits whatever order the
+    wavefronts' LDS adds arrive in (the double-precision one-row kernel's
+    last bits may change between runs)."""
+    hip = window
+    monkeypatch.setenv("BML_HIP_WIN_ROWS", str(rows_per_block))
+    A = G.random_ell(3000, 200, 160, 91, dtype)
+    dA = hip.from_ell(A)
+    outs = []
+    for _ in range(3):
+        X2 = hip.zero_matrix(A.precision, A.N, 3000)
+        tr = hip.multiply_x2(dA, X2, 1e-3)
+        outs.append((X2.get(), tuple(tr)))
+    for got, tr in outs[1:]:
+        assert tr == outs[0][1]
+        np.testing.assert_array_equal(got.nnz, outs[0][0].nnz)
+        np.testing.assert_array_equal(got.index, outs[0][0].index)
+        assert got.value.tobytes() == outs[0][0].value.tobytes()
+    ref, _ = orc.x2(A, 3000, 1e-3)
+    assert_ell_close(outs[0][0], ref, vtol(dtype), 1e-3, "fixed-point x2")
 
 
 @pytest.mark.parametrize("rows,wbits,waves", [(1, 10, 8), (1, 11, 8), (1, 12, 8), (1, 13, 8),
