@@ -429,6 +429,9 @@ def test_import_from_dense(hip, dtype, order):
     bs = hip.block_sizes(40, 40)
     MB = hip._call("bml_hip_get_mb")
     N = int(bs.sum())
+    if N > 4096:
+        pytest.skip(f"the block sizes registered earlier in this process give N={N}: "
+                    "a dense import of N^2 entries is sized for small registries")
     off = np.concatenate([[0], np.cumsum(bs)])
     rng = np.random.default_rng(9)
     D = rng.uniform(-1, 1, (N, N)) * (rng.uniform(0, 1, (N, N)) < 0.2)
