@@ -58,6 +58,8 @@ constexpr int EB_BATCH = BMLHIP_EB_BATCH;
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 struct EbParams {
@@ -86,6 +88,7 @@ struct EbParams {
     int sym;    // x2 of an exactly symmetric X: each row forms the slots it owns (sym_owner)
     int *kflag; // [NB*maxs] sym: kept flag of every slot, set by its row or its mirror
     const __bf16 *A16, *B16;  // bf16 engine: A's tiles row-major, B's transposed (k_eb_bf16)
+    int ka;  // MFMA engines: the first ka A tiles of a block row are staged in LDS
 };
 
 // The symmetric X^2 forms each off-diagonal pair of blocks (i, k), (k, i)
@@ -238,6 +241,7 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
 template <typename T, int BT>
 struct ExactAcc {
     static constexpr bool kPipelined = false;
+    static constexpr bool kStage = false;
     static constexpr int Q = (BT * BT + 63) / 64;
     T a[Q];
     __device__ void zero() {
@@ -287,6 +291,14 @@ struct ExactAcc {
     }
 };
 
+// loads through an explicit address space, global (1) or LDS (3), so that
+// code choosing between a staged and an unstaged tile keeps both kinds of
+// load instead of falling back to flat loads
+template <typename X> __device__ __forceinline__ X gld(const X *a) {
+    return *(const __attribute__((address_space(1))) X *) a;
+}
+template <typename X> using lds_ptr = const __attribute__((address_space(3))) X *;
+
 // MFMA engine, single real, 32 x 32 tiles: v_mfma_f32_32x32x2f32 takes a
 // 32x2 slice of L (lane l: row l%32) and a 2x32 slice of R (lane l: column
 // l%32); the two k of an instruction come from the two lane halves.  Step s
@@ -308,17 +320,38 @@ struct MfmaAccF32 {
     __device__ static void load(const float *L, const float *R, Frag &f) {
         const int lane = lane_id();
         const int i = lane & 31, h = lane >> 5;
-        const float4 *l4 = reinterpret_cast<const float4 *>(L + i * 32 + h * 16);
+        const floatx4 *l4 = reinterpret_cast<const floatx4 *>(L + i * 32 + h * 16);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float4 t = l4[q];
-            f.lv[4 * q] = t.x;
-            f.lv[4 * q + 1] = t.y;
-            f.lv[4 * q + 2] = t.z;
-            f.lv[4 * q + 3] = t.w;
+            const floatx4 t = gld(l4 + q);
+            f.lv[4 * q] = t[0];
+            f.lv[4 * q + 1] = t[1];
+            f.lv[4 * q + 2] = t[2];
+            f.lv[4 * q + 3] = t[3];
         }
 #pragma unroll
-        for (int s = 0; s < 16; s++) f.rv[s] = R[(h * 16 + s) * 32 + i];
+        for (int s = 0; s < 16; s++) f.rv[s] = gld(R + (h * 16 + s) * 32 + i);
+    }
+    // A tile staged in LDS: row r's 16-byte chunk c (8 per row) lives at
+    // chunk c ^ (r & 7), so the 8 lanes that read rows r .. r + 7 at one
+    // chunk hit 8 different 16-byte bank groups
+    using TA = float;
+    static constexpr bool kStage = true;  // left tiles staged in LDS
+    static constexpr int kChunks = 8;     // 16-byte chunks per tile row
+    __device__ static int swz(int r, int c) { return c ^ (r & 7); }
+    __device__ static void load_s(lds_ptr<float> Ls, const float *R, Frag &f) {
+        const int lane = lane_id();
+        const int i = lane & 31, h = lane >> 5;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const floatx4 t = *(lds_ptr<floatx4>) (Ls + i * 32 + 4 * swz(i, 4 * h + q));
+            f.lv[4 * q] = t[0];
+            f.lv[4 * q + 1] = t[1];
+            f.lv[4 * q + 2] = t[2];
+            f.lv[4 * q + 3] = t[3];
+        }
+#pragma unroll
+        for (int s = 0; s < 16; s++) f.rv[s] = gld(R + (h * 16 + s) * 32 + i);
     }
     __device__ void mma(const Frag &f) {
 #pragma unroll
@@ -367,8 +400,23 @@ struct MfmaAccBF16 : MfmaAccF32 {
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int k0 = 16 * q + 8 * h;
-            f.av[q] = *reinterpret_cast<const bf16x8 *>(L + i * 32 + k0);
-            f.bv[q] = *reinterpret_cast<const bf16x8 *>(Rt + i * 32 + k0);
+            f.av[q] = gld(reinterpret_cast<const bf16x8 *>(L + i * 32 + k0));
+            f.bv[q] = gld(reinterpret_cast<const bf16x8 *>(Rt + i * 32 + k0));
+        }
+    }
+    // A tile staged in LDS: row r (64 bytes, 4 chunks) keeps chunk c at
+    // c ^ ((r >> 1) & 3), so rows r .. r + 7 at one chunk fill one 128-byte
+    // span of banks
+    using TA = __bf16;
+    static constexpr int kChunks = 4;
+    __device__ static int swz(int r, int c) { return c ^ ((r >> 1) & 3); }
+    __device__ static void load_s(lds_ptr<__bf16> Ls, const __bf16 *Rt, Frag &f) {
+        const int lane = lane_id();
+        const int i = lane & 31, h = lane >> 5;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            f.av[q] = *(lds_ptr<bf16x8>) (Ls + i * 32 + 8 * swz(i, 2 * q + h));
+            f.bv[q] = gld(reinterpret_cast<const bf16x8 *>(Rt + i * 32 + 16 * q + 8 * h));
         }
     }
     __device__ void mma(const Frag &f) {
@@ -427,18 +475,44 @@ struct MfmaAccF64 {
         const int c = lane & 15, g = lane >> 4;
 #pragma unroll
         for (int ti = 0; ti < 2; ti++) {
-            const double2 *l2 = reinterpret_cast<const double2 *>(L + (ti * 16 + c) * 32 + g * 8);
+            const doublex2 *l2 = reinterpret_cast<const doublex2 *>(L + (ti * 16 + c) * 32 + g * 8);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const double2 t = l2[q];
-                f.lv[ti][2 * q] = t.x;
-                f.lv[ti][2 * q + 1] = t.y;
+                const doublex2 t = gld(l2 + q);
+                f.lv[ti][2 * q] = t[0];
+                f.lv[ti][2 * q + 1] = t[1];
             }
         }
 #pragma unroll
         for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-            for (int s = 0; s < 8; s++) f.rv[tj][s] = R[(g * 8 + s) * 32 + tj * 16 + c];
+            for (int s = 0; s < 8; s++) f.rv[tj][s] = gld(R + (g * 8 + s) * 32 + tj * 16 + c);
+    }
+    // A tile staged in LDS: row r (256 bytes, 16 chunks) keeps chunk c at
+    // c ^ (r & 7)
+    using TA = double;
+    // not staged: two blocks per CU leave room for 5 FP64 tiles, and the
+    // second load path costs the pipelined loop its last registers
+    static constexpr bool kStage = false;
+    static constexpr int kChunks = 16;
+    __device__ static int swz(int r, int c) { return c ^ (r & 7); }
+    __device__ static void load_s(lds_ptr<double> Ls, const double *R, Frag &f) {
+        const int lane = lane_id();
+        const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++) {
+            const int r = ti * 16 + c;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const doublex2 t = *(lds_ptr<doublex2>) (Ls + r * 32 + 2 * swz(r, 4 * g + q));
+                f.lv[ti][2 * q] = t[0];
+                f.lv[ti][2 * q + 1] = t[1];
+            }
+        }
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+            for (int s = 0; s < 8; s++) f.rv[tj][s] = gld(R + (g * 8 + s) * 32 + tj * 16 + c);
     }
     __device__ void mma(const Frag &f) {
 #pragma unroll
@@ -637,8 +711,27 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
     const int *skb = p.skb + (size_t) ib * p.maxs;
     const int nbits = count < p.MC ? count : p.MC;  // slots with a row position
     const int mbits = p.maxs < p.MC ? p.maxs : p.MC;
-    T *stg = (T *) (kbits + ((mbits + 31) / 32 + 4) / 4 * 4) + (size_t) w * BT * LD;
+    T *stg0 = (T *) (kbits + ((mbits + 31) / 32 + 4) / 4 * 4);
+    T *stg = stg0 + (size_t) w * BT * LD;
     for (int q = threadIdx.x; q < (nbits + 31) / 32; q += 64 * NUM_WAVES) kbits[q] = 0u;
+    // MFMA engines: the row's first nst A tiles, copied once into LDS (after
+    // the staging tiles), each row's 16-byte chunks permuted by Acc::swz;
+    // every pair with jp < nst then reads its left operand from LDS (a tile
+    // is used by about as many pairs as its row jb holds blocks)
+    int nst = 0;
+    typedef __attribute__((address_space(3))) unsigned char lds_u8;
+    lds_u8 *ast = (lds_u8 *) (stg0 + (p.sym ? (size_t) NUM_WAVES * BT * LD : 0));
+    if constexpr (Acc::kPipelined && Acc::kStage) {
+        using TA = typename Acc::TA;
+        constexpr int CH = Acc::kChunks, TE = 16 / (int) sizeof(TA);
+        const TA *src = (const TA *) (ENGINE == 2 ? (const void *) p.A16 : p.Av);
+        nst = p.An[ib] < p.ka ? p.An[ib] : p.ka;
+        for (int e = threadIdx.x; e < nst * BT * CH; e += 64 * NUM_WAVES) {
+            const int t = e / (BT * CH), r = e / CH % BT, c = e % CH;
+            const floatx4 v = *reinterpret_cast<const floatx4 *>(src + ((size_t) ib * p.MA + t) * B2 + r * BT + c * TE);
+            *(__attribute__((address_space(3))) floatx4 *) (ast + sizeof(TA) * (t * (int) B2 + r * BT + Acc::swz(r, c) * TE)) = v;
+        }
+    }
     if (threadIdx.x == 0) {
         next_s = 0;
         kept_s = 0;
@@ -787,6 +880,17 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
             if constexpr (ENGINE == 2) return p.B16 + t;
             else return Bv + t;
         };
+        // pair q of unit u into f: the left tile from LDS when staged
+        auto ld = [&](const Unit &u, int q, Frag &f) {
+            if constexpr (Acc::kStage) {
+                const int jp = __builtin_amdgcn_readlane(u.sa, q) - ib * p.MA;
+                if (jp < nst) {
+                    Acc::load_s((lds_ptr<typename Acc::TA>) ast + jp * (int) B2, tile_b(u, q), f);
+                    return;
+                }
+            }
+            Acc::load(tile_a(u, q), tile_b(u, q), f);
+        };
         Unit cu, nu;
         cu.np = 0;
         if (next_unit(cu, cu)) {
@@ -796,13 +900,13 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
             Frag f0, f1;
             int q = 0, slot = -1, kb = -1;
             MirrorProbe mp;
-            Acc::load(tile_a(cu, 0), tile_b(cu, 0), f0);
+            ld(cu, 0, f0);
             // run pair q of cu (in cur) while the pair after it loads into nxt
             auto step = [&](Frag &cur, Frag &nxt) -> bool {
                 if (q + 1 < cu.np)
-                    Acc::load(tile_a(cu, q + 1), tile_b(cu, q + 1), nxt);
+                    ld(cu, q + 1, nxt);
                 else if (have_nu)
-                    Acc::load(tile_a(nu, 0), tile_b(nu, 0), nxt);
+                    ld(nu, 0, nxt);
                 const int sq = __builtin_amdgcn_readlane(cu.sid, q);
                 if (sq != slot) {
                     if (slot >= 0) {
@@ -1252,13 +1356,33 @@ int launch_numeric(const EbParams &p, hipStream_t s) {
         const int mbits = p.maxs < p.MC ? p.maxs : p.MC;
         size_t lds = sizeof(unsigned) * (size_t) ((mbits + 31) / 32 + 4) / 4 * 4;
         if (p.sym) lds += sizeof(T) * (size_t) NUM_WAVES * BT * (BT + 1);
-        if (lds > 64 * 1024) {
-            if (lds > 160 * 1024) return fail_msg(5, "ellblock product: %d block slots per row exceed LDS", p.MC);
+        if (lds > 160 * 1024) return fail_msg(5, "ellblock product: %d block slots per row exceed LDS", p.MC);
+        EbParams q = p;
+        q.ka = 0;
+        if (AccOf<T, ENGINE, BT>::type::kStage) {
+            // A tiles staged in LDS: as many as fit in the block's share of a
+            // CU's LDS (default 78 KB, two blocks per CU), at most the row
+            // width
+            const size_t tb = (size_t) BT * BT * (ENGINE == 2 ? sizeof(__bf16) : sizeof(T));
+            static long budget = -1, cap = -1;
+            if (budget < 0) {
+                const char *e = getenv("BML_HIP_EB_ALDS_KB");
+                budget = 1024L * (e ? atol(e) : 78);
+                e = getenv("BML_HIP_EB_ASTAGE");
+                cap = e ? atol(e) : 1L << 30;
+            }
+            const long b = budget < 156L * 1024 ? budget : 156L * 1024;
+            long ka = b > (long) lds ? (b - (long) lds) / (long) tb : 0;
+            if (ka > cap) ka = cap;
+            if (ka > p.MA) ka = p.MA;
+            q.ka = (int) ka;
+            lds += (size_t) ka * tb;
+        }
+        if (lds > 64 * 1024)
             (void) hipFuncSetAttribute((const void *) k_eb_numeric<T, ENGINE, BT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-        }
         hipLaunchKernelGGL((k_eb_numeric<T, ENGINE, BT>), dim3(p.NB), dim3(64 * NUM_WAVES), lds, s,
-                           p);
+                           q);
         int rc = report("k_eb_numeric");
         if (rc || !p.sym) return rc;
         const int nbits = p.maxs < p.MC ? p.maxs : p.MC;
