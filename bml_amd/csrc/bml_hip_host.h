@@ -40,6 +40,10 @@ int bmlh_sync_mode(void);
 size_t bmlh_elem_size(int precision);
 /* block-product engine for products of A (BML_HIP_BLOCK_ENGINE_*, resolved) */
 int bmlh_block_engine_for(const bml_matrix_ellblock_hip_t *A);
+/* An ELLBLOCK work matrix with undefined tile values (indices and counts
+ * zero), for drivers that write every tile before reading it. */
+bml_matrix_ellblock_hip_t *bmlh_block_matrix_ellblock_work(int precision, int NB, int MB, int M,
+                                                           const int *bsize, int distrib_mode);
 /* the engine set by bml_hip_set_block_engine (BML_HIP_BLOCK_ENGINE_*) */
 int bmlh_block_engine_setting(void);
 /* the real scalar s as one element of precision prec, into buf (16 bytes) */

@@ -263,6 +263,10 @@ int bmlhip_copy_rows_shift(int prec, int N, int MA, const void *d_Av, const int 
                            int col_shift);
 /* values of an ELLPACK-R matrix from one precision to another (same N, M) */
 int bmlhip_convert(int prec_in, int prec_out, size_t count, const void *d_in, void *d_out);
+/* the first d_cnt[r] * unit elements of each of nrows rows, stride elements
+ * apart (the live slots of a matrix) */
+int bmlhip_convert_rows(int prec_in, int prec_out, int nrows, size_t stride, size_t unit,
+                        const int *d_cnt, const void *d_in, void *d_out);
 /* A <- A + beta*I then threshold (add_identity); scale: values *= s first
  * (s_bits points to one element of the precision; NULL = no scaling). */
 int bmlhip_scale_add_identity(int prec, int N, int M, void *d_Av, int *d_Ai,

@@ -165,8 +165,8 @@ static void eb_finish_into(const char *what, bml_matrix_ellblock_hip_t *out)
 
 /* ---- allocation ------------------------------------------------------------ */
 
-bml_matrix_ellblock_hip_t *bml_block_matrix_ellblock_hip(int precision, int NB, int MB, int M,
-                                                         const int *bsize, int distrib_mode)
+static bml_matrix_ellblock_hip_t *eb_alloc(int precision, int NB, int MB, int M,
+                                           const int *bsize, int distrib_mode, int zero_values)
 {
     const char *what = "bml_block_matrix_ellblock_hip";
     if (bmlh_ready())
@@ -221,13 +221,29 @@ bml_matrix_ellblock_hip_t *bml_block_matrix_ellblock_hip(int precision, int NB, 
         bml_deallocate_ellblock_hip(A);
         return NULL;
     }
-    bmlhip_zero(A->value, slots * tile_bytes(A));
+    if (zero_values)
+        bmlhip_zero(A->value, slots * tile_bytes(A));
     bmlhip_zero(A->indexb, slots * sizeof(int));
     bmlhip_zero(A->nnzb, (size_t) NB * sizeof(int));
     A->aux->stats_valid = 1;
     if (s_mb == 0)
         bml_hip_set_block_sizes(bsize, NB, MB);
     return A;
+}
+
+bml_matrix_ellblock_hip_t *bml_block_matrix_ellblock_hip(int precision, int NB, int MB, int M,
+                                                         const int *bsize, int distrib_mode)
+{
+    return eb_alloc(precision, NB, MB, M, bsize, distrib_mode, 1);
+}
+
+/* A work matrix whose tile values start undefined (block counts zero): for
+ * drivers that write every tile they later read, so the NB*MB tiles of a
+ * wide matrix are not cleared on every call */
+bml_matrix_ellblock_hip_t *bmlh_block_matrix_ellblock_work(int precision, int NB, int MB, int M,
+                                                           const int *bsize, int distrib_mode)
+{
+    return eb_alloc(precision, NB, MB, M, bsize, distrib_mode, 0);
 }
 
 bml_matrix_ellblock_hip_t *bml_zero_matrix_ellblock_hip(int precision, int N, int M,

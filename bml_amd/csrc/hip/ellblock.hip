@@ -439,10 +439,14 @@ __global__ __launch_bounds__(256) void k_eb_bf16(int M, const float *V, const in
     const int n = nnzb[ib];
     for (int jp = 0; jp < n; jp++) {
         const size_t base = ((size_t) ib * M + jp) * 1024;
-        for (int e = threadIdx.x; e < 1024; e += 256) {
-            const float v = V[base + e];
-            if (rm) rm[base + e] = (__bf16) v;
-            t[e >> 5][e & 31] = v;
+        float v[4];  // the four loads in flight together
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = V[base + threadIdx.x + 256 * q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (rm) rm[base + e] = (__bf16) v[q];
+            t[e >> 5][e & 31] = v[q];
         }
         __syncthreads();
         if (tr)
@@ -1034,7 +1038,7 @@ struct EbAddParams {
     bmlhip_status *st;
 };
 
-// One block row per block: lane 0 lists the union (A's blocks, then B's
+// One block row per block: the block lists the union (A's blocks, then B's
 // new ones, as bml_add_ellblock_typed.c:35-197), then EB_ADD_WAVES
 // wavefronts take consecutive union entries, one each per round: every
 // wavefront reads its two tiles and forms alpha*A + beta*B with the block's
@@ -1058,7 +1062,7 @@ template <typename T, int BT>
 __global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
     using E = Elem<T>;
     extern __shared__ int usm[];
-    __shared__ int keep_s[EB_ADD_WAVES * EB_ADD_PER];
+    __shared__ int keep_s[2][EB_ADD_WAVES * EB_ADD_PER];  // by round parity
     __shared__ double nrm_s[EB_ADD_WAVES];
     if (p.skip && *p.skip) return;
     const int ib = blockIdx.x;
@@ -1073,40 +1077,63 @@ __global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
     T *Av = (T *) p.Av;
     const T *Bv = (const T *) p.Bv;
     constexpr size_t B2 = (size_t) BT * BT;
-    // union, thread 0 (rows hold at most MA + MB block columns)
-    if (threadIdx.x == 0) {
-        int n = 0;
-        for (int jp = 0; jp < nA; jp++) {
-            ucol[n] = p.Ai[(size_t) ib * p.MA + jp];
-            ua[n] = jp;
-            ub[n] = -1;
-            n++;
-        }
-        for (int kp = 0; kp < nB; kp++) {
-            const int kb = p.Bi[(size_t) ib * p.MB + kp];
-            int u = 0;
-            while (u < nA && ucol[u] != kb) u++;
-            if (u < nA) {
-                ub[u] = kp;
-            } else {
-                ucol[n] = kb;
-                ua[n] = -1;
-                ub[n] = kp;
-                n++;
-            }
-        }
-        usm[3 * cap] = n;
+    // union (rows hold at most MA + MB block columns): A's list copied,
+    // then each thread takes one of B's entries, looks for its column in A's
+    // list (an LDS scan, every lane reading the same word) and the columns A
+    // lacks are appended in B's order, ranked by a block-wide prefix count
+    __shared__ int wcnt_s[EB_ADD_WAVES];
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    for (int jp = threadIdx.x; jp < nA; jp += 64 * EB_ADD_WAVES) {
+        ucol[jp] = p.Ai[(size_t) ib * p.MA + jp];
+        ua[jp] = jp;
+        ub[jp] = -1;
     }
     __syncthreads();
-    const int n = usm[3 * cap];
+    int n = nA;  // block-uniform
+    for (int k0 = 0; k0 < nB; k0 += 64 * EB_ADD_WAVES) {
+        const int kp = k0 + (int) threadIdx.x;
+        int kb = -1, u = -1;
+        if (kp < nB) {
+            kb = p.Bi[(size_t) ib * p.MB + kp];
+            for (int v = 0; v < nA; v++)
+                if (ucol[v] == kb) {
+                    u = v;
+                    break;
+                }
+        }
+        const bool isnew = kp < nB && u < 0;
+        const uint64_t m = ballot64(isnew);
+        if (lane == 0) wcnt_s[w] = popc64(m);
+        if (u >= 0) ub[u] = kp;
+        __syncthreads();
+        int pos = n, tot = 0;
+        for (int q = 0; q < EB_ADD_WAVES; q++) {
+            pos += q < w ? wcnt_s[q] : 0;
+            tot += wcnt_s[q];
+        }
+        if (isnew) {
+            pos += popc64(m & ltmask);
+            ucol[pos] = kb;
+            ua[pos] = -1;
+            ub[pos] = kp;
+        }
+        n += tot;
+        __syncthreads();  // wcnt_s is reused; the list is complete after the last chunk
+    }
     const int bi = p.bsize[ib];
-    constexpr int Q = (BT * BT + 63) / 64;
+    static_assert(BT * BT % 64 == 0, "a tile is whole rows of 64 lanes");
+    constexpr int Q = BT * BT / 64;
     int ll = 0;
     double nrm2 = 0.0;
     // each round: EB_ADD_PER union entries per wavefront, entry u0 + r with
     // r = k EB_ADD_WAVES + w; all their tiles are read (and loads of the
-    // round's entries overlap) before any kept tile is written
-    for (int u0 = 0; u0 < n; u0 += EB_ADD_WAVES * EB_ADD_PER) {
+    // round's entries overlap) before any kept tile is written.  A kept
+    // tile of entry u goes to position <= u, below every A slot a later
+    // round reads, so a wavefront starts its next round's reads as soon as
+    // its own writes are issued (the keep flags alternate between two
+    // buffers, so the next round's flags do not overwrite unread ones)
+    int par = 0;
+    for (int u0 = 0; u0 < n; u0 += EB_ADD_WAVES * EB_ADD_PER, par ^= 1) {
         T x[EB_ADD_PER][Q];
         bool keep[EB_ADD_PER];
         int jbk[EB_ADD_PER];
@@ -1118,21 +1145,38 @@ __global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
             const int bj = p.bsize[jb];
             jbk[k] = jb;
             double m = 0.0;
+            // both tiles' loads issued together (sa, sb are wave-uniform; BT^2
+            // is a multiple of 64), the block's bsize[ib] x bsize[jb] corner
+            // selected afterwards
+            T av[Q], bv[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                av[q] = E::zero();
+                bv[q] = E::zero();
+            }
+            if (sa >= 0) {
+                const T *ta = Av + ((size_t) ib * p.MA + sa) * B2;
+#pragma unroll
+                for (int q = 0; q < Q; q++) av[q] = ta[lane + 64 * q];
+            }
+            if (sb >= 0) {
+                const T *tb = Bv + ((size_t) ib * p.MB + sb) * B2;
+#pragma unroll
+                for (int q = 0; q < Q; q++) bv[q] = tb[lane + 64 * q];
+            }
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const int e = lane + 64 * q;
                 x[k][q] = E::zero();
                 T y = E::zero();
-                if (e < BT * BT && e / BT < bi && e % BT < bj) {
+                if (e / BT < bi && e % BT < bj) {
                     if (sa >= 0) {
-                        const T a = Av[((size_t) ib * p.MA + sa) * B2 + e];
-                        x[k][q] = E::axpy(x[k][q], alpha, a);
-                        y = a;
+                        x[k][q] = E::axpy(x[k][q], alpha, av[q]);
+                        y = av[q];
                     }
                     if (sb >= 0) {
-                        const T b = Bv[((size_t) ib * p.MB + sb) * B2 + e];
-                        x[k][q] = E::axpy(x[k][q], beta, b);
-                        y = E::sub(y, b);
+                        x[k][q] = E::axpy(x[k][q], beta, bv[q]);
+                        y = E::sub(y, bv[q]);
                     }
                 }
                 const double mg = E::mag(x[k][q]);
@@ -1141,16 +1185,16 @@ __global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
             }
             m = wave_max(m);
             keep[k] = live && E::above(E::from_real(m), p.thr);
-            if (lane == 0) keep_s[k * EB_ADD_WAVES + w] = keep[k] ? 1 : 0;
+            if (lane == 0) keep_s[par][k * EB_ADD_WAVES + w] = keep[k] ? 1 : 0;
         }
         __syncthreads();  // every tile of the round read; keep flags in
         int tot = 0;
-        for (int r = 0; r < EB_ADD_WAVES * EB_ADD_PER; r++) tot += keep_s[r];
+        for (int r = 0; r < EB_ADD_WAVES * EB_ADD_PER; r++) tot += keep_s[par][r];
 #pragma unroll
         for (int k = 0; k < EB_ADD_PER; k++) {
             if (!keep[k]) continue;
             int pos = ll;
-            for (int r = 0; r < k * EB_ADD_WAVES + w; r++) pos += keep_s[r];
+            for (int r = 0; r < k * EB_ADD_WAVES + w; r++) pos += keep_s[par][r];
             if (pos < p.MA) {
                 const size_t d = ((size_t) ib * p.MA + pos) * B2;
 #pragma unroll
@@ -1165,7 +1209,6 @@ __global__ __launch_bounds__(64 * EB_ADD_WAVES) void k_eb_add(EbAddParams p) {
             }
         }
         ll += tot;
-        __syncthreads();  // writes of the round done before the next reads
     }
     for (int off = 32; off > 0; off >>= 1) nrm2 += __shfl_down(nrm2, off, 64);
     if (lane == 0) nrm_s[w] = nrm2;

@@ -86,6 +86,25 @@ __global__ __launch_bounds__(256) void k_convert(size_t n, const A *in, B *out) 
         out[k] = cvt<A, B>(in[k]);
 }
 
+// the first cnt[r] * unit elements of each row r (rows `stride` elements
+// apart): the live slots of an ELLPACK or ELLBLOCK matrix; one block per row
+template <typename A, typename B>
+__global__ __launch_bounds__(256) void k_convert_rows(size_t stride, size_t unit, const int *cnt,
+                                                      const A *in, B *out) {
+    const size_t r = blockIdx.x, n = (size_t) cnt[r] * unit;
+    const A *x = in + r * stride;
+    B *y = out + r * stride;
+    size_t k = threadIdx.x;
+    for (; k + 768 < n; k += 1024) {  // four loads in flight per thread
+        A v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = x[k + 256 * q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) y[k + 256 * q] = cvt<A, B>(v[q]);
+    }
+    for (; k < n; k += 256) y[k] = cvt<A, B>(x[k]);
+}
+
 int launch_error(const char *what) {
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) return 0;
@@ -149,6 +168,25 @@ int bmlhip_convert(int prec_in, int prec_out, size_t count, const void *in, void
         return 2;
     }
     return launch_error("k_convert");
+}
+
+int bmlhip_convert_rows(int prec_in, int prec_out, int nrows, size_t stride, size_t unit,
+                        const int *d_cnt, const void *in, void *out) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    if (nrows <= 0) return 0;
+#define CVR(A, B)                                                                            \
+    hipLaunchKernelGGL((k_convert_rows<A, B>), dim3(nrows), dim3(256), 0, s, stride, unit, d_cnt, \
+                       (const A *) in, (B *) out)
+    if (prec_in == 1 && prec_out == 2) CVR(float, double);
+    else if (prec_in == 2 && prec_out == 1) CVR(double, float);
+    else if (prec_in == 3 && prec_out == 4) CVR(cf32, cf64);
+    else if (prec_in == 4 && prec_out == 3) CVR(cf64, cf32);
+    else {
+        bmlhip_set_errmsg("unsupported precision conversion");
+        return 2;
+    }
+#undef CVR
+    return launch_error("k_convert_rows");
 }
 
 }  // extern "C"

@@ -743,9 +743,10 @@ static int eb_sp2_iteration(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hi
 static int eb_convert(const bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_t *B,
                       const char *what)
 {
-    const size_t slots = (size_t) A->NB * A->MB;
-    return bmlh_ok(bmlhip_convert(A->matrix_precision, B->matrix_precision,
-                                  slots * A->bt * A->bt, A->value, B->value), what) &&
+    const size_t slots = (size_t) A->NB * A->MB, tile = (size_t) A->bt * A->bt;
+    return bmlh_ok(bmlhip_convert_rows(A->matrix_precision, B->matrix_precision, A->NB,
+                                       (size_t) A->MB * tile, tile, A->nnzb, A->value, B->value),
+                   what) &&
            bmlh_ok(bmlhip_d2d(B->indexb, A->indexb, slots * sizeof(int)), what) &&
            bmlh_ok(bmlhip_d2d(B->nnzb, A->nnzb, (size_t) A->NB * sizeof(int)), what);
 }
@@ -805,7 +806,8 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
     bml_matrix_ellblock_hip_t *X2 = NULL, *Xs = NULL, *X2s = NULL;
     bmlhip_sp2_state *d_state = NULL;
     double *d_hist = NULL;
-    X2 = bml_block_matrix_ellblock_hip(prec, NB, MB, X->M, X->bsize_host, X->distribution_mode);
+    /* work matrices: every tile they hold is written before it is read */
+    X2 = bmlh_block_matrix_ellblock_work(prec, NB, MB, X->M, X->bsize_host, X->distribution_mode);
     if (!X2)
         goto out;
     if (!bmlh_ok(bmlhip_malloc((void **) &d_state, sizeof(bmlhip_sp2_state)), what) ||
@@ -816,10 +818,10 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
 
     int single = 0;
     if (o.mixed && o.maxit > o.refine_iters) {
-        Xs = bml_block_matrix_ellblock_hip(sprec, NB, MB, X->M, X->bsize_host,
-                                           X->distribution_mode);
-        X2s = bml_block_matrix_ellblock_hip(sprec, NB, MB, X->M, X->bsize_host,
-                                            X->distribution_mode);
+        Xs = bmlh_block_matrix_ellblock_work(sprec, NB, MB, X->M, X->bsize_host,
+                                             X->distribution_mode);
+        X2s = bmlh_block_matrix_ellblock_work(sprec, NB, MB, X->M, X->bsize_host,
+                                              X->distribution_mode);
         if (!Xs || !X2s || !eb_convert(X, Xs, what))
             goto out;
         single = 1;
