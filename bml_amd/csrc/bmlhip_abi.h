@@ -307,8 +307,17 @@ typedef struct bmlhip_eb_args {
     const int *d_skip;  /* if not NULL and *d_skip != 0 the kernels do nothing */
     int sym;            /* x2 on an MFMA engine of an exactly symmetric (Hermitian)
                          * X: form the blocks kb >= ib, mirror the others */
+    /* fused SP2 update (with sym): C <- alpha X + beta (X^2 thresholded),
+     * alpha, beta from d_ab; blocks kept iff max |element| > threshold (the
+     * diagonal block always); |X - X^2|^2 -> scalars[2]; no traces */
+    int fuse;
+    const double *d_ab;
 } bmlhip_eb_args;
 int bmlhip_eb_product(const bmlhip_eb_args *a);
+/* tr X -> scalars[0] and the sum of |X_ij|^2 (tr X^2 of a symmetric or
+ * Hermitian X) -> scalars[1] */
+int bmlhip_eb_trace_frob(int prec, int NB, int bt, int MB, const void *d_Av, const int *d_Ai,
+                         const int *d_An, const int *d_skip);
 /* *symmetric = 1 iff every stored block (ib, jb) has a stored (jb, ib) whose
  * tile is its conjugate transpose, bit for bit */
 int bmlhip_eb_is_symmetric(int prec, int NB, int bt, int MB, const void *d_Av, const int *d_Ai,

@@ -89,6 +89,12 @@ struct EbParams {
     int *kflag; // [NB*maxs] sym: kept flag of every slot, set by its row or its mirror
     const __bf16 *A16, *B16;  // bf16 engine: A's tiles row-major, B's transposed (k_eb_bf16)
     int ka;  // MFMA engines: the first ka A tiles of a block row are staged in LDS
+    // fused SP2 update (sym x2 on an MFMA engine): C <- ab[0] X + ab[1] X^2,
+    // A's block columns in LDS at byte acol_off of the dynamic area, the
+    // rows' |X - X^2|^2 into rownorm
+    int fuse, acol_off;
+    const double *ab;
+    double *rownorm;
 };
 
 // The symmetric X^2 forms each off-diagonal pair of blocks (i, k), (k, i)
@@ -725,6 +731,16 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
     int nst = 0;
     typedef __attribute__((address_space(3))) unsigned char lds_u8;
     lds_u8 *ast = (lds_u8 *) (stg0 + (p.sym ? (size_t) NUM_WAVES * BT * LD : 0));
+    // fused update: A's block columns of this row in LDS, alpha and beta
+    typedef __attribute__((address_space(3))) int lds_int;
+    lds_int *acol = (lds_int *) ((unsigned char *) kbits + p.acol_off);
+    const int nAr = p.An[ib];
+    double alpha = 0.0, beta = 0.0, nrm_l = 0.0;
+    if (p.fuse) {
+        alpha = p.ab[0];
+        beta = p.ab[1];
+        for (int q = threadIdx.x; q < nAr; q += 64 * NUM_WAVES) acol[q] = p.Ai[(size_t) ib * p.MA + q];
+    }
     if constexpr (Acc::kPipelined && Acc::kStage) {
         using TA = typename Acc::TA;
         constexpr int CH = Acc::kChunks, TE = 16 / (int) sizeof(TA);
@@ -741,7 +757,7 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
         kept_s = 0;
     }
     T tx = E::zero(), tx2 = E::zero();
-    if (p.kind == 0 && threadIdx.x == 0) {
+    if (p.kind == 0 && !p.fuse && threadIdx.x == 0) {
         // tr X: the diagonal of every diagonal block met in the expand phase
         for (int jp = 0; jp < p.An[ib]; jp++) {
             const size_t sa = (size_t) ib * p.MA + jp;
@@ -763,10 +779,10 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
         }
     };
     // keep rule, tr X^2 and the stores of slot s's finished tile
-    auto finish_slot = [&](const Acc &acc, int s, int kb, const MirrorProbe &mp) {
+    auto finish_slot = [&](const Acc &acc, int s, int kb, const MirrorProbe &mp) __attribute__((always_inline)) {
         const double nrm = wave_max(acc.maxmag());
         const bool keep = kb == ib || nrm > p.thr;
-        if (p.kind == 0 && kb == ib) {
+        if (p.kind == 0 && !p.fuse && kb == ib) {
             // tr X^2: the diagonal block's diagonal, summed in order by lane 0
             acc.diag(diag_s[w]);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -791,15 +807,92 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
         }
         // sym: the tile at (ib, s) and its transpose at (kb, s')
         const int sp = kb != ib ? mirror_slot(p, mp, kb, ib) : -1;
-        if (lane == 0) {
-            p.kflag[(size_t) ib * p.maxs + s] = keep;
-            if (sp >= 0) p.kflag[(size_t) kb * p.maxs + sp] = keep;
-        }
-        if (keep && (s < p.MC || (sp >= 0 && sp < p.MC))) {
-            acc.store(stg, LD);
+        auto wave_sync = [&]() {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        bool kept = keep;
+        if (p.fuse) {
+            // U = alpha X[ib, kb] + beta Y, Y the product's block when the x2
+            // rule keeps it (else 0), formed in the staging tile; U kept iff
+            // it is the diagonal block or max |U| > thr (the add's rule;
+            // every X block is reached through its row's diagonal block, so
+            // none is left out); |X - Y|^2 counted twice off the diagonal
+            // (the mirror block)
+            acc.store(stg, LD);
+            wave_sync();
+            if (kb == ib) {  // the diagonal block takes its upper triangle
+                for (int e = lane; e < (int) B2; e += 64) {
+                    const int r = e / BT, c = e % BT;
+                    if (r > c) stg[r * LD + c] = E::conj(stg[c * LD + r]);
+                }
+                wave_sync();
+            }
+            int jx = -1;
+            for (int c0 = 0; c0 < nAr; c0 += 64) {
+                const uint64_t b = ballot64(c0 + lane < nAr && acol[c0 + lane] == kb);
+                if (b) {
+                    jx = c0 + __ffsll((unsigned long long) b) - 1;
+                    break;
+                }
+            }
+            // X's block in chunks of a lane's elements (all of them for 4-byte
+            // elements, two at a time for 8-byte ones, whose kernel has no
+            // registers to spare), each chunk's loads issued before any is
+            // used (from the LDS stage when it holds the exact tile)
+            constexpr int QE = (int) (B2 / 64);
+            static_assert(B2 % 64 == 0, "a tile is whole rows of 64 lanes");
+            constexpr int CH = sizeof(T) <= 4 ? QE : (QE >= 2 ? 2 : QE);
+            bool staged = false;
+            if constexpr (Acc::kStage) {
+                using TA = typename Acc::TA;
+                if constexpr (std::is_same<TA, T>::value) staged = jx >= 0 && jx < nst;
+            }
+            const T *xg = Av + ((size_t) ib * p.MA + (jx >= 0 ? jx : 0)) * B2;
+            double mu = 0.0, dn = 0.0;
+#pragma unroll 1
+            for (int q0 = 0; q0 < QE; q0 += CH) {
+                T xv[CH];
+#pragma unroll
+                for (int j = 0; j < CH; j++) {
+                    const int e = lane + 64 * (q0 + j);
+                    xv[j] = E::zero();
+                    if constexpr (Acc::kStage) {
+                        using TA = typename Acc::TA;
+                        if constexpr (std::is_same<TA, T>::value) {
+                            constexpr int TE = 16 / (int) sizeof(TA);
+                            const int r = e / BT, c = e % BT;
+                            if (staged)
+                                xv[j] = *((lds_ptr<TA>) ast + jx * (int) B2 + r * BT +
+                                          Acc::swz(r, c / TE) * TE + c % TE);
+                        }
+                    }
+                    if (!staged && jx >= 0) xv[j] = xg[e];
+                }
+#pragma unroll
+                for (int j = 0; j < CH; j++) {
+                    const int e = lane + 64 * (q0 + j), r = e / BT, c = e % BT;
+                    const T y = keep ? stg[r * LD + c] : E::zero();
+                    const T u = E::axpy(E::axpy(E::zero(), alpha, xv[j]), beta, y);
+                    stg[r * LD + c] = u;
+                    mu = fmax(mu, E::mag(u));
+                    dn += E::abs2(E::sub(xv[j], y));
+                }
+            }
+            nrm_l += kb == ib ? dn : 2.0 * dn;
+            kept = kb == ib || wave_max(mu) > p.thr;
+            wave_sync();
+        }
+        if (lane == 0) {
+            p.kflag[(size_t) ib * p.maxs + s] = kept;
+            if (sp >= 0) p.kflag[(size_t) kb * p.maxs + sp] = kept;
+        }
+        if (kept && (s < p.MC || (sp >= 0 && sp < p.MC))) {
+            if (!p.fuse) {
+                acc.store(stg, LD);
+                wave_sync();
+            }
             if (s < p.MC) {
                 const size_t sc = (size_t) ib * p.MC + s;
                 T *D = Cv + sc * B2;
@@ -961,8 +1054,18 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
     if (!p.sym) compact_kept<T, BT>(p, ib, nbits, kbits);
     // tx2 lives in lane 0 of the wavefront that formed the diagonal block
     __shared__ double tr2_s[NUM_WAVES];
-    if (lane == 0) tr2_s[w] = E::re(tx2);
+    if (p.fuse)
+        for (int off = 32; off > 0; off >>= 1) nrm_l += __shfl_xor(nrm_l, off, 64);
+    if (lane == 0) tr2_s[w] = p.fuse ? nrm_l : E::re(tx2);
     __syncthreads();
+    if (p.fuse) {
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int v = 0; v < NUM_WAVES; v++) t += tr2_s[v];
+            p.rownorm[ib] = t;
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
         if (!p.sym) {
             const int kept = kept_s;
@@ -1289,6 +1392,53 @@ __global__ __launch_bounds__(256) void k_eb_trace_rows(int NB, int MB, const int
     rowtr[ib] = s;
 }
 
+// SP2 with the update fused into the product: per block row, tr X (the
+// diagonal of its diagonal block) and the sum of |X_ij|^2 over its blocks,
+// which is the row's share of tr X^2 when X is symmetric (Hermitian); in
+// double, four elements per thread per tile with a fixed reduction tree
+template <typename T, int BT>
+__global__ __launch_bounds__(256) void k_eb_trace_frob(int NB, int MB, const T *Av, const int *Ai,
+                                                       const int *An, double *rowtr,
+                                                       const int *skip) {
+    __shared__ double part[2][256];
+    if (skip && *skip) return;  // uniform
+    const int ib = blockIdx.x;
+    constexpr int B2 = BT * BT;
+    const int n = An[ib];
+    double tr = 0.0, fr = 0.0;
+    for (int jp = 0; jp < n; jp++) {
+        const size_t sl = (size_t) ib * MB + jp;
+        const T *t = Av + sl * B2;
+        const bool dg = Ai[sl] == ib;
+        T v[(B2 + 255) / 256];
+#pragma unroll
+        for (int q = 0; q < (B2 + 255) / 256; q++) {
+            const int e = threadIdx.x + 256 * q;
+            v[q] = e < B2 ? t[e] : Elem<T>::zero();
+        }
+#pragma unroll
+        for (int q = 0; q < (B2 + 255) / 256; q++) {
+            const int e = threadIdx.x + 256 * q;
+            fr += Elem<T>::abs2(v[q]);
+            if (dg && e < B2 && e / BT == e % BT) tr += Elem<T>::re(v[q]);
+        }
+    }
+    part[0][threadIdx.x] = tr;
+    part[1][threadIdx.x] = fr;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int) threadIdx.x < w) {
+            part[0][threadIdx.x] += part[0][threadIdx.x + w];
+            part[1][threadIdx.x] += part[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rowtr[ib] = part[0][0];
+        rowtr[NB + ib] = part[1][0];
+    }
+}
+
 // copy block rows between matrices of different MB
 template <typename T, int BT>
 __global__ __launch_bounds__(256) void k_eb_copy(int NB, int MA, const T *Av, const int *Ai,
@@ -1402,6 +1552,8 @@ int launch_numeric(const EbParams &p, hipStream_t s) {
         if (lds > 160 * 1024) return fail_msg(5, "ellblock product: %d block slots per row exceed LDS", p.MC);
         EbParams q = p;
         q.ka = 0;
+        // fused update: A's block columns after the A tiles
+        const size_t acol_bytes = p.fuse ? ((size_t) p.MA * sizeof(int) + 15) / 16 * 16 : 0;
         if (AccOf<T, ENGINE, BT>::type::kStage) {
             // A tiles staged in LDS: as many as fit in the block's share of a
             // CU's LDS (default 78 KB, two blocks per CU), at most the row
@@ -1415,12 +1567,16 @@ int launch_numeric(const EbParams &p, hipStream_t s) {
                 cap = e ? atol(e) : 1L << 30;
             }
             const long b = budget < 156L * 1024 ? budget : 156L * 1024;
-            long ka = b > (long) lds ? (b - (long) lds) / (long) tb : 0;
+            const long used = (long) (lds + acol_bytes);
+            long ka = b > used ? (b - used) / (long) tb : 0;
             if (ka > cap) ka = cap;
             if (ka > p.MA) ka = p.MA;
             q.ka = (int) ka;
             lds += (size_t) ka * tb;
         }
+        q.acol_off = (int) lds;
+        lds += acol_bytes;
+        if (lds > 160 * 1024) return fail_msg(5, "ellblock product: %d block slots per row exceed LDS", p.MA);
         if (lds > 64 * 1024)
             (void) hipFuncSetAttribute((const void *) k_eb_numeric<T, ENGINE, BT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
@@ -1480,6 +1636,7 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     p.maxp = (int) maxp;
     const size_t NB = (size_t) a->NB;
     p.rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (NB + 1));
+    p.rownorm = p.rowtr;  // fused update: no traces, the rows' norms instead
     p.cnt = (int *) bmlhip_scratch(3, sizeof(int) * (NB + 1));
     p.skb = (int *) bmlhip_scratch(4, sizeof(int) * (NB * maxs + 1));
     p.soff = (int *) bmlhip_scratch(5, sizeof(int) * (NB * (maxs + 1) + 1));
@@ -1489,6 +1646,9 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     if ((long long) a->NB * a->MA >= (1ll << 31) || (long long) a->NB * a->MB >= (1ll << 31))
         return fail_msg(5, "ellblock product: more than 2^31 block slots");
     p.sym = a->kind == 0 && a->sym && a->engine != 0;
+    p.fuse = p.sym && a->fuse;
+    p.ab = a->d_ab;
+    if (p.fuse && !p.ab) return fail_msg(2, "ellblock product: the fused update needs alpha, beta");
     if (p.sym && !(p.kflag = (int *) bmlhip_scratch(1, sizeof(int) * (NB * maxs + 1))))
         return fail_msg(4, "ellblock product: scratch allocation failed");
     p.rowstat = (int4 *) bmlhip_scratch(BMLHIP_ROWSTAT_SLOT, sizeof(int4) * (NB + 1));
@@ -1545,6 +1705,7 @@ int eb_product_typed(const bmlhip_eb_args *a) {
                                     BMLHIP_RS_OVF_DIST | BMLHIP_RS_STORED | BMLHIP_RS_PROF,
                                     a->prec, p.skip)))
         return rc;
+    if (p.fuse) return bmlhip_sum_to_status(p.rownorm, a->NB, 2);
     if (a->kind == 0) {
         if ((rc = bmlhip_sum_to_status(p.rowtr, a->NB, 0))) return rc;
         if ((rc = bmlhip_sum_to_status(p.rowtr + a->NB, a->NB, 1))) return rc;
@@ -1631,6 +1792,25 @@ int eb_trace_typed(int NB, int bt, const int *bsize, int MB, const void *Av, con
     }
     if ((rc = report("k_eb_trace_rows"))) return rc;
     return bmlhip_sum_to_status(rowtr, NB, 0);
+}
+
+template <typename T>
+int eb_trace_frob_typed(int NB, int bt, int MB, const void *Av, const int *Ai, const int *An,
+                        const int *skip) {
+    hipStream_t s = (hipStream_t) bmlhip_stream();
+    double *rowtr = (double *) bmlhip_scratch(0, sizeof(double) * (2 * (size_t) NB + 1));
+    if (!rowtr) return fail_msg(4, "ellblock trace: scratch allocation failed");
+    if (NB == 0) return 0;
+    switch (bt) {
+    case 8: hipLaunchKernelGGL((k_eb_trace_frob<T, 8>), dim3(NB), dim3(256), 0, s, NB, MB, (const T *) Av, Ai, An, rowtr, skip); break;
+    case 16: hipLaunchKernelGGL((k_eb_trace_frob<T, 16>), dim3(NB), dim3(256), 0, s, NB, MB, (const T *) Av, Ai, An, rowtr, skip); break;
+    case 32: hipLaunchKernelGGL((k_eb_trace_frob<T, 32>), dim3(NB), dim3(256), 0, s, NB, MB, (const T *) Av, Ai, An, rowtr, skip); break;
+    default: return fail_msg(2, "tile edge %d not one of 8/16/32", bt);
+    }
+    int rc = report("k_eb_trace_frob");
+    if (rc) return rc;
+    if ((rc = bmlhip_sum_to_status(rowtr, NB, 0))) return rc;
+    return bmlhip_sum_to_status(rowtr + NB, NB, 1);
 }
 
 template <typename T>
@@ -1816,6 +1996,13 @@ int bmlhip_eb_threshold(int prec, int NB, int bt, const int *bsize, int MB, void
 int bmlhip_eb_trace(int prec, int NB, int bt, const int *bsize, int MB, const void *Av,
                     const int *Ai, const int *An) {
 #define CALL(T) eb_trace_typed<T>(NB, bt, bsize, MB, Av, Ai, An)
+    DISPATCH(prec, CALL)
+#undef CALL
+}
+
+int bmlhip_eb_trace_frob(int prec, int NB, int bt, int MB, const void *Av, const int *Ai,
+                         const int *An, const int *skip) {
+#define CALL(T) eb_trace_frob_typed<T>(NB, bt, MB, Av, Ai, An, skip)
     DISPATCH(prec, CALL)
 #undef CALL
 }

@@ -738,6 +738,56 @@ static int eb_sp2_iteration(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hi
            bmlh_ok(bmlhip_sp2_check(d_state, o->tol, d_hist, o->maxit, 0), what);
 }
 
+/* One iteration with the update fused into the product (MFMA engines, an
+ * exactly symmetric X): tr X and tr X^2 = sum |X_ij|^2 come first, so the
+ * branch is decided before the product, which then writes
+ * alpha X + beta X^2 into X2 and reduces |X - X^2|^2; the caller swaps the
+ * two matrices' buffers. */
+static int eb_sp2_iteration_fused(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hip_t *X2,
+                                  bmlhip_sp2_state *d_state, double *d_hist,
+                                  const bml_hip_sp2_options *o, double nocc, const char *what)
+{
+    bmlhip_eb_args a;
+    memset(&a, 0, sizeof a);
+    a.prec = X->matrix_precision;
+    a.kind = 0;
+    a.NB = X->NB;
+    a.bt = X->bt;
+    a.d_bsize = X->bsize;
+    a.MA = a.MB = X->MB;
+    a.MC = X2->MB;
+    a.d_Av = a.d_Bv = X->value;
+    a.d_Ai = a.d_Bi = X->indexb;
+    a.d_An = a.d_Bn = X->nnzb;
+    a.d_Cv = X2->value;
+    a.d_Ci = X2->indexb;
+    a.d_Cn = X2->nnzb;
+    a.threshold = o->threshold;
+    a.engine = bmlh_block_engine_for(X);
+    a.d_skip = &d_state->done;
+    a.sym = 1;
+    a.fuse = 1;
+    a.d_ab = d_state->ab;
+    return bmlh_ok(bmlhip_eb_trace_frob(X->matrix_precision, X->NB, X->bt, X->MB, X->value,
+                                        X->indexb, X->nnzb, &d_state->done),
+                   what) &&
+           bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
+           bmlh_ok(bmlhip_eb_product(&a), what) &&
+           bmlh_ok(bmlhip_sp2_check(d_state, o->tol, d_hist, o->maxit, 1), what);
+}
+
+static void eb_swap_buffers(bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_t *B)
+{
+    void *v = A->value;
+    int *i = A->indexb, *n = A->nnzb;
+    A->value = B->value;
+    A->indexb = B->indexb;
+    A->nnzb = B->nnzb;
+    B->value = v;
+    B->indexb = i;
+    B->nnzb = n;
+}
+
 /* values, block columns and counts of an ELLBLOCK matrix into one of the
  * same shape in another precision */
 static int eb_convert(const bml_matrix_ellblock_hip_t *A, bml_matrix_ellblock_hip_t *B,
@@ -835,10 +885,21 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
      * mantissa stalls ~1e-3 from idempotency; FP32 iterations bring it far
      * closer for less than the double-precision ones would cost) */
     int bf16 = single && bmlh_block_engine_for(Xs) == BML_HIP_BLOCK_ENGINE_MFMA_BF16;
+    /* the update fused into the product: MFMA engines on a symmetric H,
+     * with the state read after every iteration (so no iteration runs, or
+     * swaps buffers, after convergence); BML_HIP_EB_FUSE=0 turns it off */
+    const char *fe = getenv("BML_HIP_EB_FUSE");
+    const int fuse_ok = sym && o.poll == 1 && !(fe && atoi(fe) == 0);
+    void *x_value0 = X->value;
     for (int it = 0; it < o.maxit; it++) {
         bml_matrix_ellblock_hip_t *cur = single ? Xs : X, *cur2 = single ? X2s : X2;
-        if (!eb_sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, sym, what))
+        if (fuse_ok && bmlh_block_engine_for(cur) != BML_HIP_BLOCK_ENGINE_EXACT) {
+            if (!eb_sp2_iteration_fused(cur, cur2, d_state, d_hist, &o, nocc, what))
+                goto out;
+            eb_swap_buffers(cur, cur2);
+        } else if (!eb_sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, sym, what)) {
             goto out;
+        }
         int last_single = single && it + 1 >= o.maxit - o.refine_iters;
         if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {
             if (!bmlh_ok(bmlhip_d2h(&st, d_state, sizeof st), what))
@@ -867,6 +928,14 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
         goto out;
     if (single && !eb_convert(Xs, X, what))
         goto out;
+    if (X->value != x_value0) {
+        /* the iterate lies in the buffers X started with and X2 holds:
+         * copy it there and give X its own buffers back */
+        bml_copy_ellblock_hip(X, X2);
+        if (bmlh_failed())
+            goto out;
+        eb_swap_buffers(X, X2);
+    }
     {
         bmlhip_status s;
         if (!bmlh_finish_sync(what, &s))

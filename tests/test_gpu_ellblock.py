@@ -252,6 +252,49 @@ def test_sp2_mixed_precision(hip):
         assert np.abs(hip.eb_export_to_dense(Xm) - ref).max() < err
 
 
+def test_sp2_fused_update_matches_separate_add(hip, monkeypatch):
+    """On the MFMA engines SP2 folds its update X <- alpha X + beta X^2 into
+    the product's epilogue (the branch decided first from tr X and
+    sum |X_ij|^2 = tr X^2 for symmetric X).  The same solves with the update
+    as a separate bml_add pass (BML_HIP_EB_FUSE=0) take the same branches
+    while ||X^2 - X||_F > 1e-4 (nearer idempotency both choices keep the
+    trace within rounding of nocc, and rounding picks), the same
+    idempotency history (to 1e-6, or 1e-3 in the single-precision phase,
+    whose sums round differently), and reach the same density matrix to the MFMA
+    tolerance (FP64 1e-9; mixed FP32 1e-5), stopped after an odd number of
+    iterations and run to convergence (the iterate returns to X's own
+    buffers either way)."""
+    H = G.block_lattice_hamiltonian(3, 3, 2, b=32, MB=8, onsite_shift=6.0, coupling=0.05)
+    N = H.N
+    dH = hip.from_blk(H, 32)
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("BML_HIP_EB_FUSE", fuse)
+        for mixed in (False, True):
+            for maxit in (7, 60):
+                X = hip.block_matrix(H.precision, H.bsize, 32)
+                ptr0 = X._s.value
+                r, hist = hip.eb_sp2(dH, X, N / 2, threshold=1e-7, tol=1e-9, maxit=maxit,
+                                     mixed=mixed)
+                assert X._s.value == ptr0
+                out[fuse, mixed, maxit] = (r, hist.copy(), hip.eb_export_to_dense(X))
+                X.free()
+    for mixed in (False, True):
+        for maxit in (7, 60):
+            rf, hf, Df = out["1", mixed, maxit]
+            ru, hu, Du = out["0", mixed, maxit]
+            assert abs(rf.iterations - ru.iterations) <= (0 if maxit == 7 else 1), (mixed, maxit)
+            n = min(rf.iterations, ru.iterations)
+            far = hu[:n, 2] > 1e-4
+            np.testing.assert_array_equal(hf[:n, 3][far], hu[:n, 3][far])  # branches
+            np.testing.assert_allclose(hf[:n, 2][far], hu[:n, 2][far],
+                                       rtol=1e-3 if mixed else 1e-6)  # norms
+            assert np.abs(Df - Du).max() < (1e-5 if mixed else 1e-9), (mixed, maxit)
+            if maxit == 60:
+                assert rf.converged and ru.converged
+                assert abs(rf.trace_x - N / 2) < 1e-7
+
+
 def eb_is_symmetric(hip, dA) -> bool:
     """bmlhip_eb_is_symmetric through the C ABI."""
     import ctypes as C
