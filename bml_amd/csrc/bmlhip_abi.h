@@ -309,7 +309,8 @@ typedef struct bmlhip_eb_args {
                          * X: form the blocks kb >= ib, mirror the others */
     /* fused SP2 update (with sym): C <- alpha X + beta (X^2 thresholded),
      * alpha, beta from d_ab; blocks kept iff max |element| > threshold (the
-     * diagonal block always); |X - X^2|^2 -> scalars[2]; no traces */
+     * diagonal block always); |X - X^2|^2 -> scalars[2]; C's trace and
+     * sum |C_ij|^2 (the next iteration's tr X and tr X^2) -> scalars[0], [1] */
     int fuse;
     const double *d_ab;
 } bmlhip_eb_args;

@@ -735,7 +735,9 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
     typedef __attribute__((address_space(3))) int lds_int;
     lds_int *acol = (lds_int *) ((unsigned char *) kbits + p.acol_off);
     const int nAr = p.An[ib];
-    double alpha = 0.0, beta = 0.0, nrm_l = 0.0;
+    // fused: this lane's |X - X^2|^2, and the next iterate's sum |U_ij|^2 and
+    // trace over the kept blocks (the next iteration's tr X^2 and tr X)
+    double alpha = 0.0, beta = 0.0, nrm_l = 0.0, frob_l = 0.0, trn_l = 0.0;
     if (p.fuse) {
         alpha = p.ab[0];
         beta = p.ab[1];
@@ -850,7 +852,7 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
                 if constexpr (std::is_same<TA, T>::value) staged = jx >= 0 && jx < nst;
             }
             const T *xg = Av + ((size_t) ib * p.MA + (jx >= 0 ? jx : 0)) * B2;
-            double mu = 0.0, dn = 0.0;
+            double mu = 0.0, dn = 0.0, fu = 0.0, tu = 0.0;
 #pragma unroll 1
             for (int q0 = 0; q0 < QE; q0 += CH) {
                 T xv[CH];
@@ -878,10 +880,16 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
                     stg[r * LD + c] = u;
                     mu = fmax(mu, E::mag(u));
                     dn += E::abs2(E::sub(xv[j], y));
+                    fu += E::abs2(u);
+                    if (r == c) tu += E::re(u);
                 }
             }
             nrm_l += kb == ib ? dn : 2.0 * dn;
             kept = kb == ib || wave_max(mu) > p.thr;
+            if (kept && (s < p.MC || (sp >= 0 && sp < p.MC))) {
+                frob_l += kb == ib ? fu : 2.0 * fu;
+                if (kb == ib) trn_l += tu;
+            }
             wave_sync();
         }
         if (lane == 0) {
@@ -1054,18 +1062,28 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
     if (!p.sym) compact_kept<T, BT>(p, ib, nbits, kbits);
     // tx2 lives in lane 0 of the wavefront that formed the diagonal block
     __shared__ double tr2_s[NUM_WAVES];
-    if (p.fuse)
-        for (int off = 32; off > 0; off >>= 1) nrm_l += __shfl_xor(nrm_l, off, 64);
-    if (lane == 0) tr2_s[w] = p.fuse ? nrm_l : E::re(tx2);
-    __syncthreads();
     if (p.fuse) {
-        if (threadIdx.x == 0) {
+        __shared__ double fz_s[3][NUM_WAVES];
+        for (int off = 32; off > 0; off >>= 1) {
+            nrm_l += __shfl_xor(nrm_l, off, 64);
+            frob_l += __shfl_xor(frob_l, off, 64);
+            trn_l += __shfl_xor(trn_l, off, 64);
+        }
+        if (lane == 0) {
+            fz_s[0][w] = trn_l;
+            fz_s[1][w] = frob_l;
+            fz_s[2][w] = nrm_l;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
             double t = 0.0;
-            for (int v = 0; v < NUM_WAVES; v++) t += tr2_s[v];
-            p.rownorm[ib] = t;
+            for (int v = 0; v < NUM_WAVES; v++) t += fz_s[threadIdx.x][v];
+            p.rownorm[(size_t) threadIdx.x * p.NB + ib] = t;
         }
         return;
     }
+    if (lane == 0) tr2_s[w] = E::re(tx2);
+    __syncthreads();
     if (threadIdx.x == 0) {
         if (!p.sym) {
             const int kept = kept_s;
@@ -1635,8 +1653,9 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     p.maxs = (int) maxs;
     p.maxp = (int) maxp;
     const size_t NB = (size_t) a->NB;
-    p.rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 2 * (NB + 1));
-    p.rownorm = p.rowtr;  // fused update: no traces, the rows' norms instead
+    p.rowtr = (double *) bmlhip_scratch(0, sizeof(double) * 3 * (NB + 1));
+    // fused update: the next iterate's traces, then the rows' norms
+    p.rownorm = p.rowtr;
     p.cnt = (int *) bmlhip_scratch(3, sizeof(int) * (NB + 1));
     p.skb = (int *) bmlhip_scratch(4, sizeof(int) * (NB * maxs + 1));
     p.soff = (int *) bmlhip_scratch(5, sizeof(int) * (NB * (maxs + 1) + 1));
@@ -1705,7 +1724,13 @@ int eb_product_typed(const bmlhip_eb_args *a) {
                                     BMLHIP_RS_OVF_DIST | BMLHIP_RS_STORED | BMLHIP_RS_PROF,
                                     a->prec, p.skip)))
         return rc;
-    if (p.fuse) return bmlhip_sum_to_status(p.rownorm, a->NB, 2);
+    if (p.fuse) {
+        // the rows' |X - X^2|^2, and the next iterate's traces for the next
+        // branch decision
+        if ((rc = bmlhip_sum_to_status(p.rownorm + 2 * (size_t) a->NB, a->NB, 2))) return rc;
+        if ((rc = bmlhip_sum_to_status(p.rownorm, a->NB, 0))) return rc;
+        return bmlhip_sum_to_status(p.rownorm + a->NB, a->NB, 1);
+    }
     if (a->kind == 0) {
         if ((rc = bmlhip_sum_to_status(p.rowtr, a->NB, 0))) return rc;
         if ((rc = bmlhip_sum_to_status(p.rowtr + a->NB, a->NB, 1))) return rc;

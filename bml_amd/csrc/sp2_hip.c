@@ -739,13 +739,16 @@ static int eb_sp2_iteration(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hi
 }
 
 /* One iteration with the update fused into the product (MFMA engines, an
- * exactly symmetric X): tr X and tr X^2 = sum |X_ij|^2 come first, so the
- * branch is decided before the product, which then writes
- * alpha X + beta X^2 into X2 and reduces |X - X^2|^2; the caller swaps the
- * two matrices' buffers. */
+ * exactly symmetric X): the branch is decided from tr X and
+ * tr X^2 = sum |X_ij|^2 before the product, which then writes
+ * alpha X + beta X^2 into X2, reduces |X - X^2|^2 and leaves the new
+ * iterate's two traces for the next decision (have_tr: the previous
+ * iteration was fused, so they are in place; otherwise a pre-pass reduces
+ * them from X); the caller swaps the two matrices' buffers. */
 static int eb_sp2_iteration_fused(bml_matrix_ellblock_hip_t *X, bml_matrix_ellblock_hip_t *X2,
                                   bmlhip_sp2_state *d_state, double *d_hist,
-                                  const bml_hip_sp2_options *o, double nocc, const char *what)
+                                  const bml_hip_sp2_options *o, double nocc, int have_tr,
+                                  const char *what)
 {
     bmlhip_eb_args a;
     memset(&a, 0, sizeof a);
@@ -768,9 +771,10 @@ static int eb_sp2_iteration_fused(bml_matrix_ellblock_hip_t *X, bml_matrix_ellbl
     a.sym = 1;
     a.fuse = 1;
     a.d_ab = d_state->ab;
-    return bmlh_ok(bmlhip_eb_trace_frob(X->matrix_precision, X->NB, X->bt, X->MB, X->value,
-                                        X->indexb, X->nnzb, &d_state->done),
-                   what) &&
+    return (have_tr ||
+            bmlh_ok(bmlhip_eb_trace_frob(X->matrix_precision, X->NB, X->bt, X->MB, X->value,
+                                         X->indexb, X->nnzb, &d_state->done),
+                    what)) &&
            bmlh_ok(bmlhip_sp2_decide(d_state, nocc, d_hist, o->maxit), what) &&
            bmlh_ok(bmlhip_eb_product(&a), what) &&
            bmlh_ok(bmlhip_sp2_check(d_state, o->tol, d_hist, o->maxit, 1), what);
@@ -891,14 +895,18 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
     const char *fe = getenv("BML_HIP_EB_FUSE");
     const int fuse_ok = sym && o.poll == 1 && !(fe && atoi(fe) == 0);
     void *x_value0 = X->value;
+    int have_tr = 0; /* the status holds the iterate's traces (a fused iteration left them) */
     for (int it = 0; it < o.maxit; it++) {
         bml_matrix_ellblock_hip_t *cur = single ? Xs : X, *cur2 = single ? X2s : X2;
         if (fuse_ok && bmlh_block_engine_for(cur) != BML_HIP_BLOCK_ENGINE_EXACT) {
-            if (!eb_sp2_iteration_fused(cur, cur2, d_state, d_hist, &o, nocc, what))
+            if (!eb_sp2_iteration_fused(cur, cur2, d_state, d_hist, &o, nocc, have_tr, what))
                 goto out;
             eb_swap_buffers(cur, cur2);
-        } else if (!eb_sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, sym, what)) {
-            goto out;
+            have_tr = 1;
+        } else {
+            if (!eb_sp2_iteration(cur, cur2, d_state, d_hist, &o, nocc, sym, what))
+                goto out;
+            have_tr = 0;
         }
         int last_single = single && it + 1 >= o.maxit - o.refine_iters;
         if ((it + 1) % o.poll == 0 || last_single || it + 1 == o.maxit) {
