@@ -120,12 +120,16 @@ __device__ __forceinline__ unsigned eb_hash(int kb, unsigned mask) {
 // table position, *isnew set when this lane inserted it.  The columns of one
 // chunk are distinct (a row of B holds each block column once), so no two
 // lanes of a chunk insert the same key.
-__device__ __forceinline__ unsigned find_or_insert(int *keys, int kb, unsigned mask, bool *isnew) {
+// With insert false a missing key is not added: the result is NOSLOT.
+constexpr unsigned NOSLOT = ~0u;
+__device__ __forceinline__ unsigned find_or_insert(int *keys, int kb, unsigned mask, bool *isnew,
+                                                   bool insert) {
     unsigned h = eb_hash(kb, mask);
     while (true) {
         const int k = keys[h];
         if (k == kb) return h;
         if (k == EMPTY) {
+            if (!insert) return NOSLOT;
             const int old = atomicCAS(&keys[h], EMPTY, kb);
             if (old == EMPTY) {
                 *isnew = true;
@@ -137,7 +141,12 @@ __device__ __forceinline__ unsigned find_or_insert(int *keys, int kb, unsigned m
     }
 }
 
-__global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
+// cap: slots given out (a row touching more than MC block columns is an
+// overflow whatever the rest; with count >= cap no slot is added, and the
+// pairs of columns without a slot are left out of both passes alike), so
+// the table and counters are sized by MC, not by the NB columns a row
+// could touch
+__global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits, int cap) {
     extern __shared__ int sm[];
     if (p.skip && *p.skip) return;
     const int ib = blockIdx.x;
@@ -145,17 +154,33 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
     const unsigned tsize = 1u << tbits, mask = tsize - 1u;
     int *keys = sm;
     int *sval = sm + tsize;
-    int *fill = sm + 2 * tsize;  // maxs counters
+    int *fill = sm + 2 * tsize;  // cap + 63 counters
+    // the row's block columns jb and the lengths of rows jb of B, loaded
+    // lane-parallel once (not two dependent loads per jp)
+    int *jbs = fill + cap + 64, *nbs = jbs + p.MA;
+    const int nA = p.An[ib];
     for (unsigned s = lane; s < tsize; s += 64) keys[s] = EMPTY;
-    for (int s = lane; s < p.maxs; s += 64) fill[s] = 0;
+    for (int s = lane; s < cap + 64; s += 64) fill[s] = 0;
+    for (int q = lane; q < nA; q += 64) {
+        const int jb = p.Ai[(size_t) ib * p.MA + q];
+        jbs[q] = jb;
+        nbs[q] = p.Bn[jb];
+    }
     __syncthreads();
     const uint64_t ltmask = (1ull << lane) - 1ull;
-    const int nA = p.An[ib];
     int count = 0;
+    // the first chunk of row jbs[jp] of B, lane-wise; the next one is loaded
+    // while the current is hashed
+    auto first_chunk = [&](int jp) {
+        return lane < nbs[jp] ? p.Bi[(size_t) jbs[jp] * p.MB + lane] : -1;
+    };
     // pass 1: slots in first-touch order and pairs per slot
+    int kbn = nA > 0 ? first_chunk(0) : -1;
     for (int jp = 0; jp < nA; jp++) {
-        const int jb = p.Ai[(size_t) ib * p.MA + jp];
-        const int nb = p.Bn[jb];
+        const int jb = jbs[jp];
+        const int nb = nbs[jp];
+        const int kbc = kbn;
+        if (jp + 1 < nA) kbn = first_chunk(jp + 1);
         for (int k0 = 0; k0 < nb; k0 += 64) {
             const int kp = k0 + lane;
             const bool act = kp < nb;
@@ -163,8 +188,8 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
             unsigned h = 0;
             int kb = 0;
             if (act) {
-                kb = p.Bi[(size_t) jb * p.MB + kp];
-                h = find_or_insert(keys, kb, mask, &isnew);
+                kb = k0 == 0 ? kbc : p.Bi[(size_t) jb * p.MB + kp];
+                h = find_or_insert(keys, kb, mask, &isnew, count < cap);
             }
             const uint64_t nm = ballot64(isnew);
             if (isnew) {
@@ -176,7 +201,7 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
             __syncthreads();  // new slot numbers visible to every lane
             // the chunk's slots are distinct; with p.sym the slots row ib
             // does not own get no pairs (their tiles are mirrors)
-            if (act && (!p.sym || sym_owner(ib, kb))) fill[sval[h]] += 1;
+            if (act && h != NOSLOT && (!p.sym || sym_owner(ib, kb))) fill[sval[h]] += 1;
             __syncthreads();
         }
     }
@@ -215,15 +240,19 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits) {
     __syncthreads();
     // pass 2: pairs grouped by slot, jp order kept within a slot
     int *pr = p.pairs + (size_t) ib * p.maxp;
+    kbn = nA > 0 ? first_chunk(0) : -1;
     for (int jp = 0; jp < nA; jp++) {
-        const int jb = p.Ai[(size_t) ib * p.MA + jp];
-        const int nb = p.Bn[jb];
+        const int jb = jbs[jp];
+        const int nb = nbs[jp];
+        const int kbc = kbn;
+        if (jp + 1 < nA) kbn = first_chunk(jp + 1);
         for (int k0 = 0; k0 < nb; k0 += 64) {
             const int kp = k0 + lane;
-            const int kb = kp < nb ? p.Bi[(size_t) jb * p.MB + kp] : -1;
-            if (kp < nb && (!p.sym || sym_owner(ib, kb))) {
-                unsigned h = eb_hash(kb, mask);
-                while (keys[h] != kb) h = (h + 1u) & mask;
+            const int kb = kp < nb ? (k0 == 0 ? kbc : p.Bi[(size_t) jb * p.MB + kp]) : -1;
+            unsigned h = eb_hash(kb, mask);
+            if (kp < nb)
+                while (keys[h] != kb && keys[h] != EMPTY) h = (h + 1u) & mask;
+            if (kp < nb && keys[h] == kb && (!p.sym || sym_owner(ib, kb))) {
                 const int s = sval[h];
                 pr[fill[s]] = (int) (((unsigned) jp << 16) | (unsigned) kp);
                 p.pslot[(size_t) ib * p.maxp + fill[s]] = s;
@@ -1644,9 +1673,11 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     }
     const long long maxp = (long long) a->MA * a->MB;
     const long long maxs = maxp < a->NB ? maxp : a->NB;
+    // slots the symbolic pass gives out: MC + 1 show an overflow
+    const int cap = (int) (maxs < (long long) a->MC + 1 ? maxs : (long long) a->MC + 1);
     int tbits = 6;
-    while ((1ll << tbits) < 2 * maxs) tbits++;
-    const size_t lds = (size_t) (2 * (1ll << tbits) + maxs) * sizeof(int);
+    while ((1ll << tbits) < 2 * ((long long) cap + 64)) tbits++;
+    const size_t lds = (size_t) (2 * (1ll << tbits) + cap + 64 + 2 * (long long) a->MA) * sizeof(int);
     if (lds > 160 * 1024)
         return fail_msg(5, "ellblock product: %lld block pairs per row exceed the LDS slot map",
                         maxp);
@@ -1679,7 +1710,7 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     if (lds > 64 * 1024)
         (void) hipFuncSetAttribute((const void *) k_eb_symbolic,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-    hipLaunchKernelGGL(k_eb_symbolic, dim3(a->NB), dim3(64), lds, s, p, tbits);
+    hipLaunchKernelGGL(k_eb_symbolic, dim3(a->NB), dim3(64), lds, s, p, tbits, cap);
     if ((rc = report("k_eb_symbolic"))) return rc;
     if (std::is_same<T, float>::value && a->bt == 32 && a->engine == 2) {
         // bf16 engine: the tiles in bfloat16 once per product (A row-major,
