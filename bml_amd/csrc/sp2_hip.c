@@ -12,6 +12,7 @@
 #include "bml_hip_host.h"
 #include "bml_hip_tune.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -37,17 +38,20 @@ static int single_of(int prec)
 /* Stall test of the single-precision phase: the idempotency measure read at
  * every iteration no longer falls below its value two iterations earlier
  * (SP2 alternates branches, and a converging iterate shrinks it at least
- * every other step; once rounding dominates, it grows).  Needs poll == 1. */
+ * every other step; once rounding dominates, it grows), or, once it is below
+ * 1e-5 sqrt(nocc) (the quadratic regime, where every step shrinks it), it
+ * grows at all.  Needs poll == 1. */
 typedef struct {
     int seen;
     double prev1, prev2;
 } stall_track;
 
-static int stalled(const bml_hip_sp2_options *o, stall_track *t, double norm)
+static int stalled(const bml_hip_sp2_options *o, stall_track *t, double norm, double nocc)
 {
     if (!o->mixed_switch || o->poll != 1)
         return 0;
-    const int stall = t->seen >= 2 && norm >= t->prev2;
+    const int stall = (t->seen >= 2 && norm >= t->prev2) ||
+                      (t->seen >= 1 && norm >= t->prev1 && t->prev1 < 1e-5 * sqrt(nocc));
     t->prev2 = t->prev1;
     t->prev1 = norm;
     t->seen++;
@@ -303,7 +307,7 @@ void bml_hip_sp2_ellpack(const bml_matrix_ellpack_hip_t *H, bml_matrix_ellpack_h
                 r.single_iterations = st.it;
             if (st.done)
                 break;
-            if (single && stalled(&o, &stall, st.norm))
+            if (single && stalled(&o, &stall, st.norm, nocc))
                 last_single = 1;
         }
         if (last_single) {
@@ -916,7 +920,7 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
                 r.single_iterations = st.it;
             if (st.done)
                 break;
-            if (single && stalled(&o, &stall, st.norm)) {
+            if (single && stalled(&o, &stall, st.norm, nocc)) {
                 if (bf16) {
                     bf16 = 0;
                     bml_hip_set_block_engine(BML_HIP_BLOCK_ENGINE_MFMA);
