@@ -174,6 +174,25 @@ __global__ __launch_bounds__(64) void k_eb_symbolic(EbParams p, int tbits, int c
     auto first_chunk = [&](int jp) {
         return lane < nbs[jp] ? p.Bi[(size_t) jbs[jp] * p.MB + lane] : -1;
     };
+    // fused SP2 update: the row's own block columns take the first slots,
+    // in A's order (every one is reached through its row's diagonal block,
+    // which the fused update keeps); near convergence the kept blocks are
+    // about X's own, so the compaction moves few tiles
+    if (p.fuse)
+        for (int q0 = 0; q0 < nA; q0 += 64) {
+            const int q = q0 + lane;
+            bool isnew = false;
+            unsigned h = NOSLOT;
+            if (q < nA) h = find_or_insert(keys, jbs[q], mask, &isnew, count < cap);
+            const uint64_t nm = ballot64(isnew);
+            if (isnew) {
+                const int r = count + popc64(nm & ltmask);
+                sval[h] = r;
+                p.skb[(size_t) ib * p.maxs + r] = jbs[q];
+            }
+            count += popc64(nm);
+            __syncthreads();
+        }
     // pass 1: slots in first-touch order and pairs per slot
     int kbn = nA > 0 ? first_chunk(0) : -1;
     for (int jp = 0; jp < nA; jp++) {
@@ -1132,34 +1151,73 @@ __global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
 // gathers its kept flags (its own slots and the mirrored ones), compacts the
 // kept tiles into slot order and records its counts.
 template <typename T, int BT>
-__global__ __launch_bounds__(64 * NUM_WAVES) void k_eb_compact(EbParams p) {
-    extern __shared__ __align__(16) unsigned kbits[];
-    __shared__ int kept_s;
+__global__ __launch_bounds__(64) void k_eb_compact(EbParams p) {
+    // one wavefront per block row.  The kept slots below MC are listed with
+    // their ranks (src -> dst, dst < src) in LDS, then the tiles that move
+    // are copied in increasing src order, each tile's loads issued before
+    // the previous tile's stores: a destination lies below its own source
+    // and every later one, so no load reads a slot an earlier store wrote,
+    // and a store waits for its own tile's loads, which return in order
+    // after those of every earlier tile (whose sources it may overwrite).
+    extern __shared__ __align__(16) int mv[];  // pairs (src, dst), at most MC
     if (p.skip && *p.skip) return;
-    const int ib = blockIdx.x;
-    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int ib = blockIdx.x, lane = threadIdx.x;
+    constexpr size_t B2 = (size_t) BT * BT;
+    constexpr int Q = (int) ((B2 + 63) / 64);
     const int count = p.cnt[ib];
     const int nbits = count < p.MC ? count : p.MC;
     const int *fl = p.kflag + (size_t) ib * p.maxs;
-    if (threadIdx.x == 0) kept_s = 0;
-    __syncthreads();
-    for (int c0 = 64 * w; c0 < count; c0 += 64 * NUM_WAVES) {
+    const int *skb = p.skb + (size_t) ib * p.maxs;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    int kept = 0, nmv = 0;
+    for (int c0 = 0; c0 < count; c0 += 64) {
         const int sl = c0 + lane;
-        const uint64_t m = ballot64(sl < count && fl[sl] != 0);
-        if (lane == 0) {
-            atomicAdd(&kept_s, popc64(m));
-            const uint64_t in = c0 + 64 <= nbits ? m
-                                : c0 < nbits ? m & ((1ull << (nbits - c0)) - 1ull) : 0ull;
-            if (c0 < nbits) {
-                kbits[c0 >> 5] = (unsigned) in;
-                if (c0 + 32 < nbits) kbits[(c0 >> 5) + 1] = (unsigned) (in >> 32);
-            }
+        const bool kp = sl < count && fl[sl] != 0;
+        const uint64_t km = ballot64(kp);
+        const int pos = kept + popc64(km & ltmask);
+        const bool m = kp && sl < nbits && pos != sl;
+        const uint64_t mm = ballot64(m);
+        if (m) {
+            const int r = nmv + popc64(mm & ltmask);
+            mv[2 * r] = sl;
+            mv[2 * r + 1] = pos;
         }
+        kept += popc64(km);
+        nmv += popc64(mm);
     }
-    __syncthreads();
-    compact_kept<T, BT>(p, ib, nbits, kbits);
-    if (threadIdx.x == 0) {
-        const int kept = kept_s;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    T *Cv = (T *) p.Cv;
+    T *row = Cv + (size_t) ib * p.MC * B2;
+    auto load = [&](int k, T (&b)[Q]) {
+        const T *S = row + (size_t) mv[2 * k] * B2;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const size_t e = (size_t) lane + 64 * q;
+            if (e < B2) b[q] = S[e];
+        }
+    };
+    auto store = [&](int k, const T (&b)[Q]) {
+        const int src = mv[2 * k], dst = mv[2 * k + 1];
+        T *D = row + (size_t) dst * B2;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const size_t e = (size_t) lane + 64 * q;
+            if (e < B2) D[e] = b[q];
+        }
+        if (lane == 0) p.Ci[(size_t) ib * p.MC + dst] = skb[src];
+    };
+    T b0[Q], b1[Q];
+    if (nmv > 0) load(0, b0);
+    for (int k = 0; k < nmv; k += 2) {
+        if (k + 1 < nmv) load(k + 1, b1);
+        store(k, b0);
+        if (k + 1 >= nmv) break;
+        if (k + 2 < nmv) load(k + 2, b0);
+        store(k + 1, b1);
+    }
+    if (lane == 0) {
         const int *off = p.soff + (size_t) ib * (p.maxs + 1);
         p.Cn[ib] = kept < p.MC ? kept : p.MC;
         p.rowstat[ib] = make_int4(kept, count, off[count], p.An[ib]);
@@ -1632,8 +1690,13 @@ int launch_numeric(const EbParams &p, hipStream_t s) {
         int rc = report("k_eb_numeric");
         if (rc || !p.sym) return rc;
         const int nbits = p.maxs < p.MC ? p.maxs : p.MC;
-        const size_t lc = sizeof(unsigned) * (size_t) ((nbits + 31) / 32 + 1);
-        hipLaunchKernelGGL((k_eb_compact<T, BT>), dim3(p.NB), dim3(64 * NUM_WAVES), lc, s, p);
+        const size_t lc = 2 * sizeof(int) * (size_t) (nbits + 1);  // the move list
+        if (lc > 64 * 1024) {
+            if (lc > 160 * 1024) return fail_msg(5, "ellblock product: %d block slots per row exceed LDS", p.MC);
+            (void) hipFuncSetAttribute((const void *) k_eb_compact<T, BT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int) lc);
+        }
+        hipLaunchKernelGGL((k_eb_compact<T, BT>), dim3(p.NB), dim3(64), lc, s, p);
         return report("k_eb_compact");
     }
 }
