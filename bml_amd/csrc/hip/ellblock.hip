@@ -1735,7 +1735,11 @@ int eb_product_typed(const bmlhip_eb_args *a) {
         p.remap = e ? atoi(e) : 1;
     }
     const long long maxp = (long long) a->MA * a->MB;
-    const long long maxs = maxp < a->NB ? maxp : a->NB;
+    // distinct block columns a row's slots can name: those its pairs touch,
+    // plus (fused update) A's own columns, which take the first slots
+    const int fuse_req = a->kind == 0 && a->sym && a->engine != 0 && a->fuse;
+    const long long named = maxp + (fuse_req ? a->MA : 0);
+    const long long maxs = named < a->NB ? named : a->NB;
     // slots the symbolic pass gives out: MC + 1 show an overflow
     const int cap = (int) (maxs < (long long) a->MC + 1 ? maxs : (long long) a->MC + 1);
     int tbits = 6;
