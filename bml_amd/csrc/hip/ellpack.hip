@@ -1653,6 +1653,81 @@ __global__ __launch_bounds__(256) void k_win_union(int N, int M, const T *Xv, co
     }
 }
 
+// The same union from a bitmap of the N columns (N/8 bytes of LDS, and a
+// word prefix as large, against the hash's 6 * 2^14 bytes at M = 3072, which
+// leaves one block per CU): each entry sets its column's bit, and a block
+// scan of the words' bit counts ranks the columns, so U comes out in column
+// order.  For N <= 2^16.  The order U comes in matters to the product: on
+// the 32^3 SP2 step (one MI355X) the FP32 step took 55.6 ms with this
+// union against 56.4 with the hash's, but the FP64 product 58.8 against
+// 54.1 ms (and 64.4 ms with U dealt round-robin over the groups, each group
+// sampling the whole column range), so the FP64 and complex products keep
+// the hash's order.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_win_union_bm(int N, int M, const T *Xv, const int *Xi,
+                                                      const int *Xn, int *Uj, T *Uc, int *Un,
+                                                      size_t ustride, const int *skip,
+                                                      const int *row_sys, const int *sys_done) {
+    extern __shared__ __align__(16) unsigned bmw[];  // NW bit words, then NW word ranks
+    __shared__ int part_s[256];
+    if (skip && *skip) return;  // uniform
+    const int NW = (N + 31) >> 5;
+    int *wr = (int *) (bmw + NW);
+    const int b = blockIdx.x, i0 = b * R, tid = threadIdx.x;
+    for (int t = tid; t < NW; t += 256) bmw[t] = 0u;
+    __syncthreads();
+    int nr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = i0 + r;
+        nr[r] = (i < N && !(row_sys && sys_done[row_sys[i]])) ? Xn[i] : 0;
+        for (int e = tid; e < nr[r]; e += 256) {
+            const int j = Xi[(size_t) i * M + e];
+            atomicOr(&bmw[j >> 5], 1u << (j & 31));
+        }
+    }
+    __syncthreads();
+    // each thread ranks a contiguous run of words; a block scan offsets the runs
+    const int per = (NW + 255) / 256, w0 = tid * per;
+    const int w1 = w0 + per < NW ? w0 + per : NW;
+    int cnt = 0;
+    for (int t = w0; t < w1; t++) cnt += __popc(bmw[t]);
+    part_s[tid] = cnt;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const int v = tid >= d ? part_s[tid - d] : 0;
+        __syncthreads();
+        part_s[tid] += v;
+        __syncthreads();
+    }
+    int u = part_s[tid] - cnt;
+    const int total = part_s[255];
+    int *uj = Uj + (size_t) b * ustride;
+    T *uc = Uc + (size_t) b * ustride * R;
+    for (int t = w0; t < w1; t++) {
+        wr[t] = u;
+        unsigned m = bmw[t];
+        while (m) {
+            const int bit = __ffs(m) - 1;
+            m &= m - 1u;
+            uj[u] = (t << 5) + bit;
+            u++;
+        }
+    }
+    for (int k = tid; k < total * R; k += 256) uc[k] = Elem<T>::zero();
+    if (tid == 0) Un[b] = total;
+    __syncthreads();  // ranks in LDS; the zeroed coefficients land before they are set
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const size_t row = (size_t) (i0 + r) * M;
+        for (int e = tid; e < nr[r]; e += 256) {
+            const int j = Xi[row + e];
+            const unsigned below = bmw[j >> 5] & ((1u << (j & 31)) - 1u);
+            uc[(size_t) (wr[j >> 5] + __popc(below)) * R + r] = Xv[row + e];
+        }
+    }
+}
+
 // Fixed-point window sums (row blocks).  A row block sums its products as
 // 64-bit integers in units of 2^-E: integer adds are associative, so the sums
 // do not depend on the order the wavefronts' atomics arrive in, and an LDS
@@ -3063,6 +3138,27 @@ int win_union(int &rows, int N, int M, int wbits, int waves, const void *Xv, con
     wp.Un = Un;
     wp.ustride = ustride;
     if (N == 0) return 0;
+    // BML_HIP_WIN_UNION=0 or 1 picks the hash or the bitmap union for every
+    // element type; by default 4-byte elements take the bitmap's
+    static int umode = -2;
+    if (umode == -2) {
+        const char *e = getenv("BML_HIP_WIN_UNION");
+        umode = e ? atoi(e) : -1;
+    }
+    const bool bitmap = umode < 0 ? sizeof(T) == 4 : umode > 0;
+    if (N <= (1 << 16) && bitmap) {
+        // bitmap union: N/8 bytes of bits and N/8 of word ranks
+        const size_t lb = (size_t) 8 * ((N + 31) / 32);
+        if (rows == 4)
+            hipLaunchKernelGGL((k_win_union_bm<T, 4>), dim3(nblk), dim3(256), lb, s, N, M,
+                               (const T *) Xv, Xi, Xn, Uj, Uc, Un, ustride, skip, row_sys,
+                               sys_done);
+        else
+            hipLaunchKernelGGL((k_win_union_bm<T, 2>), dim3(nblk), dim3(256), lb, s, N, M,
+                               (const T *) Xv, Xi, Xn, Uj, Uc, Un, ustride, skip, row_sys,
+                               sys_done);
+        return report("k_win_union_bm");
+    }
     if (rows == 4) {
         if (lds > 64 * 1024)
             (void) hipFuncSetAttribute((const void *) k_win_union<T, 4>,
