@@ -45,6 +45,9 @@ constexpr int EMPTY = -1;
 #ifndef BMLHIP_EB_WAVES
 #define BMLHIP_EB_WAVES 4
 #endif
+#ifndef BMLHIP_EB_OCC_BF16
+#define BMLHIP_EB_OCC_BF16 3
+#endif
 #ifndef BMLHIP_EB_BATCH
 #define BMLHIP_EB_BATCH 16
 #endif
@@ -745,8 +748,12 @@ __device__ __forceinline__ int mirror_slot(const EbParams &p, const MirrorProbe 
     return -1;
 }
 
+// blocks per CU the registers are held to: 3 for the bf16 engine (its
+// 48 KB LDS share leaves room for 3; FP32 would spill 94 registers), else 2
+template <typename T, int ENGINE> struct EbOcc { static constexpr int value = 2; };
+template <> struct EbOcc<float, 2> { static constexpr int value = BMLHIP_EB_OCC_BF16; };
 template <typename T, int ENGINE, int BT>
-__global__ __launch_bounds__(64 * NUM_WAVES, 2) void k_eb_numeric(EbParams p) {
+__global__ __launch_bounds__(64 * NUM_WAVES, (EbOcc<T, ENGINE>::value)) void k_eb_numeric(EbParams p) {
     using E = Elem<T>;
     using Acc = typename AccOf<T, ENGINE, BT>::type;
     // dynamic LDS: the kept flag of each slot s < min(maxs, MC), then (sym)
