@@ -888,11 +888,12 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
     bmlhip_sp2_state st;
     memset(&st, 0, sizeof st);
     stall_track stall = { 0, 0.0, 0.0 };
-    /* bf16 inputs: once the bf16 iterate stalls, the single-precision phase
-     * goes on with FP32 MFMA until that stalls too, then double (bf16's 8-bit
-     * mantissa stalls ~1e-3 from idempotency; FP32 iterations bring it far
-     * closer for less than the double-precision ones would cost) */
+    /* bf16 inputs: once the bf16 iterate stalls, or nears the bf16 rounding
+     * floor, the single-precision phase goes on with FP32 MFMA until that
+     * stalls too, then double (FP32 iterations bring it far closer to
+     * idempotency for less than the double-precision ones would cost) */
     int bf16 = single && bmlh_block_engine_for(Xs) == BML_HIP_BLOCK_ENGINE_MFMA_BF16;
+    double recover = 0.0; /* the measure when bf16 gave way to FP32 (0: none pending) */
     /* the update fused into the product: MFMA engines on a symmetric H,
      * with the state read after every iteration (so no iteration runs, or
      * swaps buffers, after convergence); BML_HIP_EB_FUSE=0 turns it off */
@@ -920,11 +921,25 @@ void bml_hip_sp2_ellblock(const bml_matrix_ellblock_hip_t *H, bml_matrix_ellbloc
                 r.single_iterations = st.it;
             if (st.done)
                 break;
-            if (single && stalled(&o, &stall, st.norm, nocc)) {
+            /* bf16 inputs round each entry to 2^-8 of itself, which leaves
+             * X^2 about 8 * 2^-8 * sqrt(nocc) from idempotent in the
+             * Frobenius norm (config 3: the bf16 iterate diverges once the
+             * measure is below ~3 = 4 * 2^-8 * sqrt(32768), and FP32 then
+             * spends iterations climbing back): below that level the phase
+             * goes on in FP32 */
+            const int bf16_floor = bf16 && o.mixed_switch && st.norm < 8.0 * 0x1p-8 * sqrt(nocc);
+            /* after the switch to FP32 the measure first climbs while FP32
+             * corrects the bf16 iterate; the FP32 stall test waits until it
+             * is back below its value at the switch */
+            const int armed = !(recover > 0.0 && st.norm >= recover);
+            if (armed)
+                recover = 0.0;
+            if (single && (bf16_floor || (armed && stalled(&o, &stall, st.norm, nocc)))) {
                 if (bf16) {
                     bf16 = 0;
                     bml_hip_set_block_engine(BML_HIP_BLOCK_ENGINE_MFMA);
                     stall.seen = 0;
+                    recover = st.norm;
                 } else {
                     last_single = 1;
                 }
