@@ -3138,13 +3138,11 @@ int win_union(int &rows, int N, int M, int wbits, int waves, const void *Xv, con
     wp.Un = Un;
     wp.ustride = ustride;
     if (N == 0) return 0;
-    // BML_HIP_WIN_UNION=0 or 1 picks the hash or the bitmap union for every
-    // element type; by default 4-byte elements take the bitmap's
-    static int umode = -2;
-    if (umode == -2) {
-        const char *e = getenv("BML_HIP_WIN_UNION");
-        umode = e ? atoi(e) : -1;
-    }
+    // BML_HIP_WIN_UNION=0 or 1 (read per call) picks the hash or the bitmap
+    // union for every element type; by default 4-byte elements take the
+    // bitmap's
+    const char *ue = getenv("BML_HIP_WIN_UNION");
+    const int umode = ue ? atoi(ue) : -1;
     const bool bitmap = umode < 0 ? sizeof(T) == 4 : umode > 0;
     if (N <= (1 << 16) && bitmap) {
         // bitmap union: N/8 bytes of bits and N/8 of word ranks

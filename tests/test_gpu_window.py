@@ -86,6 +86,35 @@ def test_row_block_sums_are_order_independent(window, monkeypatch, rows_per_bloc
     assert_ell_close(outs[0][0], ref, vtol(dtype), 1e-3, "fixed-point x2")
 
 
+@pytest.mark.parametrize("rows_per_block", [2, 4])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_union_builders_give_the_same_bits(window, monkeypatch, rows_per_block, dtype):
+    """The row blocks' column union comes from an LDS hash (its slot order)
+    or, for N <= 2^16, from a bitmap of the columns (column order).  The
+    fixed-point sums do not depend on the order the union lists the
+    streamed rows in, so both builders give the same bits, on a product
+    whose rows span several windows and on one with empty rows."""
+    hip = window
+    monkeypatch.setenv("BML_HIP_WIN_ROWS", str(rows_per_block))
+    for N, M, fill, seed in ((5000, 192, 150, 17), (700, 96, 60, 18)):
+        A = G.random_ell(N, M, fill, seed, dtype)
+        A.nnz[::7] = 0  # empty rows
+        dA = hip.from_ell(A)
+        outs = {}
+        for mode in ("0", "1"):
+            monkeypatch.setenv("BML_HIP_WIN_UNION", mode)
+            X2 = hip.zero_matrix(A.precision, N, N)
+            tr = hip.multiply_x2(dA, X2, 1e-3)
+            outs[mode] = (X2.get(), tuple(tr))
+        (g0, t0), (g1, t1) = outs["0"], outs["1"]
+        assert t0 == t1
+        np.testing.assert_array_equal(g0.nnz, g1.nnz)
+        np.testing.assert_array_equal(g0.index, g1.index)
+        assert g0.value.tobytes() == g1.value.tobytes()
+        ref, _ = orc.x2(A, N, 1e-3)
+        assert_ell_close(g1, ref, vtol(dtype), 1e-3, f"bitmap union x2 N={N}")
+
+
 @pytest.mark.parametrize("rows,wbits,waves", [(1, 10, 8), (1, 11, 8), (1, 12, 8), (1, 13, 8),
                                                (1, 10, 4), (1, 12, 4), (2, 10, 8), (2, 11, 8),
                                                (4, 10, 8), (4, 11, 8), (4, 12, 8)])
