@@ -295,6 +295,32 @@ def test_sp2_fused_update_matches_separate_add(hip, monkeypatch):
                 assert abs(rf.trace_x - N / 2) < 1e-7
 
 
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_sp2_padded_blocks_on_mfma(hip, monkeypatch, fuse):
+    """24-orbital blocks in 32 x 32 tiles (the padding is zero): SP2 on the
+    FP64 MFMA engine, with the update in the product's epilogue and as a
+    separate add, reaches the projector onto the lowest nocc eigenvectors
+    (numpy eigh) at threshold 0; the fused epilogue's keep rule, norm and
+    traces see the padded elements as zeros."""
+    monkeypatch.setenv("BML_HIP_EB_FUSE", fuse)
+    H = G.block_lattice_hamiltonian(2, 2, 2, b=24, MB=8, coupling=0.3)
+    D = H.to_dense()
+    N = len(D)
+    w, V = np.linalg.eigh(D)
+    P = V[:, : N // 2] @ V[:, : N // 2].T
+    hip.set_block_engine(hip.BLOCK_ENGINE_MFMA)
+    try:
+        dH = hip.from_blk(H)
+        X = hip.block_matrix(H.precision, H.bsize, 8)
+        r, hist = hip.eb_sp2(dH, X, N / 2, threshold=0.0, tol=1e-9, maxit=80)
+    finally:
+        hip.set_block_engine(hip.BLOCK_ENGINE_AUTO)
+    assert X.bt == 32
+    assert r.converged and r.idempotency <= 1e-9, r.idempotency
+    assert np.abs(hip.eb_export_to_dense(X) - P).max() < 1e-8
+    assert abs(hip.eb_trace(X) - N / 2) < 1e-8
+
+
 def eb_is_symmetric(hip, dA) -> bool:
     """bmlhip_eb_is_symmetric through the C ABI."""
     import ctypes as C
