@@ -1551,6 +1551,28 @@ __global__ __launch_bounds__(256) void k_eb_trace_frob(int NB, int MB, const T *
     }
 }
 
+// the fused SP2 product's three per-row sums (next tr X, next tr X^2, and
+// |X - X^2|^2) into status scalars 0, 1, 2: one block of 1024 threads, each
+// sum over a fixed partition and a fixed tree (one launch, not three)
+__global__ __launch_bounds__(1024) void k_eb_sum3(const double *x, int n, bmlhip_status *st,
+                                                  const int *skip) {
+    __shared__ double part[1024];
+    if (skip && *skip) return;  // uniform
+    for (int a = 0; a < 3; a++) {
+        const double *v = x + (size_t) a * n;
+        double s = 0.0;
+        for (int k = threadIdx.x; k < n; k += 1024) s += v[k];
+        part[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 512; w > 0; w >>= 1) {
+            if ((int) threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) st->scalars[a] = part[0];
+        __syncthreads();
+    }
+}
+
 // copy block rows between matrices of different MB
 template <typename T, int BT>
 __global__ __launch_bounds__(256) void k_eb_copy(int NB, int MA, const T *Av, const int *Ai,
@@ -1835,9 +1857,8 @@ int eb_product_typed(const bmlhip_eb_args *a) {
     if (p.fuse) {
         // the rows' |X - X^2|^2, and the next iterate's traces for the next
         // branch decision
-        if ((rc = bmlhip_sum_to_status(p.rownorm + 2 * (size_t) a->NB, a->NB, 2))) return rc;
-        if ((rc = bmlhip_sum_to_status(p.rownorm, a->NB, 0))) return rc;
-        return bmlhip_sum_to_status(p.rownorm + a->NB, a->NB, 1);
+        hipLaunchKernelGGL(k_eb_sum3, dim3(1), dim3(1024), 0, s, p.rownorm, a->NB, p.st, p.skip);
+        return report("k_eb_sum3");
     }
     if (a->kind == 0) {
         if ((rc = bmlhip_sum_to_status(p.rowtr, a->NB, 0))) return rc;
