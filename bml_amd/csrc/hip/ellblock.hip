@@ -490,25 +490,38 @@ struct MfmaAccBF16 : MfmaAccF32 {
 // written (no pair reads them).
 __global__ __launch_bounds__(256) void k_eb_bf16(int M, const float *V, const int *nnzb,
                                                  __bf16 *rm, __bf16 *tr, const int *skip) {
-    __shared__ float t[32][33];
+    // each wavefront converts tiles w, w + 4, ... of the row on its own: a
+    // lane's 16 loads of a tile issue together, and the transpose goes
+    // through the wavefront's own LDS tile (no block barrier per tile)
+    __shared__ float t[4][32][33];
     if (skip && *skip) return;  // uniform
     const int ib = blockIdx.x;
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int n = nnzb[ib];
-    for (int jp = 0; jp < n; jp++) {
+    for (int jp = w; jp < n; jp += 4) {
         const size_t base = ((size_t) ib * M + jp) * 1024;
-        float v[4];  // the four loads in flight together
+        float v[16];
 #pragma unroll
-        for (int q = 0; q < 4; q++) v[q] = V[base + threadIdx.x + 256 * q];
+        for (int q = 0; q < 16; q++) v[q] = V[base + lane + 64 * q];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int e = threadIdx.x + 256 * q;
+        for (int q = 0; q < 16; q++) {
+            const int e = lane + 64 * q;
             if (rm) rm[base + e] = (__bf16) v[q];
-            t[e >> 5][e & 31] = v[q];
+            t[w][e >> 5][e & 31] = v[q];
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (tr)
-            for (int e = threadIdx.x; e < 1024; e += 256) tr[base + e] = (__bf16) t[e & 31][e >> 5];
-        __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int e = lane + 64 * q;
+                tr[base + e] = (__bf16) t[w][e & 31][e >> 5];
+            }
+        // the tile is read before the wavefront's next one overwrites it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
