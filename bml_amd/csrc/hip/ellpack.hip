@@ -1831,7 +1831,8 @@ __global__ __launch_bounds__(64 * W, WinbOcc<T>::value) void k_spgemm_winb(WinPa
     __shared__ double nrm_s[W][R];
     __shared__ T trx_s[R], d2_s[R];
     if (p.skip && *p.skip) return;  // uniform
-    const int b = blockIdx.x, i0 = b * R;
+    // with p.remap each XCD takes a contiguous range of row blocks
+    const int b = p.remap ? xcd_block(blockIdx.x, gridDim.x) : (int) blockIdx.x, i0 = b * R;
     if (i0 >= p.N) return;
     const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
     const int dummy = R * WC + lane;
